@@ -1,0 +1,151 @@
+/*
+ * magtense_hip.h - C ABI of the MI355X (gfx950) micromagnetic hot path.
+ *
+ * Plain C types only (no HIP or torch types in the signatures).  Every
+ * function returns 0 on success or a negative status; mt_last_error() returns
+ * the message of the last failure on the calling thread.  "device pointer"
+ * arguments are HIP device addresses (for example torch.Tensor.data_ptr() of a
+ * cuda tensor); "host pointer" arguments are ordinary host memory.
+ *
+ * Reference interfaces replaced (paths relative to the MagTense tree):
+ *   - f2py entry RunMicroMagSimulation
+ *       python/src/magtense/lib/FortranToPythonIO.f90:429-489
+ *     -> mt_run_micromag / mt_solver_* (batched: n_states independent runs)
+ *   - CUDA plug-in C ABI
+ *       source/MagTenseFortranCuda/cuda/MagTenseCudaBlasICLWrapper.cxx:13-59
+ *       (bound from Fortran in source/MagTenseMicroMag/FortranCuda.f90:15-60)
+ *     -> icl_initDemagMatrices, icl_MVMult_GetH, icl_destroy (same signatures)
+ *   - prism tensor getNFromTiles / ComputeDemagfieldTensor
+ *       python/src/magtense/lib/FortranToPythonIO.f90:25-93,
+ *       source/MagTenseMicroMag/LandauLifshitzEquationSolver.f90:883-1164
+ *     -> mt_prism_tensor
+ *   - demag product updateDemagfield / cu_MVMult_GetH
+ *       LandauLifshitzEquationSolver.f90:558-779, MagTenseCudaBlas.cu:367-407
+ *     -> mt_demag_apply_f32, mt_demag_apply_bf16x2
+ *   - right-hand side dmdt_fct  LandauLifshitzEquationSolver.f90:220-273
+ *     -> mt_ll_rhs
+ */
+#ifndef MAGTENSE_HIP_H
+#define MAGTENSE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MT_PREC_FP32 0   /* plain per-state FP32 mat-vec (reference semantics) */
+#define MT_PREC_BF16X2 1 /* split BF16 hi+lo tensor, batched MFMA GEMM, FP32 accumulation */
+
+/* ---- library --------------------------------------------------------------- */
+const char* mt_last_error(void);
+int mt_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int mt_device_count(void);
+
+/* ---- (a) prism tensor ----------------------------------------------------------
+ * K = -N_standard (MagTense sign) of prism j (centre src[3j..], full edges
+ * abc[abc_stride*j ..]) at point i (pts[3i..]); all arrays are device pointers,
+ * row-major.  Outputs (any may be null):
+ *   out32[c*n_pts*n_src + i*n_src + j]  FP32, c = xx,xy,xz,yy,yz,zz
+ *   out64                               same layout, FP64
+ *   split                               fragment-tiled BF16 hi/lo (square only,
+ *                                       mt_split_tensor_bytes(n) bytes)
+ * Replaces getN_prism_3D over all pairs (TileNComponents.f90:629-724 and
+ * LandauLifshitzEquationSolver.f90:935-984 / 1044-1133). */
+int mt_prism_tensor(const double* src, const double* abc, int abc_stride, const double* pts, int n_src, int n_pts,
+                    float* out32, double* out64, void* split);
+long long mt_split_tensor_bytes(int n);
+long long mt_packed_m_bytes(int n);
+/* FP32 receiver-major tensor (device) -> split BF16 fragment layout (device). */
+int mt_split_tensor(const float* K32, int n, void* split);
+
+/* ---- (b) demag product -----------------------------------------------------------
+ * h[s*3n + c*n + i] = pref * sum_j K_c(i,j) m[s*3n + c'*n + j] for each of
+ * nstates states (SoA per state, device pointers).  pref = -1 reproduces
+ * the reference's alpha = -1 (MagTenseCudaBlas.cu:367-407). */
+int mt_demag_apply_f32(const float* K32, int n, const float* m, float* h, int nstates, float pref);
+/* Same product from the split tensor: packs m (FP32, device) into BF16 hi/lo
+ * fragments internally; nstates <= 64 per call (larger batches are chunked). */
+int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, int nstates, float pref);
+
+/* ---- (c) right-hand side ------------------------------------------------------------
+ * One dm/dt evaluation per state on a uniform grid (device pointers):
+ *   m, dmdt: nstates x 3n doubles (SoA per state); ms, k0: n doubles (host);
+ *   u_ea: n*3 doubles (host, may be null); hext: nstates*3 doubles (host, A/m);
+ *   K32: FP32 tensor (device).  fields (device, may be null): nstates x 12n
+ *   doubles = H_exc, H_ext, H_dem, H_ani (negated, SoA). */
+int mt_ll_rhs(int nx, int ny, int nz, double Lx, double Ly, double Lz, const double* ms, const double* k0,
+              const double* u_ea, double A0, double gamma, double alpha, const float* K32, const double* m,
+              const double* hext, int nstates, double* dmdt, double* fields);
+
+/* ---- (d) batched solver ------------------------------------------------------------
+ * Mirrors RunMicroMagSimulation for a uniform grid, for n_states independent
+ * magnetization states sharing the grid, materials and tensor.  All pointers
+ * are host pointers. */
+typedef struct {
+  int nx, ny, nz;
+  double Lx, Ly, Lz;
+  int n_states;
+  const double* Ms;      /* n cells */
+  const double* K0;      /* n cells, may be null (0) */
+  const double* u_ea;    /* n*3 row-major, may be null */
+  double A0, gamma, alpha0;
+  int nt_alpha;
+  const double* alphat;  /* nt_alpha*2 row-major [t, alpha] (used when alpha0 == 0) */
+  int solver;            /* 1 explicit (field rows in sequence), 2 dynamic */
+  int nt_hext;
+  const double* hext;    /* nt_hext*4 row-major [t, hx, hy, hz] (A/m); per state if hext_per_state */
+  int hext_per_state;
+  int nt;
+  const double* t;       /* output times, t[0] = start */
+  const double* m0;      /* 3n SoA (per state if m0_per_state) */
+  int m0_per_state;
+  double tol, thres;
+  int nt_conv;
+  const double* t_conv;
+  double conv_tol;
+  int precision;         /* MT_PREC_FP32 or MT_PREC_BF16X2 */
+  int return_fields;     /* compute H_exc/H_ext/H_dem/H_ani at every output time */
+  long long max_steps;   /* > 0: pause every state after this many accepted steps */
+  const float* K_in;     /* optional host FP32 tensor, 6*n*n receiver-major (skips assembly) */
+  float* K_out;          /* optional host buffer receiving the FP32 tensor */
+  int keep_fp32_tensor;  /* keep an FP32 copy beside the split tensor */
+  int ksplit;            /* 0: automatic k-split of the MFMA GEMM */
+} mt_micromag_problem;
+
+typedef struct {
+  long long f_calls, steps, bad_steps, flag3, flag4;
+  double h_last, t_reached;
+  long long evals, ticks, demag_passes;
+  double demag_ms;
+} mt_stats;
+
+typedef struct mt_solver mt_solver;
+
+int mt_solver_create(const mt_micromag_problem* problem, mt_solver** out);
+/* Advance every state by `steps` accepted steps (0: run to the end). */
+int mt_solver_advance(mt_solver* s, long long steps);
+int mt_solver_run(mt_solver* s);
+/* t_out: n_states*n_h*nt; y_out: n_states*n_h*nt*3n; fields: 4x that (H_exc,
+ * H_ext, H_dem, H_ani interleaved per output as 12n); stats: n_states entries.
+ * n_h = nt_hext for the explicit solver, 1 otherwise.  Any pointer may be null. */
+int mt_solver_results(mt_solver* s, double* t_out, double* y_out, double* fields, mt_stats* stats);
+int mt_solver_state(mt_solver* s, double* y);
+int mt_solver_set_timing(mt_solver* s, int on);
+void mt_solver_destroy(mt_solver* s);
+int mt_run_micromag(const mt_micromag_problem* problem, double* t_out, double* y_out, double* fields,
+                    mt_stats* stats);
+
+/* ---- reference plug-in ABI (MagTenseCudaBlasICLWrapper.cxx:13-59) -------------------
+ * Host arrays, column-major N x N FP32 blocks, scalars by reference, one
+ * global instance - exactly what FortranCuda.f90 binds. */
+void icl_initDemagMatrices(const float* Kxx, const float* Kxy, const float* Kxz, const float* Kyy, const float* Kyz,
+                           const float* Kzz, int* n);
+void icl_MVMult_GetH(const float* Mx, const float* My, const float* Mz, float* Hx, float* Hy, float* Hz, int* n,
+                     float* pref);
+void icl_destroy(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAGTENSE_HIP_H */

@@ -1,0 +1,339 @@
+// C ABI: library utilities, the stand-alone kernel entry points and the
+// reference plug-in ABI (icl_*, MagTenseCudaBlasICLWrapper.cxx:13-59).
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include <hip/hip_bf16.h>
+
+#include "mt_common.h"
+#include "mt_kernels.h"
+#include "../../include/magtense_hip.h"
+
+namespace mt {
+
+namespace {
+thread_local std::string g_error;
+
+__device__ inline uint16_t bf16_bits(float v) {
+  __hip_bfloat16 h = __float2bfloat16(v);
+  return *reinterpret_cast<uint16_t*>(&h);
+}
+__device__ inline float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// FP32 states [s][3n] -> split BF16 B-fragments (columns s < nstates <= 64)
+__global__ void pack_bf16_kernel(const float* __restrict__ m, int n, int nstates, uint8_t* __restrict__ packed) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)nstates * 3 * n;
+  if (t >= total) return;
+  int s = (int)(t / (3 * (int64_t)n));
+  int e = (int)(t % (3 * (int64_t)n));
+  int c = e / n, j = e % n;
+  float f = m[t];
+  uint16_t hi = bf16_bits(f);
+  uint16_t lo = bf16_bits(f - bf16_value(hi));
+  int J = j / kTileK, half = (j % kTileK) / 8;
+  int lane = (s & 31) + 32 * half;
+  int sb = s / 32;
+  int64_t frag = (((int64_t)J * 3 + c) * 2 + sb) * 2;
+  uint16_t* p = reinterpret_cast<uint16_t*>(packed + frag * kFragBytes + lane * 16) + (j % 8);
+  p[0] = hi;
+  p[kFragBytes / 2] = lo;
+}
+
+// h[t] = pref * sum_k part[k*stride + t]
+__global__ void combine_partials_kernel(const float* __restrict__ part, int ksplit, int64_t stride, int64_t total,
+                                        float pref, float* __restrict__ h) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  float s = 0.0f;
+  for (int k = 0; k < ksplit; ++k) s += part[k * stride + t];
+  h[t] = pref * s;
+}
+
+// column-major planes T[c][i + j*n] -> receiver-major K[c][i*n + j]
+__global__ void transpose_kernel(const float* __restrict__ T, int n, float* __restrict__ K) {
+  __shared__ float tile[32][33];
+  int c = blockIdx.z;
+  const float* src = T + (size_t)c * n * n;
+  float* dst = K + (size_t)c * n * n;
+  int bi = blockIdx.x * 32, bj = blockIdx.y * 32;
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    int i = bi + threadIdx.x, j = bj + r;
+    if (i < n && j < n) tile[r][threadIdx.x] = src[(size_t)j * n + i];
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    int i = bi + r, j = bj + threadIdx.x;
+    if (i < n && j < n) dst[(size_t)i * n + j] = tile[threadIdx.x][r];
+  }
+}
+
+__global__ void scale_kernel(float* __restrict__ h, int64_t total, float pref) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < total) h[t] *= pref;
+}
+
+int device_ok() {
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) {
+    set_error("no HIP device visible: the MI355X backend needs a GPU");
+    return kErrHip;
+  }
+  return kOk;
+}
+
+}  // namespace
+
+void set_error(const std::string& msg) { g_error = msg; }
+const char* last_error() { return g_error.c_str(); }
+
+int demag_apply_f32(const float* K32, int n, const float* m, float* h, int nstates, float pref, hipStream_t st) {
+  int rc = launch_matvec_f32(K32, n, m, (int64_t)3 * n, h, (int64_t)3 * n, nstates, st);
+  if (rc) return rc;
+  if (pref != 1.0f) {
+    int64_t total = (int64_t)nstates * 3 * n;
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, h, total, pref);
+    MT_HIP(hipGetLastError());
+  }
+  return kOk;
+}
+
+}  // namespace mt
+
+using namespace mt;
+
+extern "C" {
+
+const char* mt_last_error(void) { return mt::last_error(); }
+int mt_version(void) { return 100; }
+int mt_device_count(void) {
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess) return 0;
+  return nd;
+}
+long long mt_split_tensor_bytes(int n) { return split_tensor_bytes(n); }
+long long mt_packed_m_bytes(int n) { return packed_m_bytes(n); }
+
+int mt_prism_tensor(const double* src, const double* abc, int abc_stride, const double* pts, int n_src, int n_pts,
+                    float* out32, double* out64, void* split) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n_src > 0 && n_pts > 0 && src && abc && pts, "invalid tensor arguments");
+  MT_CHECK(abc_stride == 0 || abc_stride == 3, "abc_stride must be 0 (one size for all) or 3");
+  rc = launch_assemble(src, abc, abc_stride, pts, n_src, n_pts, out32, out64, (uint8_t*)split, nullptr);
+  if (rc) return rc;
+  MT_HIP(hipStreamSynchronize(nullptr));
+  return kOk;
+}
+
+int mt_split_tensor(const float* K32, int n, void* split) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n > 0 && K32 && split, "invalid split arguments");
+  rc = launch_split(K32, n, (uint8_t*)split, nullptr);
+  if (rc) return rc;
+  MT_HIP(hipStreamSynchronize(nullptr));
+  return kOk;
+}
+
+int mt_demag_apply_f32(const float* K32, int n, const float* m, float* h, int nstates, float pref) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n > 0 && nstates > 0 && K32 && m && h, "invalid demag arguments");
+  rc = demag_apply_f32(K32, n, m, h, nstates, pref, nullptr);
+  if (rc) return rc;
+  MT_HIP(hipStreamSynchronize(nullptr));
+  return kOk;
+}
+
+int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, int nstates, float pref) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n > 0 && nstates > 0 && split && m && h, "invalid demag arguments");
+  const int64_t n3 = (int64_t)3 * n;
+  int ks = gemm_ksplit_for(n, 4);
+  uint8_t* packed = nullptr;
+  float* part = nullptr;
+  MT_HIP(hipMalloc(&packed, (size_t)packed_m_bytes(n)));
+  int cols = nstates < kMaxCols ? nstates : kMaxCols;
+  MT_HIP(hipMalloc(&part, (size_t)ks * cols * n3 * sizeof(float)));
+  for (int s0 = 0; s0 < nstates && rc == kOk; s0 += kMaxCols) {
+    int S = nstates - s0 < kMaxCols ? nstates - s0 : kMaxCols;
+    MT_HIP(hipMemsetAsync(packed, 0, (size_t)packed_m_bytes(n), nullptr));
+    int64_t tot = (int64_t)S * n3;
+    hipLaunchKernelGGL(pack_bf16_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr, m + s0 * n3, n,
+                       S, packed);
+    MT_HIP(hipGetLastError());
+    rc = launch_gemm_bf16x2((const uint8_t*)split, packed, n, part, n3, (int64_t)cols * n3, S, ks, nullptr);
+    if (rc) break;
+    hipLaunchKernelGGL(combine_partials_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr, part, ks,
+                       (int64_t)cols * n3, tot, pref, h + s0 * n3);
+    MT_HIP(hipGetLastError());
+  }
+  MT_HIP(hipStreamSynchronize(nullptr));
+  MT_HIP(hipFree(packed));
+  MT_HIP(hipFree(part));
+  return rc;
+}
+
+int mt_ll_rhs(int nx, int ny, int nz, double Lx, double Ly, double Lz, const double* ms, const double* k0,
+              const double* u_ea, double A0, double gamma, double alpha, const float* K32, const double* m,
+              const double* hext, int nstates, double* dmdt, double* fields) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(nx >= 2 && ny >= 2 && nz >= 1, "the uniform-grid exchange operator needs nx >= 2 and ny >= 2");
+  MT_CHECK(ms && K32 && m && hext && nstates > 0, "invalid rhs arguments");
+  const int n = nx * ny * nz;
+  const int64_t n3 = (int64_t)3 * n;
+  const double mu0 = 4.0 * 3.141592653589793 * 1e-7;
+  double dx = nx > 1 ? Lx / nx : Lx, dy = ny > 1 ? Ly / ny : Ly, dz = nz > 1 ? Lz / nz : Lz;
+  std::vector<double> jf(n), kf(n), an;
+  std::vector<float> m32(n);
+  bool any_k = false;
+  for (int i = 0; i < n; ++i) {
+    jf[i] = A0 / (mu0 * ms[i]);
+    double K0 = k0 ? k0[i] : 0.0;
+    kf[i] = K0 / (mu0 * ms[i]);
+    m32[i] = (float)ms[i];
+    any_k |= K0 != 0.0;
+  }
+  if (any_k && u_ea) {
+    an.resize((size_t)n * 6);
+    for (int i = 0; i < n; ++i) {
+      double ux = u_ea[i * 3], uy = u_ea[i * 3 + 1], uz = u_ea[i * 3 + 2];
+      double* A = &an[(size_t)i * 6];
+      A[0] = ux * ux; A[1] = ux * uy; A[2] = ux * uz; A[3] = uy * uy; A[4] = uy * uz; A[5] = uz * uz;
+    }
+  }
+  double *djf = nullptr, *dkf = nullptr, *dan = nullptr;
+  float *dms = nullptr, *mf = nullptr, *hb = nullptr;
+  LlgDesc* dd = nullptr;
+  MT_HIP(hipMalloc(&djf, n * sizeof(double)));
+  MT_HIP(hipMalloc(&dkf, n * sizeof(double)));
+  MT_HIP(hipMalloc(&dms, n * sizeof(float)));
+  MT_HIP(hipMalloc(&mf, nstates * n3 * sizeof(float)));
+  MT_HIP(hipMalloc(&hb, nstates * n3 * sizeof(float)));
+  MT_HIP(hipMalloc(&dd, nstates * sizeof(LlgDesc)));
+  MT_HIP(hipMemcpy(djf, jf.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP(hipMemcpy(dkf, kf.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP(hipMemcpy(dms, m32.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  if (!an.empty()) {
+    MT_HIP(hipMalloc(&dan, an.size() * sizeof(double)));
+    MT_HIP(hipMemcpy(dan, an.data(), an.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  // m (FP64) -> FP32 via the generic vector kernel's FP32 pack
+  std::vector<VecDesc> vd(nstates);
+  for (int s = 0; s < nstates; ++s) {
+    VecDesc& d = vd[s];
+    std::memset(&d, 0, sizeof(d));
+    d.dst = nullptr;
+  }
+  std::vector<float> host_tmp;  // not used: conversion done on device below
+  // Reuse the pack path: copy m into a scratch FP64 vector through a combine with pack.
+  double* scratch = nullptr;
+  VecDesc* dvd = nullptr;
+  MT_HIP(hipMalloc(&scratch, nstates * n3 * sizeof(double)));
+  MT_HIP(hipMalloc(&dvd, nstates * sizeof(VecDesc)));
+  for (int s = 0; s < nstates; ++s) {
+    VecDesc& d = vd[s];
+    d.dst = scratch + s * n3;
+    d.base = m + s * n3;
+    d.bcoef = 1.0;
+    d.post_scale = 1.0;
+    d.pack_col = s;
+    d.rkind = kRedNone;
+  }
+  MT_HIP(hipMemcpy(dvd, vd.data(), nstates * sizeof(VecDesc), hipMemcpyHostToDevice));
+  PackSpec pk{kPackF32, mf, n3, nullptr};
+  rc = launch_vec(dvd, nstates, n, nullptr, nullptr, pk, false, nullptr);
+  if (rc == kOk) rc = launch_matvec_f32(K32, n, mf, n3, hb, n3, nstates, nullptr);
+  if (rc == kOk) {
+    std::vector<LlgDesc> ld(nstates);
+    for (int s = 0; s < nstates; ++s) {
+      LlgDesc& L = ld[s];
+      std::memset(&L, 0, sizeof(L));
+      L.m = m + s * n3;
+      L.out = dmdt ? dmdt + s * n3 : nullptr;
+      L.hpart = hb + s * n3;
+      L.split_stride = 0;
+      L.nsplit = 1;
+      L.hext[0] = hext[s * 3]; L.hext[1] = hext[s * 3 + 1]; L.hext[2] = hext[s * 3 + 2];
+      L.alpha = alpha;
+      L.gamma = gamma;
+      L.fields = fields ? fields + (size_t)s * 12 * n : nullptr;
+    }
+    MT_HIP(hipMemcpy(dd, ld.data(), nstates * sizeof(LlgDesc), hipMemcpyHostToDevice));
+    LlgGrid g;
+    g.n = n; g.nx = nx; g.ny = ny; g.nz = nz;
+    g.idx2 = 1.0 / (dx * dx); g.idy2 = 1.0 / (dy * dy); g.idz2 = 1.0 / (dz * dz);
+    g.jfact = djf; g.kfact = dkf; g.ms32 = dms; g.aniso = dan;
+    rc = launch_llg(g, dd, nstates, nullptr);
+  }
+  MT_HIP(hipDeviceSynchronize());
+  (void)hipFree(djf); (void)hipFree(dkf); (void)hipFree(dms); (void)hipFree(mf); (void)hipFree(hb);
+  (void)hipFree(dd); (void)hipFree(scratch); (void)hipFree(dvd);
+  if (dan) (void)hipFree(dan);
+  return rc;
+}
+
+// ---- reference plug-in ABI: one global instance, host arrays, PCIe per call ----
+namespace {
+std::mutex g_icl_mutex;
+float* g_icl_K = nullptr;  // receiver-major FP32 planes
+float* g_icl_io = nullptr; // 6n floats: m (3n) then h (3n)
+int g_icl_n = 0;
+}  // namespace
+
+void icl_initDemagMatrices(const float* Kxx, const float* Kxy, const float* Kxz, const float* Kyy, const float* Kyz,
+                           const float* Kzz, int* n) {
+  std::lock_guard<std::mutex> lock(g_icl_mutex);
+  if (device_ok() != kOk || !n || *n <= 0) return;
+  int N = *n;
+  size_t nn = (size_t)N * N;
+  if (g_icl_K) { (void)hipFree(g_icl_K); g_icl_K = nullptr; }
+  if (g_icl_io) { (void)hipFree(g_icl_io); g_icl_io = nullptr; }
+  float* T = nullptr;
+  if (hipMalloc(&T, 6 * nn * sizeof(float)) != hipSuccess) { set_error("icl_initDemagMatrices: allocation failed"); return; }
+  if (hipMalloc(&g_icl_K, 6 * nn * sizeof(float)) != hipSuccess) { (void)hipFree(T); set_error("icl_initDemagMatrices: allocation failed"); return; }
+  (void)hipMalloc(&g_icl_io, 6 * (size_t)N * sizeof(float));
+  const float* planes[6] = {Kxx, Kxy, Kxz, Kyy, Kyz, Kzz};
+  for (int c = 0; c < 6; ++c) (void)hipMemcpy(T + c * nn, planes[c], nn * sizeof(float), hipMemcpyHostToDevice);
+  dim3 grid((N + 31) / 32, (N + 31) / 32, 6);
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, nullptr, T, N, g_icl_K);
+  (void)hipDeviceSynchronize();
+  (void)hipFree(T);
+  g_icl_n = N;
+}
+
+void icl_MVMult_GetH(const float* Mx, const float* My, const float* Mz, float* Hx, float* Hy, float* Hz, int* n,
+                     float* pref) {
+  std::lock_guard<std::mutex> lock(g_icl_mutex);
+  if (!g_icl_K || !n || *n != g_icl_n) {
+    set_error("icl_MVMult_GetH: icl_initDemagMatrices not called for this size");
+    return;
+  }
+  int N = *n;
+  float* dm = g_icl_io;
+  float* dh = g_icl_io + 3 * (size_t)N;
+  (void)hipMemcpy(dm, Mx, N * sizeof(float), hipMemcpyHostToDevice);
+  (void)hipMemcpy(dm + N, My, N * sizeof(float), hipMemcpyHostToDevice);
+  (void)hipMemcpy(dm + 2 * (size_t)N, Mz, N * sizeof(float), hipMemcpyHostToDevice);
+  if (demag_apply_f32(g_icl_K, N, dm, dh, 1, *pref, nullptr) != kOk) return;
+  (void)hipMemcpy(Hx, dh, N * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipMemcpy(Hy, dh + N, N * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipMemcpy(Hz, dh + 2 * (size_t)N, N * sizeof(float), hipMemcpyDeviceToHost);
+}
+
+void icl_destroy(void) {
+  std::lock_guard<std::mutex> lock(g_icl_mutex);
+  if (g_icl_K) (void)hipFree(g_icl_K);
+  if (g_icl_io) (void)hipFree(g_icl_io);
+  g_icl_K = nullptr;
+  g_icl_io = nullptr;
+  g_icl_n = 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,278 @@
+// (b) Dense demag-field product H = -K m over the six unique tensor blocks.
+//
+// Reference semantics: LandauLifshitzEquationSolver.f90:558-579,706-755 (MKL
+// sgemv x9 per right-hand side) and the CUDA plug-in cu_MVMult_GetH
+// (source/MagTenseFortranCuda/cuda/MagTenseCudaBlas.cu:367-407): m is cast to
+// FP32, each field component is -(K_a1 m_1 + K_a2 m_2 + K_a3 m_3) in FP32, and
+// the off-diagonal blocks serve both of their symmetric positions.
+//
+// Two device paths, both reading each tensor block exactly once per pass:
+//   * matvec_f32  - the plain per-state FP32 drop-in: one pass over the FP32
+//                   receiver-major blocks per state, six row streams per cell.
+//   * gemm_bf16x2 - the batched path: the tensor stored as BF16 hi+lo MFMA
+//                   A-fragments, up to 64 states as the N dimension, three
+//                   split products (hi*hi + hi*lo + lo*hi) per block on
+//                   v_mfma_f32_32x32x16_bf16 with FP32 accumulation.
+// Both write partial sums S = K m (positive); the LL kernel applies the sign,
+// sums k-split partials and scales by the receiver cell's Ms.
+
+#include <hip/hip_bf16.h>
+
+#include "mt_common.h"
+
+namespace mt {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Plain FP32 per-state mat-vec: block = RB receiver rows of one state.
+// ---------------------------------------------------------------------------
+template <int RB, bool VEC4>
+__global__ void __launch_bounds__(256) matvec_f32_kernel(const float* __restrict__ K, int n,
+                                                         const float* __restrict__ m, int64_t m_stride,
+                                                         float* __restrict__ out, int64_t out_stride) {
+  const int s = blockIdx.y;
+  const int i0 = blockIdx.x * RB;
+  const float* mx = m + s * m_stride;
+  const float* my = mx + n;
+  const float* mz = my + n;
+  const size_t nn = (size_t)n * n;
+  float acc[RB][3];
+#pragma unroll
+  for (int r = 0; r < RB; ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.0f;
+  if (VEC4) {
+    for (int j = threadIdx.x * 4; j < n; j += 256 * 4) {
+      float4 vx = *reinterpret_cast<const float4*>(mx + j);
+      float4 vy = *reinterpret_cast<const float4*>(my + j);
+      float4 vz = *reinterpret_cast<const float4*>(mz + j);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        int i = i0 + r;
+        if (i >= n) break;
+        const float* row = K + (size_t)i * n + j;
+        float4 kxx = *reinterpret_cast<const float4*>(row);
+        float4 kxy = *reinterpret_cast<const float4*>(row + nn);
+        float4 kxz = *reinterpret_cast<const float4*>(row + 2 * nn);
+        float4 kyy = *reinterpret_cast<const float4*>(row + 3 * nn);
+        float4 kyz = *reinterpret_cast<const float4*>(row + 4 * nn);
+        float4 kzz = *reinterpret_cast<const float4*>(row + 5 * nn);
+        acc[r][0] += kxx.x * vx.x + kxy.x * vy.x + kxz.x * vz.x + kxx.y * vx.y + kxy.y * vy.y + kxz.y * vz.y +
+                     kxx.z * vx.z + kxy.z * vy.z + kxz.z * vz.z + kxx.w * vx.w + kxy.w * vy.w + kxz.w * vz.w;
+        acc[r][1] += kxy.x * vx.x + kyy.x * vy.x + kyz.x * vz.x + kxy.y * vx.y + kyy.y * vy.y + kyz.y * vz.y +
+                     kxy.z * vx.z + kyy.z * vy.z + kyz.z * vz.z + kxy.w * vx.w + kyy.w * vy.w + kyz.w * vz.w;
+        acc[r][2] += kxz.x * vx.x + kyz.x * vy.x + kzz.x * vz.x + kxz.y * vx.y + kyz.y * vy.y + kzz.y * vz.y +
+                     kxz.z * vx.z + kyz.z * vy.z + kzz.z * vz.z + kxz.w * vx.w + kyz.w * vy.w + kzz.w * vz.w;
+      }
+    }
+  } else {
+    for (int j = threadIdx.x; j < n; j += 256) {
+      float vx = mx[j], vy = my[j], vz = mz[j];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        int i = i0 + r;
+        if (i >= n) break;
+        const float* row = K + (size_t)i * n + j;
+        float kxx = row[0], kxy = row[nn], kxz = row[2 * nn], kyy = row[3 * nn], kyz = row[4 * nn],
+              kzz = row[5 * nn];
+        acc[r][0] += kxx * vx + kxy * vy + kxz * vz;
+        acc[r][1] += kxy * vx + kyy * vy + kyz * vz;
+        acc[r][2] += kxz * vx + kyz * vy + kzz * vz;
+      }
+    }
+  }
+  // block reduction of RB*3 sums: wave shuffle, then across the four waves
+  __shared__ float red[4][RB * 3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float v = acc[r][c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) red[wave][r * 3 + c] = v;
+    }
+  __syncthreads();
+  if (threadIdx.x < RB * 3) {
+    int r = threadIdx.x / 3, c = threadIdx.x % 3;
+    int i = i0 + r;
+    float v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    if (i < n) out[s * out_stride + (int64_t)c * n + i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 batched MFMA GEMM.
+//   A (tensor): fragment ((I*nJ + J)*6 + comp)*2 + part, 1 KiB each, lane l
+//     holds row 32I + l%32, k = 16J + 8(l/32) + [0,8).
+//   B (states): fragment ((J*3 + beta)*2 + sb)*2 + part, lane l holds column
+//     32sb + l%32, same k mapping.
+//   C: 32x32 FP32 per (alpha, sb); lane l, register r -> column l%32,
+//     row 8(r/4) + 4(l/32) + r%4.
+// Grid: x = row group (WAVES row tiles) * KSPLIT + k-split index, so that with
+// KSPLIT a multiple of 8 every XCD streams one fixed slice of the B operand
+// (which then stays resident in that XCD's L2).  One workgroup shares each
+// B chunk through LDS among its WAVES waves.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__device__ inline bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+template <int WAVES, int SBLK>
+__global__ void __launch_bounds__(WAVES * 64) gemm_bf16x2_kernel(const uint4* __restrict__ Kt,
+                                                                  const uint4* __restrict__ Mp, int64_t np,
+                                                                  int ksplit, float* __restrict__ out, int n,
+                                                                  int64_t col_stride, int64_t split_stride,
+                                                                  int ncols) {
+  constexpr int MF = 3 * SBLK * 2;  // B fragments per k-chunk used by this kernel
+  __shared__ uint4 lds[2][MF][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nJ = (int)(np / kTileK);
+  const int ks = blockIdx.x % ksplit;
+  const int group = blockIdx.x / ksplit;
+  const int64_t I = (int64_t)group * WAVES + wave;  // row tile of this wave
+  const int n_row_tiles = (int)(np / kTileM);
+  const bool active = I < n_row_tiles;
+  const int jb = (int)((int64_t)nJ * ks / ksplit);
+  const int je = (int)((int64_t)nJ * (ks + 1) / ksplit);
+
+  floatx16 acc[3][SBLK];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int sb = 0; sb < SBLK; ++sb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][sb][r] = 0.0f;
+
+  // B chunk J: fragments f = (beta*2 + sb)*2 + part at Mp[(J*12 + f)*64 + lane]
+  auto load_b = [&](int J, uint4* regs) {
+    for (int q = threadIdx.x; q < MF * 64; q += WAVES * 64) {
+      int f = q / 64, l = q % 64;
+      int beta = f / (SBLK * 2), rem = f % (SBLK * 2);
+      int sb = rem / 2, part = rem % 2;
+      int gf = (beta * 2 + sb) * 2 + part;
+      regs[q / (WAVES * 64)] = Mp[((int64_t)J * 12 + gf) * 64 + l];
+    }
+  };
+  constexpr int BREGS = (MF * 64 + WAVES * 64 - 1) / (WAVES * 64);
+  auto store_b = [&](int buf, const uint4* regs) {
+    for (int q = threadIdx.x; q < MF * 64; q += WAVES * 64) lds[buf][q / 64][q % 64] = regs[q / (WAVES * 64)];
+  };
+  const uint4* Krow = Kt + (I * nJ) * 12 * 64 + lane;
+  uint4 kc[12], kn[12], bpre[BREGS];
+  if (jb < je) {
+    load_b(jb, bpre);
+    store_b(0, bpre);
+    if (active)
+#pragma unroll
+      for (int f = 0; f < 12; ++f) kc[f] = Krow[((int64_t)jb * 12 + f) * 64];
+  }
+  __syncthreads();
+  for (int J = jb; J < je; ++J) {
+    const int buf = (J - jb) & 1;
+    const bool more = J + 1 < je;
+    if (more) {
+      load_b(J + 1, bpre);
+      if (active)
+#pragma unroll
+        for (int f = 0; f < 12; ++f) kn[f] = Krow[((int64_t)(J + 1) * 12 + f) * 64];
+    }
+    if (active) {
+      bf16x8 bh[3][SBLK], bl[3][SBLK];
+#pragma unroll
+      for (int beta = 0; beta < 3; ++beta)
+#pragma unroll
+        for (int sb = 0; sb < SBLK; ++sb) {
+          bh[beta][sb] = as_bf16x8(lds[buf][(beta * SBLK + sb) * 2 + 0][lane]);
+          bl[beta][sb] = as_bf16x8(lds[buf][(beta * SBLK + sb) * 2 + 1][lane]);
+        }
+      const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int beta = 0; beta < 3; ++beta) {
+          const int c = comp[a][beta];
+          bf16x8 kh = as_bf16x8(kc[c * 2 + 0]);
+          bf16x8 kl = as_bf16x8(kc[c * 2 + 1]);
+#pragma unroll
+          for (int sb = 0; sb < SBLK; ++sb) {
+            acc[a][sb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kh, bh[beta][sb], acc[a][sb], 0, 0, 0);
+            acc[a][sb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kh, bl[beta][sb], acc[a][sb], 0, 0, 0);
+            acc[a][sb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kl, bh[beta][sb], acc[a][sb], 0, 0, 0);
+          }
+        }
+    }
+    if (more) {
+      store_b(buf ^ 1, bpre);
+      if (active)
+#pragma unroll
+        for (int f = 0; f < 12; ++f) kc[f] = kn[f];
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+  // epilogue: partial sums (positive K m) for this k-split
+  float* o = out + (int64_t)ks * split_stride;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int sb = 0; sb < SBLK; ++sb) {
+      const int col = 32 * sb + (lane & 31);
+      if (col >= ncols) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+        if (row < n) o[col * col_stride + (int64_t)a * n + row] = acc[a][sb][r];
+      }
+    }
+}
+
+}  // namespace
+
+int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, float* out, int64_t out_stride,
+                      int nstates, hipStream_t stream) {
+  constexpr int RB = 4;
+  dim3 grid((unsigned)((n + RB - 1) / RB), (unsigned)nstates);
+  bool vec4 = (n % 4) == 0 && (m_stride % 4) == 0;
+  if (vec4)
+    hipLaunchKernelGGL((matvec_f32_kernel<RB, true>), grid, dim3(256), 0, stream, K, n, m, m_stride, out, out_stride);
+  else
+    hipLaunchKernelGGL((matvec_f32_kernel<RB, false>), grid, dim3(256), 0, stream, K, n, m, m_stride, out,
+                       out_stride);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int gemm_ksplit_for(int n, int waves) {
+  int64_t np = round_up(n, kTileM);
+  int nJ = (int)(np / kTileK);
+  int groups = (int)((np / kTileM + waves - 1) / waves);
+  // enough workgroups to fill 256 CUs several times; a multiple of 8 keeps each
+  // XCD on one k-slice of the state operand
+  int ks = 8;
+  while (groups * ks < 2048 && ks * 2 <= nJ) ks *= 2;
+  if (ks > nJ) ks = nJ > 0 ? nJ : 1;
+  return ks;
+}
+
+int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
+                       int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+  constexpr int WAVES = 4;
+  int64_t np = round_up(n, kTileM);
+  int groups = (int)((np / kTileM + WAVES - 1) / WAVES);
+  dim3 grid((unsigned)(groups * ksplit));
+  if (ncols <= 32)
+    hipLaunchKernelGGL((gemm_bf16x2_kernel<WAVES, 1>), grid, dim3(WAVES * 64), 0, stream,
+                       reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out,
+                       n, col_stride, split_stride, ncols);
+  else
+    hipLaunchKernelGGL((gemm_bf16x2_kernel<WAVES, 2>), grid, dim3(WAVES * 64), 0, stream,
+                       reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out,
+                       n, col_stride, split_stride, ncols);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace mt

@@ -1,0 +1,270 @@
+// (c) Fused Landau-Lifshitz right-hand side and (d) the batched vector
+// operations of the RKSUITE 'M' stepper.
+//
+// LL right-hand side, per cell and state, in FP64 (demag term FP32):
+//   reference dmdt_fct and helpers, LandauLifshitzEquationSolver.f90:220-273,
+//   exchange :440-474 (Laplacian with the Neumann rows of :1194-1522),
+//   external field :485-514, anisotropy :523-548, demag scaling :749-754.
+// All effective fields are kept negated as in the reference; the API's alpha
+// is alpha_LL * gamma.
+//
+// Vector operations: one generic kernel forms linear combinations of state
+// vectors (stage arguments, interpolants, copies) and, in the same pass,
+// the weighted max-norms / inner products the step control needs
+// (RKSuitef90.f90 step :1790-1874, stepa :1876-1955, stepb :1957-1976,
+// step_integrate :1262-1270, stiff :2047-2357).  Reductions are two-pass
+// (per-block partials, then a fixed-order combine), so sums are deterministic.
+
+#include <hip/hip_bf16.h>
+
+#include "mt_common.h"
+#include "mt_kernels.h"
+
+namespace mt {
+
+namespace {
+
+__device__ inline uint16_t bf16_bits(float v) {
+  __hip_bfloat16 h = __float2bfloat16(v);
+  return *reinterpret_cast<uint16_t*>(&h);
+}
+__device__ inline float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc* __restrict__ descs) {
+  const LlgDesc& D = descs[blockIdx.y];
+  const int n = g.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* m = D.m;
+  const int ix = i % g.nx;
+  const int iy = (i / g.nx) % g.ny;
+  const int iz = i / (g.nx * g.ny);
+  const int sy = g.nx, sz = g.nx * g.ny;
+  double mc[3], hj[3], hm[3], hk[3];
+  for (int c = 0; c < 3; ++c) {
+    const double* v = m + (size_t)c * n;
+    double vi = v[i];
+    mc[c] = vi;
+    // Laplacian with the reference's boundary rows: (-1, 1) and (1, -1)
+    double lx = ix == 0 ? (v[i + 1] - vi) : (ix == g.nx - 1 ? (v[i - 1] - vi) : (v[i - 1] - 2.0 * vi + v[i + 1]));
+    double ly = iy == 0 ? (v[i + sy] - vi) : (iy == g.ny - 1 ? (v[i - sy] - vi) : (v[i - sy] - 2.0 * vi + v[i + sy]));
+    double lap = lx * g.idx2 + ly * g.idy2;
+    if (g.nz > 1) {
+      double lz = iz == 0 ? (v[i + sz] - vi) : (iz == g.nz - 1 ? (v[i - sz] - vi) : (v[i - sz] - 2.0 * vi + v[i + sz]));
+      lap += lz * g.idz2;
+    }
+    hj[c] = (-2.0 * lap) * g.jfact[i];
+  }
+  // demag: S = sum_k partial (K m); field = fl32(fl32(-S) * Ms_i)
+  for (int c = 0; c < 3; ++c) {
+    float s = 0.0f;
+    for (int k = 0; k < D.nsplit; ++k) s += D.hpart[(int64_t)k * D.split_stride + (int64_t)c * n + i];
+    float h = (-s) * g.ms32[i];
+    hm[c] = (double)h;
+  }
+  if (g.aniso) {
+    const double* A = g.aniso + (size_t)i * 6;  // xx xy xz yy yz zz
+    double f = -2.0 * g.kfact[i];
+    hk[0] = f * (A[0] * mc[0] + A[1] * mc[1] + A[2] * mc[2]);
+    hk[1] = f * (A[1] * mc[0] + A[3] * mc[1] + A[4] * mc[2]);
+    hk[2] = f * (A[2] * mc[0] + A[4] * mc[1] + A[5] * mc[2]);
+  } else {
+    hk[0] = hk[1] = hk[2] = 0.0;
+  }
+  double hh[3] = {-D.hext[0], -D.hext[1], -D.hext[2]};
+  double H[3];
+  for (int c = 0; c < 3; ++c) H[c] = hh[c] + hj[c] + hm[c] + hk[c];
+  double cx = -1.0 * (mc[1] * H[2] - mc[2] * H[1]);
+  double cy = -1.0 * (mc[2] * H[0] - mc[0] * H[2]);
+  double cz = -1.0 * (mc[0] * H[1] - mc[1] * H[0]);
+  double dx = mc[1] * cz - mc[2] * cy;
+  double dy = mc[2] * cx - mc[0] * cz;
+  double dz = mc[0] * cy - mc[1] * cx;
+  if (D.out) {
+    D.out[i] = -D.gamma * cx - D.alpha * dx;
+    D.out[n + i] = -D.gamma * cy - D.alpha * dy;
+    D.out[2 * n + i] = -D.gamma * cz - D.alpha * dz;
+  }
+  if (D.fields) {
+    double* F = D.fields;
+    for (int c = 0; c < 3; ++c) {
+      F[(size_t)c * n + i] = hj[c];
+      F[(size_t)(3 + c) * n + i] = hh[c];
+      F[(size_t)(6 + c) * n + i] = hm[c];
+      F[(size_t)(9 + c) * n + i] = hk[c];
+    }
+  }
+}
+
+__device__ inline double nan_to_inf(double v) { return isnan(v) ? INFINITY : v; }
+
+// Generic vector op; 8 consecutive elements per thread.
+__global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ descs, int n, int n3,
+                                                  int blocks_per_state, double* __restrict__ partials,
+                                                  PackSpec pack) {
+  const VecDesc& D = descs[blockIdx.y];
+  const int e0 = (blockIdx.x * blockDim.x + threadIdx.x) * 8;
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+  const int kind = D.rkind;
+  double vals[8];
+  for (int q = 0; q < 8; ++q) {
+    int e = e0 + q;
+    if (e >= n3) break;
+    double v = 0.0;
+    if (D.dst) {
+      v = D.bconst;
+      if (D.base) v += D.bcoef * D.base[e];
+      for (int k = 0; k < D.nterm; ++k) v += D.coef[k] * D.src[k][e];
+      v = D.post_scale * v;
+      D.dst[e] = v;
+    }
+    vals[q] = v;
+    switch (kind) {
+      case kRedNone:
+        break;
+      case kRedStepB: {  // max |sum c_k s_k| / max(0.5(|y|+|ynew|), thresh)
+        double s = 0.0;
+        for (int k = 0; k < D.rn; ++k) s += D.rcoef[k] * D.rsrc[k][e];
+        double w = fmax(0.5 * (fabs(D.ra[e]) + fabs(D.rb[e])), D.thresh);
+        r0 = fmax(r0, nan_to_inf(fabs(s) / w));
+        break;
+      }
+      case kRedFinal: {  // errest = scale * sum c_k s_k; w = max(0.5(|y|+|ynew|), thresh)
+        double s = 0.0;
+        for (int k = 0; k < D.rn; ++k) s += D.rcoef[k] * D.rsrc[k][e];
+        s = D.rscale * s;
+        double w = fmax(0.5 * (fabs(D.ra[e]) + fabs(D.rb[e])), D.thresh);
+        if (D.rd) D.rd[e] = s;
+        if (D.rw) D.rw[e] = w;
+        r0 = fmax(r0, nan_to_inf(fabs(s / w)));
+        break;
+      }
+      case kRedStepA: {  // w = max(w, |ynew|); max |y|/w, |ynew|/w, |ynew-y|/w, |f-yp|/w
+        double ya = D.ra[e], yn = D.rb[e];
+        double w = fmax(D.rw[e], fabs(yn));
+        D.rw[e] = w;
+        r0 = fmax(r0, nan_to_inf(fabs(ya) / w));
+        r1 = fmax(r1, nan_to_inf(fabs(yn) / w));
+        r2 = fmax(r2, nan_to_inf(fabs(yn - ya) / w));
+        r3 = fmax(r3, nan_to_inf(fabs(D.rc[e] - D.rd[e]) / w));
+        break;
+      }
+      case kRedYpNorm: {  // w = max(|y|, thresh); max over y != 0 of |yp|/w
+        double ya = D.ra[e];
+        double w = fmax(fabs(ya), D.thresh);
+        D.rw[e] = w;
+        if (ya != 0.0) r0 = fmax(r0, nan_to_inf(fabs(D.rb[e]) / w));
+        break;
+      }
+      case kRedWeightsAvg: {  // w = max(0.5(|a|+|b|), thresh)
+        D.rw[e] = fmax(0.5 * (fabs(D.ra[e]) + fabs(D.rb[e])), D.thresh);
+        break;
+      }
+      case kRedWdot: {  // sums of (p/w)(q/w) for up to three pairs (rsrc[2k], rsrc[2k+1])
+        double w = D.rw[e];
+        if (D.rn > 0) r0 += (D.rsrc[0][e] / w) * (D.rsrc[1][e] / w);
+        if (D.rn > 1) r1 += (D.rsrc[2][e] / w) * (D.rsrc[3][e] / w);
+        if (D.rn > 2) r2 += (D.rsrc[4][e] / w) * (D.rsrc[5][e] / w);
+        break;
+      }
+      case kRedMaxDiff: {
+        r0 = fmax(r0, nan_to_inf(fabs(D.ra[e] - D.rb[e])));
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  // pack the combination into the demag input of this state's column
+  if (D.pack_col >= 0 && pack.mode != kPackNone && D.dst) {
+    for (int q = 0; q < 8; ++q) {
+      int e = e0 + q;
+      if (e >= n3) break;
+      int c = e / n, j = e - c * n;
+      float f = (float)vals[q];
+      if (pack.mode == kPackF32) {
+        pack.f32[(int64_t)D.pack_col * pack.col_stride + e] = f;
+      } else {
+        uint16_t hi = bf16_bits(f);
+        uint16_t lo = bf16_bits(f - bf16_value(hi));
+        int J = j / kTileK, half = (j % kTileK) / 8;
+        int lane = (D.pack_col & 31) + 32 * half;
+        int sb = D.pack_col / 32;
+        int64_t frag = (((int64_t)J * 3 + c) * 2 + sb) * 2;
+        uint16_t* p = reinterpret_cast<uint16_t*>(pack.bf16 + frag * kFragBytes + lane * 16) + (j % 8);
+        p[0] = hi;
+        p[kFragBytes / 2] = lo;
+      }
+    }
+  }
+  if (kind == kRedNone || kind == kRedWeightsAvg) return;
+  const bool is_sum = kind == kRedWdot;
+  // block reduction of four values
+  __shared__ double red[4][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double v[4] = {r0, r1, r2, r3};
+  for (int k = 0; k < 4; ++k) {
+    double x = v[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      double o = __shfl_xor(x, off, 64);
+      x = is_sum ? x + o : fmax(x, o);
+    }
+    if (lane == 0) red[wave][k] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    int k = threadIdx.x;
+    double x = red[0][k];
+    for (int w = 1; w < 4; ++w) x = is_sum ? x + red[w][k] : fmax(x, red[w][k]);
+    partials[((int64_t)blockIdx.y * blocks_per_state + blockIdx.x) * 4 + k] = x;
+  }
+}
+
+// One thread per (state, value): fixed-order combine of the block partials.
+__global__ void finalize_kernel(const VecDesc* __restrict__ descs, int nstates, int blocks_per_state,
+                                const double* __restrict__ partials, double* __restrict__ results) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nstates * 4) return;
+  int s = t / 4, k = t % 4;
+  int kind = descs[s].rkind;
+  if (kind == kRedNone || kind == kRedWeightsAvg) {
+    results[t] = 0.0;
+    return;
+  }
+  bool is_sum = kind == kRedWdot;
+  double x = partials[((int64_t)s * blocks_per_state) * 4 + k];
+  for (int b = 1; b < blocks_per_state; ++b) {
+    double o = partials[((int64_t)s * blocks_per_state + b) * 4 + k];
+    x = is_sum ? x + o : fmax(x, o);
+  }
+  results[t] = x;
+}
+
+}  // namespace
+
+int launch_llg(const LlgGrid& g, const LlgDesc* d_descs, int nstates, hipStream_t stream) {
+  dim3 grid((unsigned)((g.n + 255) / 256), (unsigned)nstates);
+  hipLaunchKernelGGL(llg_kernel, grid, dim3(256), 0, stream, g, d_descs);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int vec_blocks_per_state(int n3) { return (n3 + 256 * 8 - 1) / (256 * 8); }
+
+int launch_vec(const VecDesc* d_descs, int nstates, int n, double* partials, double* results, const PackSpec& pack,
+               bool reduce, hipStream_t stream) {
+  int n3 = 3 * n;
+  int bps = vec_blocks_per_state(n3);
+  dim3 grid((unsigned)bps, (unsigned)nstates);
+  hipLaunchKernelGGL(vec_kernel, grid, dim3(256), 0, stream, d_descs, n, n3, bps, partials, pack);
+  MT_HIP(hipGetLastError());
+  if (reduce) {
+    int t = nstates * 4;
+    hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)((t + 127) / 128)), dim3(128), 0, stream, d_descs, nstates, bps,
+                       partials, results);
+    MT_HIP(hipGetLastError());
+  }
+  return kOk;
+}
+
+}  // namespace mt

@@ -1,0 +1,66 @@
+// Shared definitions for the MI355X micromagnetic library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace mt {
+
+// ---- error reporting (thread-local message, int status codes) ---------------
+void set_error(const std::string& msg);
+const char* last_error();
+
+enum Status {
+  kOk = 0,
+  kErrArg = -1,       // invalid argument
+  kErrHip = -2,       // HIP runtime failure
+  kErrSolver = -3,    // RKSUITE fatal condition
+  kErrAlloc = -4,     // device allocation failure
+  kErrState = -5,     // API used in the wrong order
+};
+
+#define MT_HIP(expr)                                                                              \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess) {                                                                       \
+      ::mt::set_error(std::string("HIP error '") + hipGetErrorString(_e) + "' at " + __FILE__ + \
+                      ":" + std::to_string(__LINE__) + " (" #expr ")");                          \
+      return ::mt::kErrHip;                                                                       \
+    }                                                                                             \
+  } while (0)
+
+#define MT_CHECK(cond, msg)          \
+  do {                               \
+    if (!(cond)) {                   \
+      ::mt::set_error(msg);          \
+      return ::mt::kErrArg;          \
+    }                                \
+  } while (0)
+
+// ---- geometry of the fragment-tiled split-BF16 demag tensor ------------------
+// The MFMA consumer (v_mfma_f32_32x32x16_bf16) takes, per lane, 8 consecutive
+// k-values of one row (A) or one column (B).  The tensor and the packed
+// magnetization are stored directly in that order so that every fragment is
+// one contiguous, fully coalesced 1 KiB read (64 lanes x 16 B).
+constexpr int kTileM = 32;   // rows per A fragment
+constexpr int kTileK = 16;   // k per MFMA
+constexpr int kFragBytes = 1024;
+constexpr int kMaxCols = 64;  // states per GEMM pass
+
+__host__ __device__ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Bytes of the split tensor for n cells (6 components, hi+lo planes).
+inline int64_t split_tensor_bytes(int n) {
+  int64_t np = round_up(n, kTileM);
+  return np * np * 6 * 4;
+}
+// Bytes of the packed magnetization operand (3 components, 64 columns, hi+lo).
+inline int64_t packed_m_bytes(int n) {
+  int64_t np = round_up(n, kTileM);
+  return np * 3 * kMaxCols * 4;
+}
+
+}  // namespace mt

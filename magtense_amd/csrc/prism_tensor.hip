@@ -1,0 +1,287 @@
+// (a) Analytical prism demagnetization tensor assembly on gfx950.
+//
+// Math: the Smith et al. (2010) point tensor of a uniformly magnetized prism as
+// used by MagTense, including its guards and sign convention:
+//   source/TileDemagTensor/TileDemagTensor/TileNComponents.f90:629-724
+//   source/TileDemagTensor/TileDemagTensor/TileRectangularPrismTensor.f90:7-149
+// and the per-pair loop of LandauLifshitzEquationSolver.f90:935-984 /
+// DemagFieldGetSolution.f90:685-730 (identity rotation: offset = p_i - c_j).
+//
+// GPU form: one thread evaluates eight consecutive source columns j of one
+// receiver row i, entirely in FP64 (the eight-term arctangent sums and the
+// logarithm ratios cancel catastrophically in the far field, so FP64 is kept
+// for every pair; the FP64 pipe of gfx950 makes this cheap relative to one
+// integration step).  The eight corner distances are shared by all six
+// components.  Results are written in up to three layouts in the same pass:
+//   * FP32 receiver-major planes K[c][i*N + j]   (the plain per-state path),
+//   * FP64 receiver-major planes                 (validation),
+//   * split BF16 hi/lo MFMA A-fragments           (the batched MFMA path).
+// Contraction into FMA is disabled here so that every rounding matches the
+// serial restatement of the reference formula.
+
+#include <hip/hip_bf16.h>
+
+#include "mt_common.h"
+
+#pragma clang fp contract(off)
+
+namespace mt {
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kSclH = 1.0001, kSclL = 0.9999;
+
+__device__ inline double sq(double v) { return v * v; }
+
+// One guarded arctangent argument f/g/h: numerator p*q, denominator d*R, with
+// the "d == 0" limit rule of TileRectangularPrismTensor.f90:53-61 (the
+// coordinate that zeroes d is scaled by 1.0001 and 0.9999 and averaged).
+// d = half_edge - s, where s is the (signed) coordinate along that axis; pos
+// is the position of d's square in the reference's sum of squares (0,1,2).
+__device__ inline double guarded_ratio(double half_edge, double s, double p, double q, double d, double R, int pos) {
+  if (d == 0.0) {
+    double dh = half_edge - s * kSclH;
+    double dl = half_edge - s * kSclL;
+    double rh = pos == 0 ? sq(dh) + sq(p) + sq(q) : (pos == 1 ? sq(p) + sq(dh) + sq(q) : sq(p) + sq(q) + sq(dh));
+    double rl = pos == 0 ? sq(dl) + sq(p) + sq(q) : (pos == 1 ? sq(p) + sq(dl) + sq(q) : sq(p) + sq(q) + sq(dl));
+    double fh = p * q / (dh * sqrt(rh));
+    double fl = p * q / (dl * sqrt(rl));
+    return 0.5 * (fh + fl);
+  }
+  return p * q / (d * R);
+}
+
+// The eight factors of nom/denom (TileNComponents.f90:665-666, 684-685, 702-703)
+// for one log term.  which: 0 -> FF (xy), 1 -> GG (yz), 2 -> HH (xz).  hx[0] =
+// a/2-x, hx[1] = a/2+x (likewise hy, hz); R[i][j][k] the matching corner
+// distances.  A factor with edge sign -1 uses -(half edge + coordinate).
+__device__ inline void log_products(const double hx[2], const double hy[2], const double hz[2],
+                                    const double R[2][2][2], int which, double& nom, double& den) {
+  // (sa,sb,sc) -> index 0 for +1, 1 for -1, reference factor order
+  const int sn[4][3] = {{0, 0, 0}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}};
+  const int sd[4][3] = {{0, 1, 0}, {1, 0, 0}, {0, 0, 1}, {1, 1, 1}};
+  double fn[4], fd[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      const int* s = pass == 0 ? sn[k] : sd[k];
+      double lead = which == 0 ? (s[2] ? -hz[1] : hz[0]) : (which == 1 ? (s[0] ? -hx[1] : hx[0]) : (s[1] ? -hy[1] : hy[0]));
+      (pass == 0 ? fn : fd)[k] = lead + R[s[0]][s[1]][s[2]];
+    }
+  }
+  nom = fn[0] * fn[1] * fn[2] * fn[3];
+  den = fd[0] * fd[1] * fd[2] * fd[3];
+}
+
+__device__ inline void corner_terms(double a, double b, double c, double x, double y, double z, double hx[2],
+                                    double hy[2], double hz[2], double R[2][2][2]) {
+  hx[0] = a / 2. - x; hx[1] = a / 2. - (-x);
+  hy[0] = b / 2. - y; hy[1] = b / 2. - (-y);
+  hz[0] = c / 2. - z; hz[1] = c / 2. - (-z);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) R[i][j][k] = sqrt(sq(hx[i]) + sq(hy[j]) + sq(hz[k]));
+}
+
+__device__ inline double off_diagonal(double a, double b, double c, double x, double y, double z, int which,
+                                      double nom, double den) {
+  double v;
+  if (den == 0.0 || nom == 0.0) {
+    // getF_limit (TileRectangularPrismTensor.f90:7-38): all three coordinates scaled
+    double hx[2], hy[2], hz[2], R[2][2][2];
+    double nl, dl, nh, dh;
+    corner_terms(a, b, c, x * kSclL, y * kSclL, z * kSclL, hx, hy, hz, R);
+    log_products(hx, hy, hz, R, which, nl, dl);
+    corner_terms(a, b, c, x * kSclH, y * kSclH, z * kSclH, hx, hy, hz, R);
+    log_products(hx, hy, hz, R, which, nh, dh);
+    double lim = (0.5 * (nl + nh)) / (0.5 * (dl + dh));
+    v = -1. / (4. * kPi) * log(lim);
+  } else {
+    v = -1. / (4. * kPi) * log(nom / den);
+  }
+  if (fabs((nom - den) / den) < 10. * 2.220446049250313e-16) v = 0.0;
+  return v;
+}
+
+}  // namespace
+
+// K = -N_standard for a prism with full edges (a,b,c) at offset (x,y,z) from
+// its centre.  out: xx, xy, xz, yy, yz, zz.
+__device__ void prism_k6(double a, double b, double c, double x, double y, double z, double out[6]) {
+  double hx[2], hy[2], hz[2], R[2][2][2];
+  corner_terms(a, b, c, x, y, z, hx, hy, hz, R);
+  double sx[2] = {x, -x}, sy[2] = {y, -y}, sz[2] = {z, -z};
+  // sign combinations in the order of TileNComponents.f90:647-649:
+  // (x,y,z) (-x,y,z) (x,-y,z) (x,y,-z) (-x,-y,z) (x,-y,-z) (-x,y,-z) (-x,-y,-z)
+  const int cmb[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+  double sxx = 0.0, syy = 0.0, szz = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    int i = cmb[q][0], j = cmb[q][1], k = cmb[q][2];
+    double r = R[i][j][k];
+    double fx = guarded_ratio(a / 2., sx[i], hy[j], hz[k], hx[i], r, 0);
+    double fy = guarded_ratio(b / 2., sy[j], hx[i], hz[k], hy[j], r, 1);
+    double fz = guarded_ratio(c / 2., sz[k], hx[i], hy[j], hz[k], r, 2);
+    sxx = (q == 0) ? atan(fx) : sxx + atan(fx);
+    syy = (q == 0) ? atan(fy) : syy + atan(fy);
+    szz = (q == 0) ? atan(fz) : szz + atan(fz);
+  }
+  double nxx = 1. / (4. * kPi) * sxx;
+  double nyy = 1. / (4. * kPi) * syy;
+  double nzz = 1. / (4. * kPi) * szz;
+  double nm, dn;
+  log_products(hx, hy, hz, R, 0, nm, dn);
+  double nxy = off_diagonal(a, b, c, x, y, z, 0, nm, dn);
+  log_products(hx, hy, hz, R, 1, nm, dn);
+  double nyz = off_diagonal(a, b, c, x, y, z, 1, nm, dn);
+  log_products(hx, hy, hz, R, 2, nm, dn);
+  double nxz = off_diagonal(a, b, c, x, y, z, 2, nm, dn);
+  out[0] = -nxx;
+  out[1] = -nxy;
+  out[2] = -nxz;
+  out[3] = -nyy;
+  out[4] = -nyz;
+  out[5] = -nzz;
+}
+
+namespace {
+
+__device__ inline uint16_t bf16_bits(float v) {
+  __hip_bfloat16 h = __float2bfloat16(v);  // round to nearest even
+  return *reinterpret_cast<uint16_t*>(&h);
+}
+__device__ inline float bf16_value(uint16_t b) {
+  uint32_t u = (uint32_t)b << 16;
+  return __uint_as_float(u);
+}
+
+// Byte offset of the 16-byte lane chunk holding (row i, cols j..j+7), j % 8 == 0,
+// of component c, part p (0 hi, 1 lo) in the fragment-tiled tensor of padded size np.
+__device__ inline int64_t frag_offset(int64_t np, int64_t i, int64_t j, int c, int p) {
+  int64_t I = i / kTileM, J = j / kTileK;
+  int64_t nJ = np / kTileK;
+  int lane = (int)(i % kTileM) + kTileM * (int)((j % kTileK) / 8);
+  int64_t frag = ((I * nJ + J) * 6 + c) * 2 + p;
+  return frag * kFragBytes + (int64_t)lane * 16;
+}
+
+__global__ void __launch_bounds__(256) assemble_kernel(const double* __restrict__ src, const double* __restrict__ abc,
+                                                       int abc_stride, const double* __restrict__ pts, int n_src,
+                                                       int n_pts, int64_t np, float* __restrict__ out32,
+                                                       double* __restrict__ out64, uint8_t* __restrict__ split) {
+  // Thread -> (row i, 8 columns starting at j0); grid covers the padded square for the split layout.
+  int64_t groups_per_row = (split ? np : round_up(n_src, 8)) / 8;
+  int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = tid / groups_per_row;
+  int64_t j0 = (tid % groups_per_row) * 8;
+  int64_t rows = split ? np : n_pts;
+  if (i >= rows) return;
+  float v32[6][8];
+  bool row_ok = i < n_pts;
+  double px = 0, py = 0, pz = 0;
+  if (row_ok) {
+    px = pts[i * 3 + 0];
+    py = pts[i * 3 + 1];
+    pz = pts[i * 3 + 2];
+  }
+  for (int e = 0; e < 8; ++e) {
+    int64_t j = j0 + e;
+    double k6[6] = {0, 0, 0, 0, 0, 0};
+    bool ok = row_ok && j < n_src;
+    if (ok) {
+      const double* s = src + j * 3;
+      const double* d = abc + j * abc_stride;
+      prism_k6(d[0], d[1], d[2], px - s[0], py - s[1], pz - s[2], k6);
+      if (out64) {
+        size_t plane = (size_t)n_pts * n_src;
+        for (int c = 0; c < 6; ++c) out64[c * plane + (size_t)i * n_src + j] = k6[c];
+      }
+    }
+    for (int c = 0; c < 6; ++c) v32[c][e] = (float)k6[c];
+  }
+  if (out32 && row_ok) {
+    size_t plane = (size_t)n_pts * n_src;
+    for (int c = 0; c < 6; ++c)
+      for (int e = 0; e < 8; ++e)
+        if (j0 + e < n_src) out32[c * plane + (size_t)i * n_src + j0 + e] = v32[c][e];
+  }
+  if (split) {
+    for (int c = 0; c < 6; ++c) {
+      uint16_t hi[8], lo[8];
+      for (int e = 0; e < 8; ++e) {
+        hi[e] = bf16_bits(v32[c][e]);
+        lo[e] = bf16_bits(v32[c][e] - bf16_value(hi[e]));
+      }
+      uint4 h4, l4;
+      h4.x = hi[0] | ((uint32_t)hi[1] << 16); h4.y = hi[2] | ((uint32_t)hi[3] << 16);
+      h4.z = hi[4] | ((uint32_t)hi[5] << 16); h4.w = hi[6] | ((uint32_t)hi[7] << 16);
+      l4.x = lo[0] | ((uint32_t)lo[1] << 16); l4.y = lo[2] | ((uint32_t)lo[3] << 16);
+      l4.z = lo[4] | ((uint32_t)lo[5] << 16); l4.w = lo[6] | ((uint32_t)lo[7] << 16);
+      *reinterpret_cast<uint4*>(split + frag_offset(np, i, j0, c, 0)) = h4;
+      *reinterpret_cast<uint4*>(split + frag_offset(np, i, j0, c, 1)) = l4;
+    }
+  }
+}
+
+// FP32 receiver-major tensor (6 x n x n) -> fragment-tiled split BF16 (padded).
+__global__ void __launch_bounds__(256) split_kernel(const float* __restrict__ K, int n, int64_t np,
+                                                    uint8_t* __restrict__ split) {
+  int64_t groups_per_row = np / 8;
+  int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = tid / groups_per_row;
+  int64_t j0 = (tid % groups_per_row) * 8;
+  if (i >= np) return;
+  size_t plane = (size_t)n * n;
+  for (int c = 0; c < 6; ++c) {
+    uint16_t hi[8], lo[8];
+    for (int e = 0; e < 8; ++e) {
+      int64_t j = j0 + e;
+      float v = (i < n && j < n) ? K[c * plane + (size_t)i * n + j] : 0.0f;
+      hi[e] = bf16_bits(v);
+      lo[e] = bf16_bits(v - bf16_value(hi[e]));
+    }
+    uint4 h4, l4;
+    h4.x = hi[0] | ((uint32_t)hi[1] << 16); h4.y = hi[2] | ((uint32_t)hi[3] << 16);
+    h4.z = hi[4] | ((uint32_t)hi[5] << 16); h4.w = hi[6] | ((uint32_t)hi[7] << 16);
+    l4.x = lo[0] | ((uint32_t)lo[1] << 16); l4.y = lo[2] | ((uint32_t)lo[3] << 16);
+    l4.z = lo[4] | ((uint32_t)lo[5] << 16); l4.w = lo[6] | ((uint32_t)lo[7] << 16);
+    *reinterpret_cast<uint4*>(split + frag_offset(np, i, j0, c, 0)) = h4;
+    *reinterpret_cast<uint4*>(split + frag_offset(np, i, j0, c, 1)) = l4;
+  }
+}
+
+}  // namespace
+
+// Host launchers -------------------------------------------------------------
+
+int launch_assemble(const double* src, const double* abc, int abc_stride, const double* pts, int n_src, int n_pts,
+                    float* out32, double* out64, uint8_t* split, hipStream_t stream) {
+  if (split && n_src != n_pts) {
+    set_error("split tensor layout needs a square tensor (n_src == n_pts)");
+    return kErrArg;
+  }
+  int64_t np = round_up(n_src, kTileM);
+  int64_t groups_per_row = (split ? np : round_up(n_src, 8)) / 8;
+  int64_t rows = split ? np : n_pts;
+  int64_t threads = rows * groups_per_row;
+  int64_t blocks = (threads + 255) / 256;
+  hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, abc, abc_stride, pts, n_src,
+                     n_pts, np, out32, out64, split);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_split(const float* K, int n, uint8_t* split, hipStream_t stream) {
+  int64_t np = round_up(n, kTileM);
+  int64_t threads = np * (np / 8);
+  int64_t blocks = (threads + 255) / 256;
+  hipLaunchKernelGGL(split_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, K, n, np, split);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace mt

@@ -1,0 +1,1353 @@
+// FP64 CPU oracle for the MagTense micromagnetic hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a plain serial restatement of the
+// reference algorithm and is used as the checker: only tests/, the smoke test
+// and the CPU-baseline leg of the benchmark may load it.  The product library
+// (magtense_amd/csrc) never links or calls it.
+//
+// What it restates (reference paths relative to the MagTense source tree):
+//   * prism tensor      source/TileDemagTensor/TileDemagTensor/TileNComponents.f90:629-724
+//                       source/TileDemagTensor/TileDemagTensor/TileRectangularPrismTensor.f90:7-149
+//   * tile/point loop   source/DemagField/DemagField/DemagFieldGetSolution.f90:685-730
+//   * uniform grid      source/MagTenseMicroMag/LandauLifshitzEquationSolver.f90:815-872
+//   * tensor assembly   LandauLifshitzEquationSolver.f90:883-986 (uniform), :1044-1133 (prisms)
+//   * exchange CSR      LandauLifshitzEquationSolver.f90:1194-1522
+//   * anisotropy        LandauLifshitzEquationSolver.f90:1566-1587
+//   * field factors     LandauLifshitzEquationSolver.f90:359-430
+//   * dm/dt             LandauLifshitzEquationSolver.f90:220-273, 440-548, 558-779
+//   * interp1 / locate  source/NumericalIntegration/NumericalIntegration/util.f90:39-85
+//   * ODE driver        source/NumericalIntegration/NumericalIntegration/ODE_Solvers.f90:124-237
+//   * RKSUITE 'M'       source/NumericalIntegration/NumericalIntegration/RKSuitef90.f90
+//                       (method_const :159-593, setup :596-856, range_integrate :942-1183,
+//                        step_integrate :1185-1587, step :1726-1978, stiff :1981-2367,
+//                        interpolate :2655-2997)
+//   * field storage     LandauLifshitzEquationSolver.f90:306-349
+//
+// Conventions: every vector is SoA [mx(0:N); my; mz]; demag blocks are stored
+// receiver-major: K[c][i*N + j] is component c (xx,xy,xz,yy,yz,zz) of the
+// tensor at receiver i due to source prism j (the transpose of the reference's
+// column-major Kxx(:,j) storage, identical for symmetric grids).
+//
+// Deviations (documented, all outside normal operation):
+//   * interp1 clamps to the last table value when t lies right of the table
+//     (the reference reads one element past the end there, util.f90:58).
+//   * fatal RKSUITE conditions return an error code instead of STOPping.
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+const double kPi = 3.141592653589793;      // TileTensorHelperFunctions.f90:5
+const double kMu0 = 4.0 * kPi * 1e-7;
+
+// ----------------------------------------------------------------------------
+// Prism tensor (Smith et al. 2010, as used by MagTense)
+// ----------------------------------------------------------------------------
+const double kSclH = 1.0001, kSclL = 0.9999;  // TileRectangularPrismTensor.f90:11,48
+
+double f3(double a, double b, double c, double x, double y, double z) {
+  // TileRectangularPrismTensor.f90:45-70
+  if (a / 2. - x == 0) {
+    double xh = x * kSclH, xl = x * kSclL;
+    double fh = (b / 2. - y) * (c / 2. - z) /
+                ((a / 2. - xh) * std::sqrt((a / 2. - xh) * (a / 2. - xh) + (b / 2. - y) * (b / 2. - y) +
+                                           (c / 2. - z) * (c / 2. - z)));
+    double fl = (b / 2. - y) * (c / 2. - z) /
+                ((a / 2. - xl) * std::sqrt((a / 2. - xl) * (a / 2. - xl) + (b / 2. - y) * (b / 2. - y) +
+                                           (c / 2. - z) * (c / 2. - z)));
+    return 0.5 * (fh + fl);
+  }
+  return (b / 2. - y) * (c / 2. - z) /
+         ((a / 2. - x) *
+          std::sqrt((a / 2. - x) * (a / 2. - x) + (b / 2. - y) * (b / 2. - y) + (c / 2. - z) * (c / 2. - z)));
+}
+
+double g3(double a, double b, double c, double x, double y, double z) {
+  // TileRectangularPrismTensor.f90:74-93
+  if (b / 2. - y == 0) {
+    double yh = y * kSclH, yl = y * kSclL;
+    double gh = (a / 2. - x) * (c / 2. - z) /
+                ((b / 2. - yh) * std::sqrt((a / 2. - x) * (a / 2. - x) + (b / 2. - yh) * (b / 2. - yh) +
+                                           (c / 2. - z) * (c / 2. - z)));
+    double gl = (a / 2. - x) * (c / 2. - z) /
+                ((b / 2. - yl) * std::sqrt((a / 2. - x) * (a / 2. - x) + (b / 2. - yl) * (b / 2. - yl) +
+                                           (c / 2. - z) * (c / 2. - z)));
+    return 0.5 * (gh + gl);
+  }
+  return (a / 2. - x) * (c / 2. - z) /
+         ((b / 2. - y) *
+          std::sqrt((a / 2. - x) * (a / 2. - x) + (b / 2. - y) * (b / 2. - y) + (c / 2. - z) * (c / 2. - z)));
+}
+
+double h3(double a, double b, double c, double x, double y, double z) {
+  // TileRectangularPrismTensor.f90:97-116
+  if (c / 2. - z == 0) {
+    double zh = z * kSclH, zl = z * kSclL;
+    double hh = (a / 2. - x) * (b / 2. - y) /
+                ((c / 2. - zh) * std::sqrt((a / 2. - x) * (a / 2. - x) + (b / 2. - y) * (b / 2. - y) +
+                                           (c / 2. - zh) * (c / 2. - zh)));
+    double hl = (a / 2. - x) * (b / 2. - y) /
+                ((c / 2. - zl) * std::sqrt((a / 2. - x) * (a / 2. - x) + (b / 2. - y) * (b / 2. - y) +
+                                           (c / 2. - zl) * (c / 2. - zl)));
+    return 0.5 * (hl + hh);
+  }
+  return (a / 2. - x) * (b / 2. - y) /
+         ((c / 2. - z) *
+          std::sqrt((a / 2. - x) * (a / 2. - x) + (b / 2. - y) * (b / 2. - y) + (c / 2. - z) * (c / 2. - z)));
+}
+
+double rr(double a, double b, double c, double x, double y, double z) {
+  return std::sqrt((a / 2. - x) * (a / 2. - x) + (b / 2. - y) * (b / 2. - y) + (c / 2. - z) * (c / 2. - z));
+}
+// TileRectangularPrismTensor.f90:120-149
+double FF3(double a, double b, double c, double x, double y, double z) { return (c / 2. - z) + rr(a, b, c, x, y, z); }
+double GG3(double a, double b, double c, double x, double y, double z) { return (a / 2. - x) + rr(a, b, c, x, y, z); }
+double HH3(double a, double b, double c, double x, double y, double z) { return (b / 2. - y) + rr(a, b, c, x, y, z); }
+
+typedef double (*LogFn)(double, double, double, double, double, double);
+
+double nom_of(LogFn F, double a, double b, double c, double x, double y, double z) {
+  return F(a, b, c, x, y, z) * F(-a, -b, c, x, y, z) * F(a, -b, -c, x, y, z) * F(-a, b, -c, x, y, z);
+}
+double denom_of(LogFn F, double a, double b, double c, double x, double y, double z) {
+  return F(a, -b, c, x, y, z) * F(-a, b, c, x, y, z) * F(a, b, -c, x, y, z) * F(-a, -b, -c, x, y, z);
+}
+
+double getF_limit(double a, double b, double c, double x, double y, double z, LogFn F) {
+  // TileRectangularPrismTensor.f90:7-38
+  double xh = x * kSclH, yh = y * kSclH, zh = z * kSclH;
+  double xl = x * kSclL, yl = y * kSclL, zl = z * kSclL;
+  double nom_l = nom_of(F, a, b, c, xl, yl, zl);
+  double nom_h = nom_of(F, a, b, c, xh, yh, zh);
+  double den_l = denom_of(F, a, b, c, xl, yl, zl);
+  double den_h = denom_of(F, a, b, c, xh, yh, zh);
+  double nom = 0.5 * (nom_l + nom_h);
+  double den = 0.5 * (den_l + den_h);
+  return nom / den;
+}
+
+double offdiag(LogFn F, double a, double b, double c, double x, double y, double z) {
+  // TileNComponents.f90:665-716 (one off-diagonal component, before the final sign flip)
+  double nom = nom_of(F, a, b, c, x, y, z);
+  double den = denom_of(F, a, b, c, x, y, z);
+  double v;
+  if (den == 0 || nom == 0) {
+    v = -1. / (4. * kPi) * std::log(getF_limit(a, b, c, x, y, z, F));
+  } else {
+    v = -1. / (4. * kPi) * std::log(nom / den);
+  }
+  if (std::fabs((nom - den) / den) < 10. * DBL_EPSILON) v = 0;
+  return v;
+}
+
+typedef double (*AtanFn)(double, double, double, double, double, double);
+double diag(AtanFn F, double a, double b, double c, double x, double y, double z) {
+  // TileNComponents.f90:647-661 (the eight-term arctangent sum, summed in the reference's order)
+  return 1. / (4. * kPi) *
+         (std::atan(F(a, b, c, x, y, z)) + std::atan(F(a, b, c, -x, y, z)) + std::atan(F(a, b, c, x, -y, z)) +
+          std::atan(F(a, b, c, x, y, -z)) + std::atan(F(a, b, c, -x, -y, z)) + std::atan(F(a, b, c, x, -y, -z)) +
+          std::atan(F(a, b, c, -x, y, -z)) + std::atan(F(a, b, c, -x, -y, -z)));
+}
+
+// Returns the MagTense tensor K (= -N_standard) for a prism of full edges a,b,c
+// evaluated at offset (x,y,z) from its centre.  Order: xx,xy,xz,yy,yz,zz.
+void prism_k6(double a, double b, double c, double x, double y, double z, double out[6]) {
+  double nxx = diag(f3, a, b, c, x, y, z);
+  double nyy = diag(g3, a, b, c, x, y, z);
+  double nzz = diag(h3, a, b, c, x, y, z);
+  double nxy = offdiag(FF3, a, b, c, x, y, z);
+  double nyz = offdiag(GG3, a, b, c, x, y, z);
+  double nxz = offdiag(HH3, a, b, c, x, y, z);
+  out[0] = -nxx;  // TileNComponents.f90:722
+  out[1] = -nxy;
+  out[2] = -nxz;
+  out[3] = -nyy;
+  out[4] = -nyz;
+  out[5] = -nzz;
+}
+
+// ----------------------------------------------------------------------------
+// Grid
+// ----------------------------------------------------------------------------
+struct Grid {
+  int nx, ny, nz, n;
+  double dx, dy, dz;
+  std::vector<double> pts;  // n*3, row-major
+};
+
+void setup_grid(int nx, int ny, int nz, double Lx, double Ly, double Lz, Grid& g) {
+  // LandauLifshitzEquationSolver.f90:815-872
+  g.nx = nx; g.ny = ny; g.nz = nz; g.n = nx * ny * nz;
+  g.pts.assign((size_t)g.n * 3, 0.0);
+  std::vector<double> xs(nx), ys(ny), zs(nz);
+  if (nx > 1) { g.dx = Lx / nx; for (int i = 0; i < nx; ++i) xs[i] = -Lx / 2 + i * g.dx + g.dx / 2; }
+  else { g.dx = Lx; xs[0] = 0.; }
+  if (ny > 1) { g.dy = Ly / ny; for (int i = 0; i < ny; ++i) ys[i] = -Ly / 2 + i * g.dy + g.dy / 2; }
+  else { g.dy = Ly; ys[0] = 0.; }
+  if (nz > 1) { g.dz = Lz / nz; for (int i = 0; i < nz; ++i) zs[i] = -Lz / 2 + i * g.dz + g.dz / 2; }
+  else { g.dz = Lz; zs[0] = 0.; }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        size_t ind = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
+        g.pts[ind * 3 + 0] = xs[i];
+        g.pts[ind * 3 + 1] = ys[j];
+        g.pts[ind * 3 + 2] = zs[k];
+      }
+}
+
+// ----------------------------------------------------------------------------
+// Exchange operator: the reference's CSR Laplacian with Neumann rows
+// (LandauLifshitzEquationSolver.f90:1194-1522).  Stored row-wise with columns
+// ascending, values already divided by d^2 and summed over directions.
+// ----------------------------------------------------------------------------
+struct Csr {
+  int n;
+  std::vector<int> rowptr, col;
+  std::vector<double> val;
+};
+
+void build_exchange(const Grid& g, Csr& A) {
+  int nx = g.nx, ny = g.ny, nz = g.nz, n = g.n;
+  A.n = n;
+  A.rowptr.assign(n + 1, 0);
+  A.col.clear(); A.val.clear();
+  double ix2 = 1. / (g.dx * g.dx), iy2 = 1. / (g.dy * g.dy), iz2 = 1. / (g.dz * g.dz);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        int r = i + j * nx + k * nx * ny;
+        // collect (col, val) per direction, then merge by column (mkl_sparse_d_add)
+        std::vector<std::pair<int, double>> e;
+        // x: boundary rows (-1, 1) / (1, -1), interior (1, -2, 1)
+        if (i == 0) { e.push_back({r, -1. * ix2}); e.push_back({r + 1, 1. * ix2}); }
+        else if (i == nx - 1) { e.push_back({r - 1, 1. * ix2}); e.push_back({r, -1. * ix2}); }
+        else { e.push_back({r - 1, 1. * ix2}); e.push_back({r, -2. * ix2}); e.push_back({r + 1, 1. * ix2}); }
+        // y
+        if (j == 0) { e.push_back({r, -1. * iy2}); e.push_back({r + nx, 1. * iy2}); }
+        else if (j == ny - 1) { e.push_back({r - nx, 1. * iy2}); e.push_back({r, -1. * iy2}); }
+        else { e.push_back({r - nx, 1. * iy2}); e.push_back({r, -2. * iy2}); e.push_back({r + nx, 1. * iy2}); }
+        // z only when nz > 1 (LandauLifshitzEquationSolver.f90:1386)
+        if (nz > 1) {
+          int s = nx * ny;
+          if (k == 0) { e.push_back({r, -1. * iz2}); e.push_back({r + s, 1. * iz2}); }
+          else if (k == nz - 1) { e.push_back({r - s, 1. * iz2}); e.push_back({r, -1. * iz2}); }
+          else { e.push_back({r - s, 1. * iz2}); e.push_back({r, -2. * iz2}); e.push_back({r + s, 1. * iz2}); }
+        }
+        std::stable_sort(e.begin(), e.end(),
+                         [](const std::pair<int, double>& p, const std::pair<int, double>& q) { return p.first < q.first; });
+        for (size_t q = 0; q < e.size(); ++q) {
+          if (!A.col.empty() && (int)A.col.size() > A.rowptr[r] && A.col.back() == e[q].first) {
+            A.val.back() += e[q].second;
+          } else {
+            A.col.push_back(e[q].first);
+            A.val.push_back(e[q].second);
+          }
+        }
+        A.rowptr[r + 1] = (int)A.col.size();
+      }
+}
+
+// ----------------------------------------------------------------------------
+// interp1_MagTense / locate  (util.f90:39-85)
+// ----------------------------------------------------------------------------
+int locate(const double* xx, int n, double x) {
+  int jl = 0, ju = n + 1;
+  while (ju - jl > 1) {
+    int jm = (ju + jl) / 2;
+    if ((xx[n - 1] >= xx[0]) == (x >= xx[jm - 1])) jl = jm; else ju = jm;
+  }
+  if (x == xx[0]) return 1;
+  if (x == xx[n - 1]) return n - 1;
+  return jl;
+}
+
+// Contiguous-abscissa version used by the solver.
+double interp1c(const std::vector<double>& xs, const std::vector<double>& ys, double x) {
+  int n = (int)xs.size();
+  int j = locate(xs.data(), n, x);
+  if (j == 0) return 0.0;
+  if (j >= n) return ys[n - 1];  // deviation: clamp (reference reads past the table end)
+  double xl = (x - xs[j - 1]) / (xs[j] - xs[j - 1]);
+  return (1 - xl) * ys[j - 1] + xl * ys[j];
+}
+
+// ----------------------------------------------------------------------------
+// Micromagnetic problem and dm/dt
+// ----------------------------------------------------------------------------
+struct Problem {
+  Grid g;
+  Csr Aex;
+  int n;
+  std::vector<float> K;      // 6*N*N FP32 (receiver-major)
+  std::vector<double> Kd;    // 6*N*N FP64 when demag_fp64
+  bool demag_fp64 = false;
+  std::vector<double> Axx, Axy, Axz, Ayy, Ayz, Azz;
+  std::vector<double> Jfact, Kfact;
+  std::vector<float> Mfact;
+  double gamma, alpha0;
+  std::vector<double> alpha_t, alpha_v;
+  int solver;
+  std::vector<double> hext_t, hext_x, hext_y, hext_z;  // dynamic table
+  int hext_ind = 0;                                      // explicit mode row
+  // scratch
+  std::vector<double> Hj, Hh, Hk;
+  std::vector<float> Hm, ms;
+  long long rhs_calls = 0;
+};
+
+double alpha_at(const Problem& P, double t) {
+  // LandauLifshitzEquationSolver.f90:281-294
+  if (P.alpha0 == 0) return interp1c(P.alpha_t, P.alpha_v, t);
+  return P.alpha0;
+}
+
+void update_fields(Problem& P, const double* m, double t) {
+  int n = P.n;
+  const double* mx = m; const double* my = m + n; const double* mz = m + 2 * n;
+  // exchange: LandauLifshitzEquationSolver.f90:440-474
+  for (int c = 0; c < 3; ++c) {
+    const double* v = m + (size_t)c * n;
+    for (int r = 0; r < n; ++r) {
+      double s = 0.0;
+      for (int q = P.Aex.rowptr[r]; q < P.Aex.rowptr[r + 1]; ++q) s += P.Aex.val[q] * v[P.Aex.col[q]];
+      double temp = -2. * s;
+      P.Hj[(size_t)c * n + r] = temp * P.Jfact[r];
+    }
+  }
+  // external field: LandauLifshitzEquationSolver.f90:485-514
+  double hx, hy, hz;
+  if (P.solver == 1) {
+    hx = P.hext_x[P.hext_ind]; hy = P.hext_y[P.hext_ind]; hz = P.hext_z[P.hext_ind];
+  } else {
+    hx = interp1c(P.hext_t, P.hext_x, t);
+    hy = interp1c(P.hext_t, P.hext_y, t);
+    hz = interp1c(P.hext_t, P.hext_z, t);
+  }
+  for (int r = 0; r < n; ++r) { P.Hh[r] = -hx; P.Hh[n + r] = -hy; P.Hh[2 * n + r] = -hz; }
+  // anisotropy: LandauLifshitzEquationSolver.f90:523-548
+  for (int r = 0; r < n; ++r) {
+    double f = -2. * P.Kfact[r];
+    P.Hk[r] = f * (P.Axx[r] * mx[r] + P.Axy[r] * my[r] + P.Axz[r] * mz[r]);
+    P.Hk[n + r] = f * (P.Axy[r] * mx[r] + P.Ayy[r] * my[r] + P.Ayz[r] * mz[r]);
+    P.Hk[2 * n + r] = f * (P.Axz[r] * mx[r] + P.Ayz[r] * my[r] + P.Azz[r] * mz[r]);
+  }
+  // demag: LandauLifshitzEquationSolver.f90:558-579, 706-755.
+  // FP32 semantics: m cast to FP32; each of the nine gemv steps is the
+  // correctly rounded FP32 value of (beta*H - K m) (exact products, FP64 sums);
+  // the result is scaled by the receiver cell's FP32 Ms.
+  for (int r = 0; r < 3 * n; ++r) P.ms[r] = (float)m[r];
+  static const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  size_t nn = (size_t)n * n;
+  if (P.demag_fp64) return;  // FP64 product is formed by demag_fp64()
+  for (int a = 0; a < 3; ++a) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      float h = 0.0f;
+      for (int b = 0; b < 3; ++b) {
+        const float* Krow = P.K.data() + comp[a][b] * nn + (size_t)i * n;
+        const float* v = P.ms.data() + (size_t)b * n;
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += (double)Krow[j] * (double)v[j];
+        h = (float)((double)h - s);
+      }
+      P.Hm[(size_t)a * n + i] = h * P.Mfact[i];
+    }
+  }
+}
+
+// FP64 demag field (used when demag_fp64): H = -Ms_i * sum_j K m
+void demag_fp64(const Problem& P, const double* m, double* H) {
+  int n = P.n;
+  size_t nn = (size_t)n * n;
+  static const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  for (int a = 0; a < 3; ++a)
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int b = 0; b < 3; ++b) {
+        const double* Krow = P.Kd.data() + comp[a][b] * nn + (size_t)i * n;
+        const double* v = m + (size_t)b * n;
+        for (int j = 0; j < n; ++j) s += Krow[j] * v[j];
+      }
+      H[(size_t)a * n + i] = -s * (double)P.Mfact[i];
+    }
+}
+
+void dmdt(Problem& P, double t, const double* m, double* out, std::vector<double>& Hm64) {
+  // LandauLifshitzEquationSolver.f90:220-273
+  int n = P.n;
+  ++P.rhs_calls;
+  update_fields(P, m, t);
+  if (P.demag_fp64) demag_fp64(P, m, Hm64.data());
+  double al = alpha_at(P, t);
+  for (int r = 0; r < n; ++r) {
+    double mx = m[r], my = m[n + r], mz = m[2 * n + r];
+    double hmx = P.demag_fp64 ? Hm64[r] : (double)P.Hm[r];
+    double hmy = P.demag_fp64 ? Hm64[n + r] : (double)P.Hm[n + r];
+    double hmz = P.demag_fp64 ? Hm64[2 * n + r] : (double)P.Hm[2 * n + r];
+    double Hx = P.Hh[r] + P.Hj[r] + hmx + P.Hk[r];
+    double Hy = P.Hh[n + r] + P.Hj[n + r] + hmy + P.Hk[n + r];
+    double Hz = P.Hh[2 * n + r] + P.Hj[2 * n + r] + hmz + P.Hk[2 * n + r];
+    double cx = -1. * (my * Hz - mz * Hy);
+    double cy = -1. * (mz * Hx - mx * Hz);
+    double cz = -1. * (mx * Hy - my * Hx);
+    double dx = my * cz - mz * cy;
+    double dy = mz * cx - mx * cz;
+    double dz = mx * cy - my * cx;
+    out[r] = -P.gamma * cx - al * dx;
+    out[n + r] = -P.gamma * cy - al * dy;
+    out[2 * n + r] = -P.gamma * cz - al * dz;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// RKSUITE method 'M' (Bogacki-Shampine 4(5) FSAL pair) in range mode.
+// ----------------------------------------------------------------------------
+typedef void (*RhsFn)(void* ctx, double t, const double* y, double* dydt);
+
+struct RkStats {
+  long long f_calls;        // total RHS evaluations (primary integration)
+  long long steps;          // accepted steps
+  long long bad_steps;      // rejected attempts after the first step
+  long long flag3;          // "too much work" returns
+  long long flag4;          // stiffness returns
+  double h_last;
+};
+
+struct RkComm {
+  int n;
+  RhsFn f; void* ctx;
+  double t, t_old, t_start, t_end, dir;
+  double h, h_old, h_start, h_average;
+  double tol;
+  long long f_count, full_f_count, step_count, bad_step_count;
+  bool at_t_start, at_t_end;
+  std::vector<double> thresh, weights, ymax;
+  std::vector<double> scratch, y, yp, y_old, err_estimates;  // y_new == yp_old == vtemp == scratch
+  std::vector<double> stages;  // 7 * n (ptr(2..8) = 1..7)
+  double a[13][13], b[13], c[13], r[11][6], e[7];
+  double cost, safety, expon, stability_radius, tan_angle, rs, rs1, rs2, rs3, rs4, toosml;
+  int order, last_stage, max_stiff_iters;
+  double round_off, sqrrmc, cubrmc, sqtiny;
+  double tlast, range_t_end;
+  std::vector<double> xstage, ytemp, p;  // p: 5 * n
+  long long stiff_bad_step_count, hit_t_end_count;
+  double errold;
+  bool chkeff, phase2;
+  int st_range, st_step, st_interp;  // saved states (usable = -2)
+  long long flag3, flag4;
+  double* stage(int k) { return stages.data() + (size_t)(k - 1) * n; }  // k = 1..7
+};
+
+const int kUsable = -2, kFatal = 911;
+
+void method_const_M(RkComm& C) {
+  // RKSuitef90.f90:225-412 (METHD = 2) and :560-593
+  std::memset(C.a, 0, sizeof(C.a)); std::memset(C.b, 0, sizeof(C.b)); std::memset(C.c, 0, sizeof(C.c));
+  std::memset(C.r, 0, sizeof(C.r)); std::memset(C.e, 0, sizeof(C.e));
+  double (*a)[13] = C.a;
+  // a(i,j) stored at a[i-1][j-1]
+  a[1][0] = 1.0 / 6.0;
+  a[2][0] = 2.0 / 27.0; a[2][1] = 4.0 / 27.0;
+  a[3][0] = 183.0 / 1372.0; a[3][1] = -162.0 / 343.0; a[3][2] = 1053.0 / 1372.0;
+  a[4][0] = 68.0 / 297.0; a[4][1] = -4.0 / 11.0; a[4][2] = 42.0 / 143.0; a[4][3] = 1960.0 / 3861.0;
+  a[5][0] = 597.0 / 22528.0; a[5][1] = 81.0 / 352.0; a[5][2] = 63099.0 / 585728.0; a[5][3] = 58653.0 / 366080.0;
+  a[5][4] = 4617.0 / 20480.0;
+  a[6][0] = 174197.0 / 959244.0; a[6][1] = -30942.0 / 79937.0; a[6][2] = 8152137.0 / 19744439.0;
+  a[6][3] = 666106.0 / 1039181.0; a[6][4] = -29421.0 / 29068.0; a[6][5] = 482048.0 / 414219.0;
+  a[7][0] = 587.0 / 8064.0; a[7][1] = 0.0; a[7][2] = 4440339.0 / 15491840.0; a[7][3] = 24353.0 / 124800.0;
+  a[7][4] = 387.0 / 44800.0; a[7][5] = 2152.0 / 5985.0; a[7][6] = 7267.0 / 94080.0;
+  double* b = C.b;
+  b[0] = 2479.0 / 34992.0; b[1] = 0.0; b[2] = 123.0 / 416.0; b[3] = 612941.0 / 3411720.0; b[4] = 43.0 / 1440.0;
+  b[5] = 2272.0 / 6561.0; b[6] = 79937.0 / 1113912.0; b[7] = 3293.0 / 556956.0;
+  double* e = C.e;
+  e[0] = -3.0 / 1280.0; e[1] = 0.0; e[2] = 6561.0 / 632320.0; e[3] = -343.0 / 20800.0; e[4] = 243.0 / 12800.0;
+  e[5] = -1.0 / 95.0; e[6] = 0.0;
+  double* c = C.c;
+  c[0] = 0.0; c[1] = 1.0 / 6.0; c[2] = 2.0 / 9.0; c[3] = 3.0 / 7.0; c[4] = 2.0 / 3.0; c[5] = 3.0 / 4.0; c[6] = 1.0;
+  c[7] = 1.0; c[8] = 1.0 / 2.0; c[9] = 5.0 / 6.0; c[10] = 1.0 / 9.0;
+  a[8][0] = 455.0 / 6144.0; a[9][0] = -837888343715.0 / 13176988637184.0; a[10][0] = 98719073263.0 / 1551965184000.0;
+  a[8][1] = 0.0; a[9][1] = 30409415.0 / 52955362.0; a[10][1] = 1307.0 / 123552.0;
+  a[8][2] = 10256301.0 / 35409920.0; a[9][2] = -48321525963.0 / 759168069632.0;
+  a[10][2] = 4632066559387.0 / 70181753241600.0;
+  a[8][3] = 2307361.0 / 17971200.0; a[9][3] = 8530738453321.0 / 197654829557760.0;
+  a[10][3] = 7828594302389.0 / 382182512025600.0;
+  a[8][4] = -387.0 / 102400.0; a[9][4] = 1361640523001.0 / 1626788720640.0; a[10][4] = 40763687.0 / 11070259200.0;
+  a[8][5] = 73.0 / 5130.0; a[9][5] = -13143060689.0 / 38604458898.0; a[10][5] = 34872732407.0 / 224610586200.0;
+  a[8][6] = -7267.0 / 215040.0; a[9][6] = 18700221969.0 / 379584034816.0; a[10][6] = -2561897.0 / 30105600.0;
+  a[8][7] = 1.0 / 32.0; a[9][7] = -5831595.0 / 847285792.0; a[10][7] = 1.0 / 10.0;
+  a[9][8] = -5183640.0 / 26477681.0; a[10][8] = -1.0 / 10.0;
+  a[10][9] = -1403317093.0 / 11371610250.0;
+  // r(i,j) at r[i-1][j-1]
+  double (*r)[6] = C.r;
+  r[0][5] = -12134338393.0 / 1050809760.0; r[0][4] = -1620741229.0 / 50038560.0;
+  r[0][3] = -2048058893.0 / 59875200.0; r[0][2] = -87098480009.0 / 5254048800.0;
+  r[0][1] = -11513270273.0 / 3502699200.0;
+  r[2][5] = -33197340367.0 / 1218433216.0; r[2][4] = -539868024987.0 / 6092166080.0;
+  r[2][3] = -39991188681.0 / 374902528.0; r[2][2] = -69509738227.0 / 1218433216.0;
+  r[2][1] = -29327744613.0 / 2436866432.0;
+  r[3][5] = -284800997201.0 / 19905339168.0; r[3][4] = -7896875450471.0 / 165877826400.0;
+  r[3][3] = -333945812879.0 / 5671036800.0; r[3][2] = -16209923456237.0 / 497633479200.0;
+  r[3][1] = -2382590741699.0 / 331755652800.0;
+  r[4][5] = -540919.0 / 741312.0; r[4][4] = -103626067.0 / 43243200.0; r[4][3] = -633779.0 / 211200.0;
+  r[4][2] = -32406787.0 / 18532800.0; r[4][1] = -36591193.0 / 86486400.0;
+  r[5][5] = 7157998304.0 / 374350977.0; r[5][4] = 30405842464.0 / 623918295.0; r[5][3] = 183022264.0 / 5332635.0;
+  r[5][2] = -3357024032.0 / 1871754885.0; r[5][1] = -611586736.0 / 89131185.0;
+  r[6][5] = -138073.0 / 9408.0; r[6][4] = -719433.0 / 15680.0; r[6][3] = -1620541.0 / 31360.0;
+  r[6][2] = -385151.0 / 15680.0; r[6][1] = -65403.0 / 15680.0;
+  r[7][5] = 1245.0 / 64.0; r[7][4] = 3991.0 / 64.0; r[7][3] = 4715.0 / 64.0; r[7][2] = 2501.0 / 64.0;
+  r[7][1] = 149.0 / 16.0; r[7][0] = 1.0;
+  r[8][5] = 55.0 / 3.0; r[8][4] = 71.0; r[8][3] = 103.0; r[8][2] = 199.0 / 3.0; r[8][1] = 16.0;
+  r[9][5] = -1774004627.0 / 75810735.0; r[9][4] = -1774004627.0 / 25270245.0; r[9][3] = -26477681.0 / 359975.0;
+  r[9][2] = -11411880511.0 / 379053675.0; r[9][1] = -423642896.0 / 126351225.0;
+  r[10][5] = 35.0; r[10][4] = 105.0; r[10][3] = 117.0; r[10][2] = 59.0; r[10][1] = 12.0;
+
+  C.order = 4; C.tan_angle = 5.2; C.stability_radius = 3.9; C.safety = 0.8;
+  C.last_stage = 7;                          // ptr(8)
+  C.cost = 7.0;                              // fsal: no_of_stages - 1
+  C.max_stiff_iters = std::min(8, std::max(1, (int)(0.05 * C.cost * 50.0)));
+  C.expon = 1.0 / (C.order + 1.0);
+  double cdiff = 1.0;
+  for (int i = 0; i < 7; ++i)
+    for (int j = i + 1; j < 8; ++j) {
+      double d = c[i] - c[j];
+      if (d != 0) cdiff = std::min(cdiff, std::fabs(d));
+    }
+  C.rs = 2.0; C.rs1 = 1.0 / C.rs; C.rs2 = C.rs * C.rs; C.rs3 = C.rs * C.rs2; C.rs4 = 1.0 / C.rs3;
+  C.round_off = 10.0 * DBL_EPSILON;
+  C.sqrrmc = std::sqrt(DBL_EPSILON);
+  C.cubrmc = std::pow(DBL_EPSILON, 1.0 / 3.0);
+  C.sqtiny = std::sqrt(DBL_MIN);
+  C.toosml = C.round_off / cdiff;
+}
+
+// returns 0 or a negative error code
+int rk_setup(RkComm& C, int n, RhsFn f, void* ctx, double t_start, const double* y_start, double t_end, double tol,
+             const double* thres) {
+  // RKSuitef90.f90:596-856 with method 'M', task 'R', errass false, h_start 0
+  C.n = n; C.f = f; C.ctx = ctx;
+  C.h_start = 0.0; C.h_old = 0.0; C.f_count = 0; C.full_f_count = 0; C.step_count = 0; C.bad_step_count = 0;
+  C.at_t_start = true; C.at_t_end = false; C.stiff_bad_step_count = 0; C.hit_t_end_count = 0;
+  C.errold = 0.0; C.h_average = 0.0; C.chkeff = false; C.phase2 = true;
+  C.st_range = -1; C.st_step = -1; C.st_interp = -1;
+  C.flag3 = 0; C.flag4 = 0;
+  method_const_M(C);
+  if (n <= 0) return -1;
+  if (t_start == t_end) return -2;
+  C.t_end = t_end; C.t_start = t_start; C.t_old = t_start; C.t = t_start;
+  C.dir = (t_end - t_start) >= 0 ? 1.0 : -1.0;
+  if (tol > 0.01 || tol < C.round_off) return -3;
+  C.tol = tol;
+  double mn = thres[0];
+  for (int i = 1; i < n; ++i) mn = std::min(mn, thres[i]);
+  if (mn < C.sqtiny) return -4;
+  double hmin = std::max(C.sqtiny, C.toosml * std::max(std::fabs(t_start), std::fabs(t_end)));
+  if (std::fabs(t_end - t_start) < hmin) return -5;
+  C.h = std::max(std::fabs(t_end - t_start) / C.rs3, hmin);
+  C.thresh.assign(thres, thres + n);
+  C.err_estimates.assign(n, 0.0); C.weights.assign(n, 0.0); C.y_old.assign(n, 0.0);
+  C.scratch.assign(n, 0.0); C.y.assign(y_start, y_start + n); C.yp.assign(n, 0.0);
+  C.stages.assign((size_t)7 * n, 0.0); C.ymax.resize(n);
+  for (int i = 0; i < n; ++i) C.ymax[i] = std::fabs(y_start[i]);
+  C.p.assign((size_t)5 * n, 0.0); C.ytemp.assign(n, 0.0); C.xstage.assign(n, 0.0);
+  C.st_range = C.st_range;  // SETUP ok
+  return 0;
+}
+
+inline void call_f(RkComm& C, double t, const double* y, double* out) { C.f(C.ctx, t, y, out); }
+
+// step_r1 (RKSuitef90.f90:1726-1978) for the primary integration.
+void rk_step(RkComm& C, double tnow, double& htry, double& err, double hmin) {
+  int n = C.n;
+  double* y = C.y.data();
+  double* yp = C.yp.data();
+  double* y_new = C.scratch.data();
+  double* errest = C.err_estimates.data();
+  double* w = C.weights.data();
+  const double* th = C.thresh.data();
+  for (;;) {  // attempt_step
+    if (C.phase2)
+      for (int k = 0; k < n; ++k) w[k] = std::max(th[k], std::fabs(y[k]));
+    bool restart = false;
+    for (int i = 2; i <= 8; ++i) {
+      double ai1 = C.a[i - 1][0];
+      for (int k = 0; k < n; ++k) errest[k] = ai1 * yp[k];
+      for (int j = 2; j <= i - 1; ++j) {
+        double aij = C.a[i - 1][j - 1];
+        if (aij != 0.0) {
+          const double* s = C.stage(j - 1);
+          for (int k = 0; k < n; ++k) errest[k] = errest[k] + aij * s[k];
+        }
+      }
+      for (int k = 0; k < n; ++k) y_new[k] = y[k] + htry * errest[k];
+      if (i == 7) {
+        // stepb (RKSuitef90.f90:1957-1976); note it scales by |comm%h|, not htry
+        double m = 0.0;
+        const double* s3 = C.stage(2); const double* s4 = C.stage(3);
+        const double* s5 = C.stage(4); const double* s6 = C.stage(5);
+        for (int k = 0; k < n; ++k) {
+          double v = C.e[0] * yp[k] + C.e[2] * s3[k] + C.e[3] * s4[k] + C.e[4] * s5[k] + C.e[5] * s6[k];
+          double wt = std::max(0.5 * (std::fabs(y[k]) + std::fabs(y_new[k])), th[k]);
+          m = std::max(m, std::fabs(v) / wt);
+        }
+        err = std::fabs(C.h) * m;
+        if (err > C.tol) return;
+      }
+      double tstg = tnow + C.c[i - 1] * htry;
+      if (C.at_t_end && C.c[i - 1] == 1.0) tstg = C.t_end;
+      double* out = C.stage(i - 1);
+      call_f(C, tstg, y_new, out);
+      C.f_count += 1;
+      if (C.phase2 && i <= 3 && std::fabs(htry) > hmin) {
+        // stepa (RKSuitef90.f90:1876-1955)
+        double ynrm = 0, ystgnm = 0, argdif = 0, fdiff = 0;
+        for (int k = 0; k < n; ++k) {
+          w[k] = std::max(w[k], std::fabs(y_new[k]));
+          ynrm = std::max(ynrm, std::fabs(y[k]) / w[k]);
+          ystgnm = std::max(ystgnm, std::fabs(y_new[k]) / w[k]);
+          argdif = std::max(argdif, std::fabs(y_new[k] - y[k]) / w[k]);
+          fdiff = std::max(fdiff, std::fabs(out[k] - yp[k]) / w[k]);
+        }
+        double tdiff = tstg - tnow;
+        double twt = std::fabs(C.t_end - tnow);
+        ynrm = std::max(ynrm, std::fabs(tnow) / twt);
+        ystgnm = std::max(ystgnm, std::fabs(tstg) / twt);
+        argdif = std::max(argdif, std::fabs(tdiff) / twt);
+        bool cutbak = false;
+        if (argdif > C.round_off * std::max(ynrm, ystgnm)) {
+          if (std::fabs(htry) * fdiff > C.stability_radius * argdif) {
+            double scl = std::max(C.rs4, std::min(C.rs1, (C.stability_radius * argdif) / (std::fabs(htry) * fdiff)));
+            htry = scl * htry;
+            cutbak = true;
+          }
+        }
+        if (cutbak) {
+          C.at_t_end = false;
+          if (std::fabs(htry) <= hmin) {
+            htry = std::copysign(hmin, htry);
+            C.phase2 = false;
+          }
+          restart = true;
+          break;
+        }
+      }
+    }
+    if (restart) continue;
+    // error estimate (RKSuitef90.f90:1861-1874)
+    if (C.b[0] != 0.0)
+      for (int k = 0; k < n; ++k) errest[k] = errest[k] - C.b[0] * yp[k];
+    for (int i = 2; i <= 8; ++i) {
+      double bi = C.b[i - 1];
+      if (bi != 0.0) {
+        const double* s = C.stage(i - 1);
+        for (int k = 0; k < n; ++k) errest[k] = errest[k] - bi * s[k];
+      }
+    }
+    for (int k = 0; k < n; ++k) errest[k] = htry * errest[k];
+    double m = 0.0;
+    for (int k = 0; k < n; ++k) {
+      w[k] = std::max(0.5 * (std::fabs(y[k]) + std::fabs(y_new[k])), th[k]);
+      m = std::max(m, std::fabs(errest[k] / w[k]));
+    }
+    err = m;
+    return;
+  }
+}
+
+// stiff_r1 (RKSuitef90.f90:1981-2367)
+void rk_stiff(RkComm& C, bool toomch, bool& sure_stiff) {
+  int n = C.n;
+  sure_stiff = false;
+  bool lots_of_fails = false;
+  if ((C.step_count - 10) % 40 == 0) {
+    lots_of_fails = C.stiff_bad_step_count >= 10;
+    C.stiff_bad_step_count = 0;
+  }
+  bool maybe_stiff = toomch || lots_of_fails;
+  double* w = C.weights.data();
+  double* y = C.y.data();
+  double* v0 = C.err_estimates.data();
+  double* v1 = C.stage(1); double* v2 = C.stage(2); double* v3 = C.stage(3);
+  auto wdot = [&](const double* u, const double* v) {
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += (u[k] / w[k]) * (v[k] / w[k]);
+    return s;
+  };
+  double v0v0 = 0, scale = 0;
+  if (maybe_stiff) {
+    for (int k = 0; k < n; ++k) w[k] = std::max(0.5 * (std::fabs(y[k]) + std::fabs(C.y_old[k])), C.thresh[k]);
+    double q = std::fabs(C.h / C.h_average);
+    maybe_stiff = 0.2 < q && q < 5.0;
+    if (maybe_stiff) {
+      maybe_stiff = C.cost * std::fabs((C.t_end - C.t) / C.h_average) > 5000.0;
+      if (maybe_stiff) {
+        v0v0 = wdot(v0, v0);
+        double yy = wdot(y, y);
+        double ynrm = std::sqrt(yy);
+        scale = ynrm * C.sqrrmc;
+        if (scale == 0.0) {
+          scale = v0v0 * C.sqrrmc;
+          maybe_stiff = scale > 0.0;
+        }
+      }
+    }
+  }
+  if (!maybe_stiff) return;
+  if (v0v0 == 0.0) {
+    for (int k = 0; k < n; ++k) v0[k] = 1.0;
+    v0v0 = wdot(v0, v0);
+  }
+  double v0nrm = std::sqrt(v0v0);
+  for (int k = 0; k < n; ++k) v0[k] = v0[k] / v0nrm;
+  v0v0 = 1.0;
+  typedef std::complex<double> cd;
+  cd root_pair[2], prv_root_pair[2];
+  double rold = 0, rho = 0;
+  double* vtemp = C.scratch.data();
+  auto approx_jacobian = [&](const double* v, double vdotv, double* out) {
+    double temp1 = scale / std::sqrt(vdotv);
+    for (int k = 0; k < n; ++k) vtemp[k] = y[k] + temp1 * v[k];
+    call_f(C, C.t, vtemp, out);
+    C.f_count += 1;
+    for (int k = 0; k < n; ++k) out[k] = (C.h_average / temp1) * (out[k] - C.yp[k]);
+  };
+  auto dominant = [&](double a11, double a10, double a00) {
+    double ratio = a10 / a00;
+    rho = std::fabs(ratio);
+    double det = a00 * a11 - a10 * a10;
+    double res = std::fabs(det / a00);
+    bool big = det == 0.0 || (res <= std::fabs(a11) * 0.001 * 0.001 && std::fabs(ratio - rold) <= 0.001 * rho);
+    if (big) { root_pair[0] = cd(ratio, 0.0); root_pair[1] = cd(0.0, 0.0); }
+    rold = ratio;
+    return big;
+  };
+  auto qroots = [&](double alpha, double beta, cd out[2]) {
+    cd temp = alpha / 2.0;
+    cd sqdisc = std::sqrt(temp * temp - beta);
+    if (sqdisc == cd(0.0, 0.0)) { out[0] = -temp; out[1] = -temp; }
+    else {
+      cd r1 = -temp + sqdisc, r2 = -temp + sqdisc;  // as written in RKSuitef90.f90:2308
+      if (std::abs(r1) > std::abs(r2)) { out[0] = r1; out[1] = r2; } else { out[0] = r2; out[1] = r1; }
+    }
+  };
+  int ntry = 1;
+  for (;;) {
+    approx_jacobian(v0, v0v0, v1);
+    double v1v1 = wdot(v1, v1);
+    if (std::sqrt(v1v1) > 1.0e10 * std::sqrt(v0v0)) return;
+    double v1v0 = wdot(v1, v0);
+    if (ntry == 1) {
+      rold = v1v0 / v0v0;
+      if (std::fabs(rold) < C.cubrmc) return;
+    } else {
+      if (dominant(v1v1, v1v0, v0v0)) break;
+    }
+    approx_jacobian(v1, v1v1, v2);
+    double v2v2 = wdot(v2, v2), v2v0 = wdot(v2, v0), v2v1 = wdot(v2, v1);
+    if (dominant(v2v2, v2v1, v1v1)) break;
+    double det1 = v0v0 * v1v1 - v1v0 * v1v0;
+    double alpha1 = (-v0v0 * v2v1 + v1v0 * v2v0) / det1;
+    double beta1 = (v1v0 * v2v1 - v1v1 * v2v0) / det1;
+    approx_jacobian(v2, v2v2, v3);
+    double v3v3 = wdot(v3, v3), v3v1 = wdot(v3, v1), v3v2 = wdot(v3, v2);
+    if (dominant(v3v3, v3v2, v2v2)) break;
+    double det2 = v1v1 * v2v2 - v2v1 * v2v1;
+    double alpha2 = (-v1v1 * v3v2 + v2v1 * v3v1) / det2;
+    double beta2 = (v2v1 * v3v2 - v2v2 * v3v1) / det2;
+    double res2 = std::fabs(v3v3 + alpha2 * v3v2 + beta2 * v3v1 + alpha2 * v3v2 + alpha2 * alpha2 * v2v2 +
+                            alpha2 * beta2 * v2v1 + beta2 * v3v1 + beta2 * alpha2 * v2v1 + beta2 * beta2 * v1v1);
+    if (res2 <= std::fabs(v3v3) * 0.001 * 0.001) {
+      qroots(alpha1, beta1, prv_root_pair);
+      qroots(alpha2, beta2, root_pair);
+      rho = std::abs(prv_root_pair[0]);
+      double dist = std::min(std::abs(root_pair[0] - prv_root_pair[0]), std::abs(root_pair[0] - prv_root_pair[1]));
+      if (dist <= 0.001 * rho) break;
+    }
+    ntry += 1;
+    if (ntry > C.max_stiff_iters) return;
+    double v3nrm = std::sqrt(v3v3);
+    for (int k = 0; k < n; ++k) v0[k] = v3[k] / v3nrm;
+    v0v0 = 1.0;
+  }
+  if (root_pair[0].real() < 0.0) {
+    if (!(std::abs(root_pair[1]) >= 0.9 * rho && root_pair[1].real() > 0.0)) {
+      if (std::fabs(root_pair[0].imag()) <= std::fabs(root_pair[0].real()) * C.tan_angle) {
+        sure_stiff = rho >= 0.9 * C.stability_radius;
+      }
+    }
+  }
+}
+
+// step_integrate_r1 (RKSuitef90.f90:1185-1587).  Returns the flag (1 ok, 3 work, 4 stiff, 5 tiny h, <0 fatal).
+int rk_step_integrate(RkComm& C) {
+  int n = C.n;
+  int ier = 1;
+  double hmin = 0;
+  if (C.at_t_start) {
+    call_f(C, C.t, C.y.data(), C.yp.data());
+    C.f_count += 1;
+    for (int k = 0; k < n; ++k) C.weights[k] = std::max(std::fabs(C.y[k]), C.thresh[k]);
+    bool phase1 = C.h_start == 0.0;
+    C.phase2 = phase1;
+    if (phase1) {
+      C.h = std::fabs(C.h);
+      double ypnorm = 0.0;
+      for (int k = 0; k < n; ++k)
+        if (C.y[k] != 0.0) ypnorm = std::max(ypnorm, std::fabs(C.yp[k]) / C.weights[k]);
+      double tau = std::pow(C.tol, C.expon);
+      if (C.h * ypnorm > tau) C.h = tau / ypnorm;
+      hmin = std::max(C.sqtiny, C.toosml * std::max(std::fabs(C.t_start), std::fabs(C.t_end)));
+      C.h = C.dir * std::max(C.h, hmin);
+    }
+  } else {
+    if (C.at_t_end) return -10;  // fatal: already at t_end
+  }
+  bool phase3 = true;
+  bool failed = false;
+  for (;;) {  // take_step
+    C.h = std::copysign(std::fabs(C.h), C.dir);
+    C.at_t_end = C.dir * ((C.t + C.h) - C.t_end) >= 0.0;
+    if (C.at_t_end) C.h = C.t_end - C.t;
+    else if (C.dir * ((C.t + 2.0 * C.h) - C.t_end) >= 0.0) C.h = (C.t_end - C.t) / 2.0;
+    hmin = std::max(C.sqtiny, C.toosml * std::max(std::fabs(C.t), std::fabs(C.t + C.h)));
+    if (std::fabs(C.h) < hmin) { ier = 5; break; }
+    if (C.chkeff) {
+      C.hit_t_end_count += 1;
+      if (C.hit_t_end_count >= 100 && C.hit_t_end_count >= C.step_count / 3) {
+        ier = 2; C.hit_t_end_count = 0; break;
+      }
+    }
+    if (!failed) {
+      bool toomch = C.f_count > 5000;
+      if (toomch) {
+        ier = 3;
+        C.full_f_count += C.f_count;
+        C.f_count = 0;
+      }
+      bool sure_stiff = false;
+      rk_stiff(C, toomch, sure_stiff);
+      if (sure_stiff) ier = 4;
+      if (ier != 1) break;
+    }
+    double htry = C.h;
+    double err = 0.0;
+    rk_step(C, C.t, htry, err, hmin);
+    C.h = htry;
+    if (err > C.tol) {
+      double alpha;
+      if (C.at_t_start) { phase3 = false; alpha = C.rs1; }
+      else {
+        C.bad_step_count += 1;
+        C.stiff_bad_step_count += 1;
+        if (failed) alpha = C.rs1;
+        else { alpha = C.safety * std::pow(C.tol / err, C.expon); alpha = std::max(alpha, C.rs1); }
+      }
+      C.h = alpha * C.h;
+      failed = true;
+      continue;
+    }
+    double beta = std::pow(err / C.tol, C.expon);
+    if (!C.at_t_start) {
+      double t1 = std::pow(err, C.expon) / C.h;
+      double t2 = std::pow(C.errold, C.expon) / C.h_old;
+      if (t1 < t2 * 100.0 && t2 < t1 * 100.0) beta = beta * (t1 / t2);
+    }
+    double alpha = C.rs3;
+    if (C.safety < beta * alpha) alpha = C.safety / beta;
+    if (C.at_t_start) {
+      C.phase2 = false;
+      phase3 = phase3 && !C.at_t_end && (alpha > C.rs2);
+      if (phase3) { C.h = alpha * C.h; continue; }
+    }
+    alpha = std::min(alpha, C.rs);
+    if (failed) alpha = std::min(alpha, 1.0);
+    alpha = std::max(alpha, C.rs1);
+    C.h_old = C.h;
+    C.h = alpha * C.h;
+    if (C.at_t_start) C.h_average = C.h_old;
+    else C.h_average = 0.9 * C.h_average + 0.1 * C.h_old;
+    C.at_t_start = false;
+    C.errold = err;
+    C.t_old = C.t;
+    if (C.at_t_end) C.t = C.t_end; else C.t = C.t + C.h_old;
+    C.step_count += 1;
+    // y_old = y; y = y_new; yp_old = yp (yp_old shares storage with y_new); yp = last stage
+    C.y_old = C.y;
+    C.y = C.scratch;
+    C.scratch = C.yp;
+    std::memcpy(C.yp.data(), C.stage(C.last_stage), sizeof(double) * n);
+    break;
+  }
+  if (ier < kFatal) {
+    C.at_t_end = C.t == C.t_end;
+    C.chkeff = C.at_t_end;
+    for (int k = 0; k < n; ++k) C.ymax[k] = std::max(std::fabs(C.y[k]), C.ymax[k]);
+    if (ier == 1) C.st_interp = kUsable;
+  }
+  C.st_step = ier;
+  if (ier == 3) C.flag3 += 1;
+  if (ier == 4) C.flag4 += 1;
+  return ier;
+}
+
+// interpolate_r1 with METHOD 'M' (RKSuitef90.f90:2655-2997); yp_old lives in scratch.
+void rk_interpolate(RkComm& C, double t_want, double* y_want) {
+  int n = C.n;
+  bool initialised = C.st_interp != kUsable;
+  double* p1 = C.p.data(); double* p2 = p1 + n; double* p3 = p2 + n; double* p4 = p3 + n; double* p5 = p4 + n;
+  const double* yp = C.yp.data();
+  const double* yp_old = C.scratch.data();
+  if (!initialised) {
+    // extra_stages (RKSuitef90.f90:2929-2995)
+    double* ytemp = C.ytemp.data();
+    for (int i = 9; i <= 11; ++i) {
+      for (int j = 1; j <= i - 1; ++j) {
+        double aij = C.a[i - 1][j - 1];
+        if (j == 1) for (int k = 0; k < n; ++k) ytemp[k] = aij * yp_old[k];
+        else if (j <= 7) { const double* s = C.stage(j - 1); for (int k = 0; k < n; ++k) ytemp[k] = ytemp[k] + aij * s[k]; }
+        else if (j == 8) for (int k = 0; k < n; ++k) ytemp[k] = ytemp[k] + aij * yp[k];
+        else if (j == 9) { const double* s = C.stage(7); for (int k = 0; k < n; ++k) ytemp[k] = ytemp[k] + aij * s[k]; }
+        else if (j == 10) for (int k = 0; k < n; ++k) ytemp[k] = ytemp[k] + aij * C.xstage[k];
+      }
+      for (int k = 0; k < n; ++k) ytemp[k] = C.y_old[k] + C.h_old * ytemp[k];
+      double ts = C.t_old + C.c[i - 1] * C.h_old;
+      if (i == 9) call_f(C, ts, ytemp, C.stage(7));
+      else if (i == 10) call_f(C, ts, ytemp, C.xstage.data());
+      else call_f(C, ts, ytemp, C.stage(1));
+      C.f_count += 1;
+    }
+    // form_intrp (RKSuitef90.f90:2826-2873); r(i,j) -> C.r[i-1][j-1]
+    double (*r)[6] = C.r;
+    const double *s1 = C.stage(1), *s2 = C.stage(2), *s3 = C.stage(3), *s4 = C.stage(4), *s5 = C.stage(5),
+                 *s6 = C.stage(6), *s7 = C.stage(7);
+    const double* xs = C.xstage.data();
+    for (int k = 0; k < n; ++k) {
+      p5[k] = r[4][5] * s4[k] + ((r[9][5] * xs[k] + r[7][5] * yp[k]) + (r[6][5] * s6[k] + r[5][5] * s5[k])) +
+              ((r[3][5] * s3[k] + r[8][5] * s7[k]) + (r[2][5] * s2[k] + r[10][5] * s1[k]) + r[0][5] * yp_old[k]);
+      p4[k] = (r[9][4] * xs[k] + r[8][4] * s7[k]) + ((r[6][4] * s6[k] + r[5][4] * s5[k]) + r[4][4] * s4[k]) +
+              ((r[3][4] * s3[k] + r[7][4] * yp[k]) + (r[2][4] * s2[k] + r[10][4] * s1[k]) + r[0][4] * yp_old[k]);
+      p3[k] = ((r[3][3] * s3[k] + r[7][3] * yp[k]) + (r[6][3] * s6[k] + r[5][3] * s5[k]) + r[4][3] * s4[k]) +
+              ((r[9][3] * xs[k] + r[8][3] * s7[k]) + (r[2][3] * s2[k] + r[10][3] * s1[k]) + r[0][3] * yp_old[k]);
+      p2[k] = r[4][2] * s4[k] + r[5][2] * s5[k] +
+              ((r[2][2] * s2[k] + r[8][2] * s7[k]) + (r[9][2] * xs[k] + r[7][2] * yp[k]) + r[0][2] * yp_old[k]) +
+              ((r[3][2] * s3[k] + r[10][2] * s1[k]) + r[6][2] * s6[k]);
+      p1[k] = r[4][1] * s4[k] + ((r[5][1] * s5[k] + r[7][1] * yp[k]) + r[0][1] * yp_old[k]) +
+              ((r[2][1] * s2[k] + r[8][1] * s7[k]) + r[9][1] * xs[k]) +
+              ((r[3][1] * s3[k] + r[10][1] * s1[k]) + r[6][1] * s6[k]);
+    }
+    for (size_t k = 0; k < (size_t)5 * n; ++k) C.p[k] = C.h_old * C.p[k];
+    C.st_interp = 1;
+  }
+  // evaluate_intrp (RKSuitef90.f90:2876-2899), intrp_degree = 6
+  double sigma = (t_want - C.t) / C.h_old;
+  for (int k = 0; k < n; ++k) {
+    double v = p5[k] * sigma;
+    v = (v + p4[k]) * sigma;
+    v = (v + p3[k]) * sigma;
+    v = (v + p2[k]) * sigma;
+    v = (v + p1[k]) * sigma;
+    y_want[k] = (v + C.h_old * yp[k]) * sigma + C.y[k];
+  }
+}
+
+// range_integrate_r1 (RKSuitef90.f90:942-1183).  Returns flag; y_got/t_got set.
+int rk_range_integrate(RkComm& C, double t_want, double& t_got, double* y_got) {
+  int n = C.n;
+  if (C.st_range == 5 || C.st_range == 6) return -20;
+  if (C.at_t_start) {
+    C.range_t_end = C.t_end;
+    C.tlast = C.t_start;
+    t_got = C.t_start;
+    if (C.h_start == 0.0) {
+      C.h = std::min(std::fabs(C.h), std::fabs(t_want - C.t_start));
+      double hmin = std::max(C.sqtiny, C.toosml * std::max(std::fabs(C.t_start), std::fabs(C.t_end)));
+      C.h = std::max(C.h, hmin);
+    }
+  } else {
+    if (C.tlast == C.range_t_end) return -21;
+  }
+  if (C.dir * (t_want - C.tlast) <= 0.0) return -22;
+  if (C.dir * (t_want - C.range_t_end) > 0.0) return -23;
+  int ier = 1;
+  for (;;) {
+    bool goback = C.dir * (C.t - t_want) >= 0.0;
+    if (goback) {
+      rk_interpolate(C, t_want, y_got);
+      t_got = t_want;
+      break;
+    }
+    int flag = rk_step_integrate(C);
+    ier = flag;
+    if (flag == 1) continue;
+    if (flag < 0) { C.st_range = kFatal; return flag; }
+    t_got = C.t;
+    std::memcpy(y_got, C.y.data(), sizeof(double) * n);
+    break;
+  }
+  C.tlast = t_got;
+  C.st_range = ier;
+  return ier;
+}
+
+// ----------------------------------------------------------------------------
+// Tensor assembly
+// ----------------------------------------------------------------------------
+void assemble_uniform(const Grid& g, std::vector<double>& Kd) {
+  // LandauLifshitzEquationSolver.f90:935-984: tile j (source, centred at its
+  // cell centre, edges dx,dy,dz) evaluated at every cell centre i.
+  int n = g.n;
+  size_t nn = (size_t)n * n;
+  Kd.assign(6 * nn, 0.0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i < n; ++i) {
+      double x = g.pts[i * 3 + 0] - g.pts[j * 3 + 0];
+      double y = g.pts[i * 3 + 1] - g.pts[j * 3 + 1];
+      double z = g.pts[i * 3 + 2] - g.pts[j * 3 + 2];
+      double k6[6];
+      prism_k6(g.dx, g.dy, g.dz, x, y, z, k6);
+      for (int c = 0; c < 6; ++c) Kd[c * nn + (size_t)i * n + j] = k6[c];
+    }
+  }
+}
+
+struct RhsCtx {
+  Problem* P;
+  std::vector<double> Hm64;
+};
+
+void micromag_rhs(void* ctx, double t, const double* y, double* out) {
+  RhsCtx* R = (RhsCtx*)ctx;
+  dmdt(*R->P, t, y, out, R->Hm64);
+}
+
+struct GenericCtx {
+  void (*cb)(double, const double*, double*, void*);
+  void* user;
+};
+
+void generic_rhs(void* ctx, double t, const double* y, double* out) {
+  GenericCtx* g = (GenericCtx*)ctx;
+  g->cb(t, y, out, g->user);
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI (ctypes)
+// ============================================================================
+extern "C" {
+
+// K (MagTense sign) of one prism, full edges a,b,c, at offset pos from its centre.  out: xx,xy,xz,yy,yz,zz
+void or_prism_k6(double a, double b, double c, const double* pos, double* out) {
+  prism_k6(a, b, c, pos[0], pos[1], pos[2], out);
+}
+
+// Uniform-grid cell centres (row-major n*3) and spacings.
+int or_setup_grid(int nx, int ny, int nz, double Lx, double Ly, double Lz, double* pts, double* dxyz) {
+  Grid g;
+  setup_grid(nx, ny, nz, Lx, Ly, Lz, g);
+  std::memcpy(pts, g.pts.data(), sizeof(double) * g.pts.size());
+  dxyz[0] = g.dx; dxyz[1] = g.dy; dxyz[2] = g.dz;
+  return 0;
+}
+
+// Full FP64 tensor on a uniform grid: K6[c*N*N + i*N + j].
+int or_tensor_uniform(int nx, int ny, int nz, double Lx, double Ly, double Lz, double* K6) {
+  Grid g;
+  setup_grid(nx, ny, nz, Lx, Ly, Lz, g);
+  std::vector<double> Kd;
+  assemble_uniform(g, Kd);
+  std::memcpy(K6, Kd.data(), sizeof(double) * Kd.size());
+  return 0;
+}
+
+// Tensor for arbitrary prisms (sources) at arbitrary points (receivers):
+// out[c*n_pts*n_src + i*n_src + j] for source j (centre src[j*3..], edges abc[j*3..])
+// at point pts[i*3..].  Only source columns [j0, j1) are filled (the rest is untouched).
+int or_tensor_points(int n_src, const double* src, const double* abc, int n_pts, const double* pts, int j0, int j1,
+                     double* out) {
+  size_t plane = (size_t)n_pts * n_src;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int j = j0; j < j1; ++j)
+    for (int i = 0; i < n_pts; ++i) {
+      double k6[6];
+      prism_k6(abc[j * 3], abc[j * 3 + 1], abc[j * 3 + 2], pts[i * 3] - src[j * 3], pts[i * 3 + 1] - src[j * 3 + 1],
+               pts[i * 3 + 2] - src[j * 3 + 2], k6);
+      for (int c = 0; c < 6; ++c) out[c * plane + (size_t)i * n_src + j] = k6[c];
+    }
+  return 0;
+}
+
+// Exchange operator in CSR (0-based, columns ascending); returns nnz.  Pass null arrays to query nnz.
+int or_exchange_csr(int nx, int ny, int nz, double Lx, double Ly, double Lz, int* rowptr, int* col, double* val) {
+  if (nx < 2 || ny < 2) return -1;
+  Grid g;
+  setup_grid(nx, ny, nz, Lx, Ly, Lz, g);
+  Csr A;
+  build_exchange(g, A);
+  if (rowptr) {
+    std::memcpy(rowptr, A.rowptr.data(), sizeof(int) * A.rowptr.size());
+    std::memcpy(col, A.col.data(), sizeof(int) * A.col.size());
+    std::memcpy(val, A.val.data(), sizeof(double) * A.val.size());
+  }
+  return (int)A.col.size();
+}
+
+// Problem description shared by or_dmdt and or_run_micromag.
+typedef struct {
+  int nx, ny, nz;
+  double Lx, Ly, Lz;
+  const double* u_ea;    // n*3 row-major (may be null: no anisotropy axis)
+  const double* Ms;      // n
+  const double* K0;      // n
+  double A0, gamma, alpha0;
+  int solver;            // 1 explicit, 2 dynamic
+  int nt_hext;
+  const double* hext;    // nt_hext*4 row-major [t, hx, hy, hz] (A/m)
+  int nt_alpha;
+  const double* alphat;  // nt_alpha*2 row-major
+  int nt;
+  const double* t;
+  const double* m0;      // 3n SoA
+  double tol, thres;
+  int nt_conv;
+  const double* t_conv;
+  double conv_tol;
+  int demag_fp64;        // 0: FP32 demag product as the reference; 1: FP64 product from FP64 tensor
+  const float* K_in;     // optional 6*n*n FP32 tensor (receiver-major); null: assemble
+  int return_fields;     // 1: fill H_exc/H_ext/H_dem/H_ani
+} or_problem;
+
+typedef struct {
+  long long f_calls, steps, bad_steps, flag3, flag4;
+  double h_last;
+} or_stats;
+
+}  // extern "C"
+
+namespace {
+
+int build_problem(const or_problem* q, Problem& P) {
+  if (q->nx < 2 || q->ny < 2) return -1;
+  setup_grid(q->nx, q->ny, q->nz, q->Lx, q->Ly, q->Lz, P.g);
+  int n = P.g.n;
+  P.n = n;
+  build_exchange(P.g, P.Aex);
+  size_t nn = (size_t)n * n;
+  P.demag_fp64 = q->demag_fp64 != 0;
+  if (P.demag_fp64 || !q->K_in) {
+    assemble_uniform(P.g, P.Kd);
+    if (!P.demag_fp64) {
+      P.K.resize(6 * nn);
+      for (size_t k = 0; k < 6 * nn; ++k) P.K[k] = (float)P.Kd[k];
+      std::vector<double>().swap(P.Kd);
+    }
+  } else {
+    P.K.assign(q->K_in, q->K_in + 6 * nn);
+  }
+  P.Axx.resize(n); P.Axy.resize(n); P.Axz.resize(n); P.Ayy.resize(n); P.Ayz.resize(n); P.Azz.resize(n);
+  for (int i = 0; i < n; ++i) {
+    double ux = q->u_ea ? q->u_ea[i * 3] : 0, uy = q->u_ea ? q->u_ea[i * 3 + 1] : 0, uz = q->u_ea ? q->u_ea[i * 3 + 2] : 0;
+    P.Axx[i] = ux * ux; P.Axy[i] = ux * uy; P.Axz[i] = ux * uz; P.Ayy[i] = uy * uy; P.Ayz[i] = uy * uz; P.Azz[i] = uz * uz;
+  }
+  P.Jfact.resize(n); P.Kfact.resize(n); P.Mfact.resize(n);
+  for (int i = 0; i < n; ++i) {
+    P.Jfact[i] = q->A0 / (kMu0 * q->Ms[i]);
+    P.Mfact[i] = (float)q->Ms[i];
+    P.Kfact[i] = q->K0[i] / (kMu0 * q->Ms[i]);
+  }
+  P.gamma = q->gamma; P.alpha0 = q->alpha0; P.solver = q->solver;
+  for (int k = 0; k < q->nt_alpha; ++k) { P.alpha_t.push_back(q->alphat[k * 2]); P.alpha_v.push_back(q->alphat[k * 2 + 1]); }
+  for (int k = 0; k < q->nt_hext; ++k) {
+    P.hext_t.push_back(q->hext[k * 4]); P.hext_x.push_back(q->hext[k * 4 + 1]);
+    P.hext_y.push_back(q->hext[k * 4 + 2]); P.hext_z.push_back(q->hext[k * 4 + 3]);
+  }
+  P.Hj.assign(3 * (size_t)n, 0.0); P.Hh.assign(3 * (size_t)n, 0.0); P.Hk.assign(3 * (size_t)n, 0.0);
+  P.Hm.assign(3 * (size_t)n, 0.0f); P.ms.assign(3 * (size_t)n, 0.0f);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// One right-hand-side evaluation dm/dt(t, m); optionally returns the four
+// (negated) field terms, each 3n SoA (pass nulls to skip).
+int or_dmdt(const or_problem* q, double t, const double* m, double* out, double* Hj, double* Hh, double* Hm,
+            double* Hk) {
+  Problem P;
+  int rc = build_problem(q, P);
+  if (rc) return rc;
+  std::vector<double> Hm64(3 * (size_t)P.n);
+  dmdt(P, t, m, out, Hm64);
+  size_t n3 = 3 * (size_t)P.n;
+  if (Hj) std::memcpy(Hj, P.Hj.data(), sizeof(double) * n3);
+  if (Hh) std::memcpy(Hh, P.Hh.data(), sizeof(double) * n3);
+  if (Hk) std::memcpy(Hk, P.Hk.data(), sizeof(double) * n3);
+  if (Hm) for (size_t k = 0; k < n3; ++k) Hm[k] = P.demag_fp64 ? Hm64[k] : (double)P.Hm[k];
+  return 0;
+}
+
+// Whole simulation: MagTense_ODE_RKSuite driver (ODE_Solvers.f90:124-237) per
+// applied-field row (explicit) or once (dynamic), then the field storage of
+// LandauLifshitzEquationSolver.f90:306-349.
+// Outputs (n_h = nt_hext for explicit, 1 for dynamic):
+//   t_out[n_h*nt], y_out[n_h*nt*3n] (SoA per time), fields[4][n_h*nt*3n] when requested.
+int or_run_micromag(const or_problem* q, double* t_out, double* y_out, double* H_exc, double* H_ext, double* H_dem,
+                    double* H_ani, or_stats* stats) {
+  Problem P;
+  int rc = build_problem(q, P);
+  if (rc) return rc;
+  int n = P.n, n3 = 3 * n, nt = q->nt;
+  int n_h = q->solver == 1 ? q->nt_hext : 1;
+  std::vector<double> m0(q->m0, q->m0 + n3);
+  RhsCtx R; R.P = &P; R.Hm64.assign(n3, 0.0);
+  std::vector<double> thres(n3, q->thres);
+  if (stats) std::memset(stats, 0, sizeof(or_stats));
+  // Combined output/convergence times (simple_unique then simple_sort)
+  std::vector<double> tc(q->t, q->t + nt);
+  for (int k = 0; k < q->nt_conv; ++k) tc.push_back(q->t_conv[k]);
+  std::vector<double> uniq;
+  for (double v : tc) if (std::find(uniq.begin(), uniq.end(), v) == uniq.end()) uniq.push_back(v);
+  std::vector<double> sorted = uniq;
+  std::stable_sort(sorted.begin(), sorted.end());
+  std::vector<double> y_step(n3), y_last(n3);
+  for (int hi = 0; hi < n_h; ++hi) {
+    P.hext_ind = hi;
+    double* to = t_out + (size_t)hi * nt;
+    double* yo = y_out + (size_t)hi * nt * n3;
+    for (int k = 0; k < nt; ++k) to[k] = 0.0;
+    std::memset(yo, 0, sizeof(double) * (size_t)nt * n3);
+    RkComm C;
+    rc = rk_setup(C, n3, micromag_rhs, &R, q->t[0], m0.data(), q->t[nt - 1], q->tol, thres.data());
+    if (rc) return rc;
+    y_last = m0;
+    to[0] = q->t[0];
+    std::memcpy(yo, m0.data(), sizeof(double) * n3);
+    int k = 1;
+    for (size_t i = 1; i < sorted.size(); ++i) {
+      double tw = sorted[i], t_step = 0;
+      int flag = rk_range_integrate(C, tw, t_step, y_step.data());
+      if (flag < 0) return flag;
+      bool is_conv = false;
+      for (int c = 0; c < q->nt_conv; ++c) if (q->t_conv[c] == tw) is_conv = true;
+      if (is_conv) {
+        double ce = 0.0;
+        for (int r = 0; r < n3; ++r) ce = std::max(ce, std::fabs(y_step[r] - y_last[r]));
+        if (ce < q->conv_tol) {
+          if (k < nt) { std::memcpy(yo + (size_t)k * n3, y_step.data(), sizeof(double) * n3); to[k] = t_step; }
+          break;
+        }
+        y_last = y_step;
+      }
+      bool is_out = false;
+      for (int c = 0; c < nt; ++c) if (q->t[c] == tw) is_out = true;
+      if (is_out && k < nt) {
+        std::memcpy(yo + (size_t)k * n3, y_step.data(), sizeof(double) * n3);
+        to[k] = t_step;
+        k += 1;
+      }
+    }
+    if (stats) {
+      stats->f_calls += C.full_f_count + C.f_count;
+      stats->steps += C.step_count;
+      stats->bad_steps += C.bad_step_count;
+      stats->flag3 += C.flag3;
+      stats->flag4 += C.flag4;
+      stats->h_last = C.h;
+    }
+    // next field starts from the last stored state (LandauLifshitzEquationSolver.f90:177)
+    std::memcpy(m0.data(), yo + (size_t)(nt - 1) * n3, sizeof(double) * n3);
+    if (q->return_fields && H_exc) {
+      std::vector<double> tmp(n3);
+      for (int j = 0; j < nt; ++j) {
+        const double* m = yo + (size_t)j * n3;
+        dmdt(P, q->t[j], m, tmp.data(), R.Hm64);
+        size_t off = ((size_t)hi * nt + j) * n3;
+        std::memcpy(H_exc + off, P.Hj.data(), sizeof(double) * n3);
+        std::memcpy(H_ext + off, P.Hh.data(), sizeof(double) * n3);
+        std::memcpy(H_ani + off, P.Hk.data(), sizeof(double) * n3);
+        for (int r = 0; r < n3; ++r) H_dem[off + r] = P.demag_fp64 ? R.Hm64[r] : (double)P.Hm[r];
+      }
+    }
+  }
+  return 0;
+}
+
+// Generic RKSUITE 'M' integration of y' = cb(t, y) with outputs at t[0..nt) (t[0] = start).
+int or_rk_integrate(int neq, void (*cb)(double, const double*, double*, void*), void* user, int nt, const double* t,
+                    const double* y0, double tol, double thres, double* t_out, double* y_out, or_stats* stats) {
+  GenericCtx g; g.cb = cb; g.user = user;
+  RkComm C;
+  std::vector<double> th(neq, thres);
+  int rc = rk_setup(C, neq, generic_rhs, &g, t[0], y0, t[nt - 1], tol, th.data());
+  if (rc) return rc;
+  t_out[0] = t[0];
+  std::memcpy(y_out, y0, sizeof(double) * neq);
+  for (int i = 1; i < nt; ++i) {
+    double tg = 0;
+    int flag = rk_range_integrate(C, t[i], tg, y_out + (size_t)i * neq);
+    if (flag < 0) return flag;
+    t_out[i] = tg;
+  }
+  if (stats) {
+    stats->f_calls = C.full_f_count + C.f_count;
+    stats->steps = C.step_count;
+    stats->bad_steps = C.bad_step_count;
+    stats->flag3 = C.flag3;
+    stats->flag4 = C.flag4;
+    stats->h_last = C.h;
+  }
+  return 0;
+}
+
+// The RKSUITE 'M' coefficient tables (for order-condition tests): a[11*11], b[8], c[11], e[7], r[11*6].
+void or_rk_tables(double* a, double* b, double* c, double* e, double* r) {
+  RkComm C;
+  method_const_M(C);
+  for (int i = 0; i < 11; ++i) for (int j = 0; j < 11; ++j) a[i * 11 + j] = C.a[i][j];
+  for (int i = 0; i < 8; ++i) b[i] = C.b[i];
+  for (int i = 0; i < 11; ++i) c[i] = C.c[i];
+  for (int i = 0; i < 7; ++i) e[i] = C.e[i];
+  for (int i = 0; i < 11; ++i) for (int j = 0; j < 6; ++j) r[i * 6 + j] = C.r[i][j];
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Demag rows with the reference's FP32 semantics (LandauLifshitzEquationSolver.f90:706-754):
+// out[a*R + r] = fl32(fl32(-sum_b K_ab(row r) m_b) * Ms_r) for R receiver rows whose
+// tensor rows are Krows[c*R*n + r*n + j].  Used to time a bounded CPU sample.
+int or_demag_rows(int n, int R, const float* Krows, const float* ms_rows, const double* m, double* out) {
+  static const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  std::vector<float> m32(3 * (size_t)n);
+  for (size_t k = 0; k < 3 * (size_t)n; ++k) m32[k] = (float)m[k];
+  size_t plane = (size_t)R * n;
+  for (int a = 0; a < 3; ++a) {
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < R; ++r) {
+      float h = 0.0f;
+      for (int b = 0; b < 3; ++b) {
+        const float* Krow = Krows + comp[a][b] * plane + (size_t)r * n;
+        const float* v = m32.data() + (size_t)b * n;
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += (double)Krow[j] * (double)v[j];
+        h = (float)((double)h - s);
+      }
+      out[(size_t)a * R + r] = (double)(h * ms_rows[r]);
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"
