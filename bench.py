@@ -1,0 +1,295 @@
+"""Benchmark: LL steps/s at N = 32768 cells with a 64-state batch on one MI355X.
+
+Workload (the precision + batching configuration): a 256 x 64 x 2 film
+(2 x 2 x 1.5 nm cells, N = 32768), Ms = 8e5 A/m, A = 1.3e-11 J/m, no
+anisotropy, alpha = 0.02 (API alpha = 0.02 * gamma), gamma = 2.21e5.  64
+independent states, each seeded random unit vectors, at 64 applied fields
+evenly spaced from -100 to +100 mT along x (a field sweep).  RKSUITE 'M',
+tol 1e-6.  One benchmark step = one accepted RK45 step of all 64 states (7
+right-hand sides, each one pass over the demag tensor).
+
+The timed path is the split-BF16 MFMA demag product; the plain per-state FP32
+mat-vec path (the drop-in) is timed beside it on the same states, and the
+FP64 CPU oracle is timed on a bounded sample on the host cores.  Prints one
+JSON line.
+"""
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MU0 = 4e-7 * np.pi
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(B=64, seed=1234, res=(256, 64, 2), L=(512e-9, 128e-9, 3e-9)):
+    n = int(np.prod(res))
+    m0 = np.empty((B, 3 * n))
+    for b in range(B):
+        rng = np.random.default_rng(seed + b)
+        m = rng.normal(size=(n, 3))
+        m /= np.linalg.norm(m, axis=1, keepdims=True)
+        m0[b] = np.concatenate([m[:, 0], m[:, 1], m[:, 2]])
+    fields = np.zeros((B, 3))
+    fields[:, 0] = (-100.0 + 200.0 * np.arange(B) / max(B - 1, 1)) / 1000.0 / MU0
+    return dict(res=res, L=L, n=n, B=B, m0=m0, fields=fields)
+
+
+def make_solver(W, precision, t_end=1e-9):
+    from magtense_amd import _lib
+
+    L = _lib.lib()
+    n, B = W["n"], W["B"]
+    keep = {}
+    keep["Ms"] = np.full(n, 8e5)
+    keep["t"] = np.array([0.0, t_end])
+    hext = np.zeros((B, 2, 4))
+    hext[:, 0, 0] = 0.0
+    hext[:, 1, 0] = t_end
+    hext[:, :, 1:] = W["fields"][:, None, :]
+    keep["hext"] = np.ascontiguousarray(hext)
+    keep["m0"] = np.ascontiguousarray(W["m0"])
+    keep["alphat"] = np.zeros((1, 2))
+    keep["tconv"] = np.zeros(1)
+    q = _lib.MtProblem()
+    q.nx, q.ny, q.nz = W["res"]
+    q.Lx, q.Ly, q.Lz = W["L"]
+    q.n_states = B
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    q.Ms = dp(keep["Ms"])
+    q.A0 = 1.3e-11
+    q.gamma = 2.21e5
+    q.alpha0 = 0.02 * 2.21e5
+    q.nt_alpha = 1
+    q.alphat = dp(keep["alphat"])
+    q.solver = 2
+    q.nt_hext = 2
+    q.hext = dp(keep["hext"])
+    q.hext_per_state = 1
+    q.nt = 2
+    q.t = dp(keep["t"])
+    q.m0 = dp(keep["m0"])
+    q.m0_per_state = 1
+    q.tol = 1e-6
+    q.thres = 1e-6
+    q.nt_conv = 1
+    q.t_conv = dp(keep["tconv"])
+    q.conv_tol = 1e-4
+    q.precision = precision
+    q.return_fields = 0
+    h = C.c_void_p()
+    _lib.check(L.mt_solver_create(C.byref(q), C.byref(h)))
+    return L, h, keep
+
+
+def stats(L, h, B):
+    from magtense_amd import _lib
+
+    st = (_lib.MtStats * B)()
+    _lib.check(L.mt_solver_results(h, None, None, None, C.cast(st, C.c_void_p)))
+    return st
+
+
+def time_steps(L, h, steps, B):
+    """Advance until B*steps accepted state-steps were taken (a full batch the whole time)."""
+    from magtense_amd import _lib
+
+    s0 = L.mt_solver_steps_total(h)
+    t0 = time.perf_counter()
+    _lib.check(L.mt_solver_advance_total(h, steps * B))
+    wall = time.perf_counter() - t0
+    return wall, (L.mt_solver_steps_total(h) - s0) / B
+
+
+def field_error_vs_fp64(W, n_rows=1024, n_states=8, seed=7):
+    """Relative demag-field error of the split-BF16 and FP32 products vs an FP64
+    product on sampled receiver rows (FP64 tensor rows from the GPU kernel)."""
+    import torch
+    from magtense_amd import _lib
+
+    L = _lib.lib()
+    n = W["n"]
+    res, Lg = W["res"], W["L"]
+    d = [Lg[k] / res[k] for k in range(3)]
+    ax = [np.array([-Lg[k] / 2 + i * d[k] + d[k] / 2 for i in range(res[k])]) for k in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(n, n_rows, replace=False))
+    dp = torch.as_tensor(pts).cuda()
+    dsub = torch.as_tensor(np.ascontiguousarray(pts[rows])).cuda()
+    dabc = torch.as_tensor(np.array(d)).cuda()
+    K64 = torch.empty(6 * n_rows * n, dtype=torch.float64, device="cuda")
+    _lib.check(L.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, dsub.data_ptr(), n, n_rows, None,
+                                 K64.data_ptr(), None))
+    K32 = torch.empty(6 * n * n, dtype=torch.float32, device="cuda")
+    split = torch.empty(L.mt_split_tensor_bytes(n), dtype=torch.uint8, device="cuda")
+    _lib.check(L.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, dp.data_ptr(), n, n, K32.data_ptr(), None,
+                                 split.data_ptr()))
+    m = W["m0"][:n_states].reshape(n_states, 3, n)
+    dm = torch.as_tensor(m.astype(np.float32)).cuda()
+    h32 = torch.empty(n_states * 3 * n, dtype=torch.float32, device="cuda")
+    hbf = torch.empty_like(h32)
+    _lib.check(L.mt_demag_apply_f32(K32.data_ptr(), n, dm.data_ptr(), h32.data_ptr(), n_states, -1.0))
+    _lib.check(L.mt_demag_apply_bf16x2(split.data_ptr(), n, dm.data_ptr(), hbf.data_ptr(), n_states, -1.0))
+    del K32, split
+    Kr = K64.cpu().numpy().reshape(6, n_rows, n)
+    comp = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    ref = np.zeros((n_states, 3, n_rows))
+    for a in range(3):
+        for b in range(3):
+            ref[:, a] -= m[:, b] @ Kr[comp[a][b]].T
+    g32 = h32.cpu().numpy().reshape(n_states, 3, n)[:, :, rows]
+    gbf = hbf.cpu().numpy().reshape(n_states, 3, n)[:, :, rows]
+    nr = np.linalg.norm(ref)
+    return dict(rows=n_rows, states=n_states,
+                bf16x2_rel=float(np.linalg.norm(gbf - ref) / nr),
+                fp32_rel=float(np.linalg.norm(g32 - ref) / nr),
+                bf16x2_max_rel=float(np.abs(gbf - ref).max() / np.abs(ref).max()))
+
+
+def cpu_baseline(W, rows=2048, reps=5):
+    """FP64 oracle (FP32 demag semantics, 9 gemv-equivalents per RHS) on the host:
+    time the demag product of `rows` receiver rows for one state and extrapolate
+    to a full 64-state batch step (7 RHS per accepted step)."""
+    from oracle import oracle as O
+
+    n = W["n"]
+    res, Lg = W["res"], W["L"]
+    pts, d = O.setup_grid(list(res), list(Lg))
+    rng = np.random.default_rng(3)
+    sel = np.sort(rng.choice(n, rows, replace=False))
+    src = pts
+    abc = np.tile(d, (n, 1))
+    t0 = time.perf_counter()
+    Krows = O.tensor_points(src, abc, pts[sel]).astype(np.float32)
+    t_tensor = time.perf_counter() - t0
+    ms = np.full(rows, 8e5, dtype=np.float32)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.demag_rows(Krows, ms, W["m0"][0])
+        times.append(time.perf_counter() - t0)
+    t_rows = float(np.median(times))
+    t_rhs_state = t_rows * n / rows
+    t_batch_step = t_rhs_state * 7 * W["B"]
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=1.0 / t_batch_step, unit="LL batch-steps/s (64 states)", cores=cores, kind="port",
+                sample=(f"FP64 oracle demag product (FP32 tensor, FP64-exact sums, reference gemv order) for "
+                        f"{rows} of {n} receiver rows x 1 state, median of {reps}: {t_rows:.3f} s; extrapolated "
+                        f"x{n // rows} rows x7 RHS x{W['B']} states (extrapolated); tensor rows took "
+                        f"{t_tensor:.1f} s"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-field-error", action="store_true")
+    ap.add_argument("--batch", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}); "
+              "the multi-GPU row-sharded design is described in DESIGN.md and not launched.", file=sys.stderr)
+        sys.exit(2)
+    # torch ships its own HIP runtime; initialise it before libmagtense_hip's
+    # (both then share the device; the reverse order fails to initialise torch)
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.zeros(1, device="cuda")
+    except ImportError:
+        pass
+    from magtense_amd import _lib
+
+    W = workload(B=args.batch)
+    n, B = W["n"], W["B"]
+    # ---- split-BF16 MFMA path (the measured value) ---------------------------------
+    t0 = time.perf_counter()
+    L, h, keep = make_solver(W, _lib.PREC_BF16X2)
+    t_setup = time.perf_counter() - t0
+    time_steps(L, h, args.warmup, B)
+    st0 = stats(L, h, B)
+    wall, batch_steps = time_steps(L, h, args.steps, B)
+    st1 = stats(L, h, B)
+    passes = st1[0].demag_passes - st0[0].demag_passes
+    rejected = sum(st1[b].bad_steps - st0[b].bad_steps for b in range(B))
+    # kernel duration with HIP events on the solver's stream (separate short run)
+    L.mt_solver_set_timing(h, 1)
+    sta = stats(L, h, B)
+    time_steps(L, h, 2, B)
+    stb = stats(L, h, B)
+    L.mt_solver_set_timing(h, 0)
+    gemm_launches = stb[0].demag_passes - sta[0].demag_passes
+    gemm_ms = stb[0].demag_ms / max(gemm_launches, 1)
+    L.mt_solver_destroy(h)
+    del keep
+    steps_per_s = batch_steps / wall
+    log(f"bf16x2: setup {t_setup:.2f} s, {args.steps} steps in {wall:.3f} s = {steps_per_s:.2f} steps/s; "
+        f"demag passes {passes}, rejected {rejected}, gemm {gemm_ms:.3f} ms/launch")
+    alg_bytes = 6.0 * n * n * 4 + 2 * 3 * n * B * 4
+    achieved = alg_bytes / (gemm_ms * 1e-3) / 1e9
+    # ---- plain FP32 per-state path ------------------------------------------------------
+    plain = None
+    if not args.no_plain:
+        L2, h2, keep2 = make_solver(W, _lib.PREC_FP32)
+        time_steps(L2, h2, 1, B)
+        wall2, bs2 = time_steps(L2, h2, args.plain_steps, B)
+        L2.mt_solver_destroy(h2)
+        del keep2
+        plain = bs2 / wall2
+        log(f"fp32 plain: {args.plain_steps} steps in {wall2:.3f} s = {plain:.4f} steps/s")
+    ferr = None if args.no_field_error else field_error_vs_fp64(W)
+    log(f"field error vs FP64: {ferr}")
+    cpu = None if args.no_cpu else cpu_baseline(W)
+    log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "LL steps/s at N=32768, 64-state batch: split-BF16 MFMA vs FP32 plain, GPU",
+        "value": steps_per_s,
+        "unit": "LL batch-steps/s (one accepted RK45 step of all 64 states)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / batch_steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x2 demag product (FP32 accumulate), FP64 state/RHS/stepper",
+        "data": "synthetic: seeded random unit magnetization, 64-field sweep -100..+100 mT",
+        "config": {"workload": "256x64x2 film (2x2x1.5 nm), N=32768, 64-state field sweep, RKSUITE 'M' tol 1e-6",
+                   "n_cells": n, "batch": B, "precision": "bf16x2", "tol": 1e-6},
+        "state_steps_per_s": steps_per_s * B,
+        "rhs_per_s": passes / wall,
+        "rejected_attempts": int(rejected),
+        "plain_fp32_steps_per_s": plain,
+        "speedup_vs_plain_fp32": (steps_per_s / plain) if plain else None,
+        "setup_s": t_setup,
+        "field_error_vs_fp64": ferr,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "gemm_bf16x2_kernel", "kernel_ms": gemm_ms,
+                     "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -125,6 +125,10 @@ int mt_solver_create(const mt_micromag_problem* problem, mt_solver** out);
 /* Advance every state by `steps` accepted steps (0: run to the end). */
 int mt_solver_advance(mt_solver* s, long long steps);
 int mt_solver_run(mt_solver* s);
+/* Advance until the states' accepted steps, summed over the batch, grew by
+ * total_steps (states keep independent step sizes; the batch stays full). */
+int mt_solver_advance_total(mt_solver* s, long long total_steps);
+long long mt_solver_steps_total(mt_solver* s);
 /* t_out: n_states*n_h*nt; y_out: n_states*n_h*nt*3n; fields: 4x that (H_exc,
  * H_ext, H_dem, H_ani interleaved per output as 12n); stats: n_states entries.
  * n_h = nt_hext for the explicit solver, 1 otherwise.  Any pointer may be null. */

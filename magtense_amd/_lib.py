@@ -57,7 +57,7 @@ class MtStats(C.Structure):
 EXPORTS = [
     "mt_last_error", "mt_version", "mt_device_count", "mt_prism_tensor", "mt_split_tensor_bytes",
     "mt_packed_m_bytes", "mt_split_tensor", "mt_demag_apply_f32", "mt_demag_apply_bf16x2", "mt_ll_rhs",
-    "mt_solver_create", "mt_solver_advance", "mt_solver_run", "mt_solver_results", "mt_solver_state",
+    "mt_solver_create", "mt_solver_advance", "mt_solver_run", "mt_solver_advance_total", "mt_solver_steps_total", "mt_solver_results", "mt_solver_state",
     "mt_solver_set_timing", "mt_solver_destroy", "mt_run_micromag", "icl_initDemagMatrices", "icl_MVMult_GetH",
     "icl_destroy",
 ]
@@ -96,6 +96,9 @@ def load():
     lib.mt_solver_create.argtypes = [C.POINTER(MtProblem), C.POINTER(vp)]
     lib.mt_solver_advance.argtypes = [vp, C.c_longlong]
     lib.mt_solver_run.argtypes = [vp]
+    lib.mt_solver_advance_total.argtypes = [vp, C.c_longlong]
+    lib.mt_solver_steps_total.argtypes = [vp]
+    lib.mt_solver_steps_total.restype = C.c_longlong
     lib.mt_solver_results.argtypes = [vp, vp, vp, vp, vp]
     lib.mt_solver_state.argtypes = [vp, vp]
     lib.mt_solver_set_timing.argtypes = [vp, C.c_int]

@@ -18,6 +18,8 @@
 
 #include <hip/hip_bf16.h>
 
+#include <cstdlib>
+
 #include "mt_common.h"
 
 namespace mt {
@@ -229,6 +231,148 @@ __global__ void __launch_bounds__(WAVES * 64) gemm_bf16x2_kernel(const uint4* __
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-BF16 GEMM, pipelined version.  A workgroup of four waves owns two
+// 32-row tiles; waves (tile, sb) split the 64 state columns in two halves.
+// Every k-chunk (16 columns of the tensor) is streamed HBM -> LDS with
+// global_load_lds_dwordx4 (no VGPRs held by loads in flight) through a ring of
+// STAGES buffers: 24 KiB of tensor fragments (two tiles x 6 components x
+// hi/lo) and 12 KiB of state fragments per stage, each fragment loaded once
+// for the workgroup.  One barrier per chunk: after it, the chunk is complete
+// for every wave and the slot consumed one iteration earlier is free for the
+// chunk STAGES-1 ahead.
+// ---------------------------------------------------------------------------
+// Asynchronous 16-byte-per-lane copy global -> LDS (global_load_lds_dwordx4):
+// lane l's 16 bytes land at LDS byte address lds + 16*l.  Issued as inline
+// assembly so that the compiler does not serialise it with a blanket vmcnt(0)
+// before every LDS read; completion is tracked explicitly with s_waitcnt
+// vmcnt and s_barrier by the caller.
+// M0 (the LDS base of the copy, sampled at issue) is saved and restored around
+// the instruction, with the one wait state an M0 write needs before an LDS DMA.
+__device__ inline void lds_dma16(const void* src, uint32_t lds) {
+  uint32_t saved;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %2\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dwordx4 %1, off\n\t"
+      "s_mov_b32 m0, %0\n\t"
+      "s_nop 0"
+      : "=&s"(saved)
+      : "v"(src), "s"(__builtin_amdgcn_readfirstlane(lds))
+      : "memory");
+}
+__device__ inline uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// s_waitcnt immediate (gfx9 encoding) that waits for vmcnt <= v only:
+// vmcnt[3:0] bits 3:0, vmcnt[5:4] bits 15:14, expcnt 7 and lgkmcnt 15 = no wait.
+constexpr int vmcnt_imm(int v) { return 0x0F70 | (v & 15) | ((v >> 4) << 14); }
+
+template <int STAGES>
+__global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __restrict__ Kt,
+                                                              const uint4* __restrict__ Mp, int64_t np, int ksplit,
+                                                              float* __restrict__ out, int n, int64_t col_stride,
+                                                              int64_t split_stride, int ncols) {
+  constexpr int FR = 36;             // fragments per stage: 2 tiles x 12 + 12 state fragments
+  constexpr int LOADS = 9;           // per wave per stage: 6 tensor + 3 state fragments
+  __shared__ uint4 smem[STAGES][FR][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tl = wave >> 1, sb = wave & 1;
+  const int nJ = (int)(np / kTileK);
+  const int n_row_tiles = (int)(np / kTileM);
+  const int ks = blockIdx.x % ksplit;
+  const int group = blockIdx.x / ksplit;
+  const int jb = (int)((int64_t)nJ * ks / ksplit);
+  const int je = (int)((int64_t)nJ * (ks + 1) / ksplit);
+  const int64_t I = (int64_t)group * 2 + tl;
+  const bool tile_ok = I < n_row_tiles;
+  const bool cols_ok = 32 * sb < ncols;
+
+  // this wave's share of the loads of one stage
+  int64_t ksrc_tile[6];
+  int kdst[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    int f = wave * 6 + q;            // 0..23
+    int t = f / 12, kf = f % 12;
+    int64_t It = (int64_t)group * 2 + t;
+    if (It >= n_row_tiles) It = 0;   // keep the load count uniform; never consumed
+    ksrc_tile[q] = (It * nJ) * 12 + kf;
+    kdst[q] = t * 12 + kf;
+  }
+  auto issue = [&](int J, int slot) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const uint4* src = Kt + (ksrc_tile[q] + (int64_t)J * 12) * 64 + lane;
+      lds_dma16(src, lds_addr(&smem[slot][kdst[q]][0]));
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      int f = wave * 3 + q;          // 0..11
+      const uint4* src = Mp + ((int64_t)J * 12 + f) * 64 + lane;
+      lds_dma16(src, lds_addr(&smem[slot][24 + f][0]));
+    }
+  };
+
+  floatx16 acc[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[a][r] = 0.0f;
+
+  const int nchunks = je - jb;
+  if (nchunks > 0) {
+#pragma unroll
+    for (int s = 0; s < STAGES - 1; ++s) issue(jb + (s < nchunks ? s : nchunks - 1), s);
+    for (int j = 0; j < nchunks; ++j) {
+      // my loads of chunk j are done once at most STAGES-2 later stages are pending
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm((STAGES - 2) * LOADS));
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+      {
+        int jn = j + STAGES - 1;
+        issue(jb + (jn < nchunks ? jn : nchunks - 1), jn % STAGES);
+      }
+      const int slot = j % STAGES;
+      if (tile_ok && cols_ok) {
+        bf16x8 kf[12], bh[3], bl[3];
+#pragma unroll
+        for (int f = 0; f < 12; ++f) kf[f] = as_bf16x8(smem[slot][tl * 12 + f][lane]);
+#pragma unroll
+        for (int beta = 0; beta < 3; ++beta) {
+          bh[beta] = as_bf16x8(smem[slot][24 + (beta * 2 + sb) * 2 + 0][lane]);
+          bl[beta] = as_bf16x8(smem[slot][24 + (beta * 2 + sb) * 2 + 1][lane]);
+        }
+        const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int beta = 0; beta < 3; ++beta) {
+            const int c = comp[a][beta];
+            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kf[2 * c], bh[beta], acc[a], 0, 0, 0);
+            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kf[2 * c], bl[beta], acc[a], 0, 0, 0);
+            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kf[2 * c + 1], bh[beta], acc[a], 0, 0, 0);
+          }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // no LDS write may outlive the workgroup
+  }
+  if (!tile_ok || !cols_ok) return;
+  float* o = out + (int64_t)ks * split_stride;
+  const int col = 32 * sb + (lane & 31);
+  if (col >= ncols) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+      if (row < n) o[col * col_stride + (int64_t)a * n + row] = acc[a][r];
+    }
+}
+
 }  // namespace
 
 int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, float* out, int64_t out_stride,
@@ -249,16 +393,33 @@ int gemm_ksplit_for(int n, int waves) {
   int64_t np = round_up(n, kTileM);
   int nJ = (int)(np / kTileK);
   int groups = (int)((np / kTileM + waves - 1) / waves);
-  // enough workgroups to fill 256 CUs several times; a multiple of 8 keeps each
-  // XCD on one k-slice of the state operand
-  int ks = 8;
+  // enough workgroups to fill 256 CUs several times (measured on N = 32768:
+  // k-split 4 streams 6.0 TB/s, 8 5.9 TB/s, 16 5.5 TB/s)
+  int ks = 4;
   while (groups * ks < 2048 && ks * 2 <= nJ) ks *= 2;
   if (ks > nJ) ks = nJ > 0 ? nJ : 1;
   return ks;
 }
 
+int gemm_variant() {
+  static int v = [] {
+    const char* e = getenv("MT_GEMM_VARIANT");
+    return e ? atoi(e) : 2;
+  }();
+  return v;
+}
+
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+  if (gemm_variant() == 2) {
+    int64_t np2 = round_up(n, kTileM);
+    int groups2 = (int)((np2 / kTileM + 1) / 2);
+    hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4>), dim3((unsigned)(groups2 * ksplit)), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np2, ksplit,
+                       out, n, col_stride, split_stride, ncols);
+    MT_HIP(hipGetLastError());
+    return kOk;
+  }
   constexpr int WAVES = 4;
   int64_t np = round_up(n, kTileM);
   int groups = (int)((np / kTileM + WAVES - 1) / WAVES);

@@ -98,17 +98,22 @@ __global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc
 
 __device__ inline double nan_to_inf(double v) { return isnan(v) ? INFINITY : v; }
 
-// Generic vector op; 8 consecutive elements per thread.
+// Generic vector op.  A block covers 2048 consecutive elements of one state;
+// thread t handles elements t + 256q (q < 8), so every load and store of the
+// combination and the reductions is coalesced.  The BF16 pack needs eight
+// consecutive elements per 16-byte store and goes through LDS.
 __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ descs, int n, int n3,
                                                   int blocks_per_state, double* __restrict__ partials,
                                                   PackSpec pack) {
   const VecDesc& D = descs[blockIdx.y];
-  const int e0 = (blockIdx.x * blockDim.x + threadIdx.x) * 8;
+  const int t0 = blockIdx.x * 2048;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
   const int kind = D.rkind;
-  double vals[8];
+  const bool do_pack = D.pack_col >= 0 && pack.mode != kPackNone && D.dst;
+  __shared__ float tile[2048];
+#pragma unroll 2
   for (int q = 0; q < 8; ++q) {
-    int e = e0 + q;
+    const int e = t0 + threadIdx.x + 256 * q;
     if (e >= n3) break;
     double v = 0.0;
     if (D.dst) {
@@ -118,7 +123,10 @@ __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ de
       v = D.post_scale * v;
       D.dst[e] = v;
     }
-    vals[q] = v;
+    if (do_pack) {
+      if (pack.mode == kPackF32) pack.f32[(int64_t)D.pack_col * pack.col_stride + e] = (float)v;
+      else tile[threadIdx.x + 256 * q] = (float)v;
+    }
     switch (kind) {
       case kRedNone:
         break;
@@ -175,25 +183,43 @@ __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ de
         break;
     }
   }
-  // pack the combination into the demag input of this state's column
-  if (D.pack_col >= 0 && pack.mode != kPackNone && D.dst) {
-    for (int q = 0; q < 8; ++q) {
-      int e = e0 + q;
-      if (e >= n3) break;
-      int c = e / n, j = e - c * n;
-      float f = (float)vals[q];
-      if (pack.mode == kPackF32) {
-        pack.f32[(int64_t)D.pack_col * pack.col_stride + e] = f;
-      } else {
-        uint16_t hi = bf16_bits(f);
-        uint16_t lo = bf16_bits(f - bf16_value(hi));
-        int J = j / kTileK, half = (j % kTileK) / 8;
+  if (do_pack && pack.mode == kPackBf16) {
+    __syncthreads();
+    const int l0 = threadIdx.x * 8;  // eight consecutive elements of this block's tile
+    const int e0 = t0 + l0;
+    if (e0 < n3) {
+      if ((n & 7) == 0 && e0 + 8 <= n3) {
+        const int c = e0 / n, j0 = e0 - c * n;
+        uint32_t hw[4], lw[4];
+        for (int q = 0; q < 4; ++q) {
+          float f0 = tile[l0 + 2 * q], f1 = tile[l0 + 2 * q + 1];
+          uint16_t h0 = bf16_bits(f0), h1 = bf16_bits(f1);
+          uint16_t g0 = bf16_bits(f0 - bf16_value(h0)), g1 = bf16_bits(f1 - bf16_value(h1));
+          hw[q] = h0 | ((uint32_t)h1 << 16);
+          lw[q] = g0 | ((uint32_t)g1 << 16);
+        }
+        int J = j0 / kTileK, half = (j0 % kTileK) / 8;
         int lane = (D.pack_col & 31) + 32 * half;
         int sb = D.pack_col / 32;
         int64_t frag = (((int64_t)J * 3 + c) * 2 + sb) * 2;
-        uint16_t* p = reinterpret_cast<uint16_t*>(pack.bf16 + frag * kFragBytes + lane * 16) + (j % 8);
-        p[0] = hi;
-        p[kFragBytes / 2] = lo;
+        uint4* p = reinterpret_cast<uint4*>(pack.bf16 + frag * kFragBytes + lane * 16);
+        p[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        p[kFragBytes / 16] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+      } else {
+        for (int q = 0; q < 8 && e0 + q < n3; ++q) {
+          int e = e0 + q;
+          int c = e / n, j = e - c * n;
+          float f = tile[l0 + q];
+          uint16_t hi = bf16_bits(f);
+          uint16_t lo = bf16_bits(f - bf16_value(hi));
+          int J = j / kTileK, half = (j % kTileK) / 8;
+          int lane = (D.pack_col & 31) + 32 * half;
+          int sb = D.pack_col / 32;
+          int64_t frag = (((int64_t)J * 3 + c) * 2 + sb) * 2;
+          uint16_t* p = reinterpret_cast<uint16_t*>(pack.bf16 + frag * kFragBytes + lane * 16) + (j % 8);
+          p[0] = hi;
+          p[kFragBytes / 2] = lo;
+        }
       }
     }
   }

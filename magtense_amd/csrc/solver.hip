@@ -307,6 +307,7 @@ struct Solver {
   long long ticks = 0, eval_ticks = 0, demag_passes = 0;
   double gemm_ms = 0.0;
   bool timing = false;
+  long long total_target = 0;   // > 0: stop once the states' accepted steps sum to this
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   ~Solver() { release(); }
@@ -1055,6 +1056,11 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
 int Solver::run_until_idle() {
   std::vector<Sim*> evals, vecs;
   for (;;) {
+    if (total_target > 0) {
+      long long acc = 0;
+      for (const Sim& s : sims) acc += s.steps_run;
+      if (acc >= total_target) break;
+    }
     evals.clear();
     vecs.clear();
     for (Sim& s : sims) {
@@ -1301,6 +1307,30 @@ int mt_solver_advance(mt_solver* h, long long steps) {
 }
 
 int mt_solver_run(mt_solver* h) { return mt_solver_advance(h, 0); }
+
+int mt_solver_advance_total(mt_solver* h, long long total_steps) {
+  using namespace mt;
+  if (!h || total_steps <= 0) {
+    set_error("mt_solver_advance_total: null solver or non-positive step count");
+    return kErrArg;
+  }
+  Solver& P = *reinterpret_cast<Solver*>(h);
+  long long acc = 0;
+  for (Sim& s : P.sims) {
+    s.budget = 0;
+    acc += s.steps_run;
+  }
+  P.total_target = acc + total_steps;
+  int rc = P.run_until_idle();
+  P.total_target = 0;
+  return rc;
+}
+
+long long mt_solver_steps_total(mt_solver* h) {
+  long long acc = 0;
+  for (const mt::Sim& s : reinterpret_cast<Solver*>(h)->sims) acc += s.steps_run;
+  return acc;
+}
 
 int mt_solver_results(mt_solver* h, double* t_out, double* y_out, double* fields, mt_stats* stats) {
   using namespace mt;
