@@ -256,6 +256,9 @@ def main():
         del keep2
         plain = bs2 / wall2
         log(f"fp32 plain: {args.plain_steps} steps in {wall2:.3f} s = {plain:.4f} steps/s")
+    rd, cp = C.c_double(0), C.c_double(0)
+    _lib.check(L.mt_measure_bandwidth(8 << 30, 10, C.byref(rd), C.byref(cp)))
+    log(f"measured streaming bandwidth: read {rd.value:.0f} GB/s, copy {cp.value:.0f} GB/s")
     ferr = None if args.no_field_error else field_error_vs_fp64(W)
     log(f"field error vs FP64: {ferr}")
     cpu = None if args.no_cpu else cpu_baseline(W)
@@ -284,8 +287,9 @@ def main():
         "field_error_vs_fp64": ferr,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "gemm_bf16x2_kernel", "kernel_ms": gemm_ms,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "kernel": "gemm_bf16x2_lds_kernel", "kernel_ms": gemm_ms,
+                     "alg_bytes_per_launch": alg_bytes,
+                     "measured_read_peak": rd.value, "frac_of_measured_read_peak": achieved / rd.value},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -41,6 +41,13 @@ int mt_version(void);
 /* Number of visible HIP devices (0 when none); never fails. */
 int mt_device_count(void);
 
+/* Measures the device's streaming bandwidth on a `bytes` buffer (grid-stride
+ * 16-byte loads; copy counts bytes read + written), averaged over `iters`
+ * launches.  No reference counterpart: it gives the measured ceiling beside the
+ * published 8 TB/s and the known-size launches that calibrate HBM counters.
+ * Either output may be null. */
+int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* copy_gbps);
+
 /* ---- (a) prism tensor ----------------------------------------------------------
  * K = -N_standard (MagTense sign) of prism j (centre src[3j..], full edges
  * abc[abc_stride*j ..]) at point i (pts[3i..]); all arrays are device pointers,
@@ -138,6 +145,21 @@ int mt_solver_set_timing(mt_solver* s, int on);
 void mt_solver_destroy(mt_solver* s);
 int mt_run_micromag(const mt_micromag_problem* problem, double* t_out, double* y_out, double* fields,
                     mt_stats* stats);
+
+/* ---- differentiable integration (no reference counterpart) -------------------------
+ * Fixed-step RK4 of the same LL right-hand side for n_states states (constant
+ * applied field per state: the first row of hext; alpha0, gamma, Ms, K0, A0,
+ * u_ea and the tensor as in mt_micromag_problem; plain FP32 demag product).
+ * mt_rk4_forward stores checkpoints every checkpoint_every steps;
+ * mt_rk4_backward runs the hand-written discrete adjoint (recomputing each
+ * checkpoint segment) for the cotangent dL/dm(T) and returns dL/dm0
+ * (n_states*3n), dL/dH_ext (n_states*3, A/m) and dL/dMs (n_states*n).  Host pointers. */
+typedef struct mt_rk4 mt_rk4;
+int mt_rk4_create(const mt_micromag_problem* problem, double dt, long long nsteps, int checkpoint_every,
+                  mt_rk4** out);
+int mt_rk4_forward(mt_rk4* h, const double* m0, double* mT);
+int mt_rk4_backward(mt_rk4* h, const double* dL_dmT, double* grad_m0, double* grad_hext, double* grad_ms);
+void mt_rk4_destroy(mt_rk4* h);
 
 /* ---- reference plug-in ABI (MagTenseCudaBlasICLWrapper.cxx:13-59) -------------------
  * Host arrays, column-major N x N FP32 blocks, scalars by reference, one

@@ -76,6 +76,21 @@ __global__ void scale_kernel(float* __restrict__ h, int64_t total, float pref) {
   if (t < total) h[t] *= pref;
 }
 
+// streaming kernels used to measure the achievable HBM bandwidth on the device
+__global__ void read_bw_kernel(const uint4* __restrict__ p, int64_t n16, uint32_t* __restrict__ sink) {
+  uint32_t acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x) {
+    uint4 v = p[i];
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the loads; practically never taken
+}
+
+__global__ void copy_bw_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 int device_ok() {
   int nd = 0;
   if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) {
@@ -116,6 +131,48 @@ int mt_device_count(void) {
 }
 long long mt_split_tensor_bytes(int n) { return split_tensor_bytes(n); }
 long long mt_packed_m_bytes(int n) { return packed_m_bytes(n); }
+
+int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* copy_gbps) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(bytes >= (1 << 20) && iters >= 1, "need bytes >= 1 MiB and iters >= 1");
+  const int64_t n16 = bytes / 16;
+  uint4 *a = nullptr, *b = nullptr;
+  uint32_t* sink = nullptr;
+  MT_HIP(hipMalloc(&a, n16 * 16));
+  MT_HIP(hipMalloc(&sink, 64));
+  if (copy_gbps) MT_HIP(hipMalloc(&b, n16 * 16));
+  MT_HIP(hipMemset(a, 0x11, n16 * 16));
+  hipEvent_t e0, e1;
+  MT_HIP(hipEventCreate(&e0));
+  MT_HIP(hipEventCreate(&e1));
+  const dim3 grid(2048), block(256);
+  float ms = 0;
+  if (read_gbps) {
+    hipLaunchKernelGGL(read_bw_kernel, grid, block, 0, nullptr, a, n16, sink);
+    MT_HIP(hipEventRecord(e0, nullptr));
+    for (int k = 0; k < iters; ++k) hipLaunchKernelGGL(read_bw_kernel, grid, block, 0, nullptr, a, n16, sink);
+    MT_HIP(hipEventRecord(e1, nullptr));
+    MT_HIP(hipEventSynchronize(e1));
+    MT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *read_gbps = (double)n16 * 16 * iters / (ms * 1e-3) / 1e9;
+  }
+  if (copy_gbps) {
+    hipLaunchKernelGGL(copy_bw_kernel, grid, block, 0, nullptr, a, b, n16);
+    MT_HIP(hipEventRecord(e0, nullptr));
+    for (int k = 0; k < iters; ++k) hipLaunchKernelGGL(copy_bw_kernel, grid, block, 0, nullptr, a, b, n16);
+    MT_HIP(hipEventRecord(e1, nullptr));
+    MT_HIP(hipEventSynchronize(e1));
+    MT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *copy_gbps = 2.0 * (double)n16 * 16 * iters / (ms * 1e-3) / 1e9;  // bytes read + written
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(sink);
+  if (b) (void)hipFree(b);
+  return kOk;
+}
 
 int mt_prism_tensor(const double* src, const double* abc, int abc_stride, const double* pts, int n_src, int n_pts,
                     float* out32, double* out64, void* split) {

@@ -1351,3 +1351,174 @@ int or_demag_rows(int n, int R, const float* Krows, const float* ms_rows, const 
 }
 
 }  // extern "C"
+
+// ============================================================================
+// FP64 discrete adjoint of fixed-step RK4 (checker for the GPU adjoint).
+// Operators are transposed explicitly (CSR scatter, dense block transpose), so
+// the symmetry the GPU kernels rely on is checked rather than assumed.
+// ============================================================================
+namespace {
+
+struct AdjCtx {
+  Problem* P;
+  double hext[3];
+  std::vector<double> Hm64;
+};
+
+// negated total field H(m) and its pieces, FP64 (demag from the FP64 tensor when
+// demag_fp64, else the reference's FP32 semantics)
+void total_field(AdjCtx& A, const double* m, std::vector<double>& H, std::vector<double>& hm, std::vector<double>& hj,
+                 std::vector<double>& hk) {
+  Problem& P = *A.P;
+  int n = P.n;
+  P.hext_x.assign(2, A.hext[0]); P.hext_y.assign(2, A.hext[1]); P.hext_z.assign(2, A.hext[2]);
+  P.hext_t = {0.0, 1.0};
+  P.solver = 2;
+  update_fields(P, m, 0.0);
+  if (P.demag_fp64) demag_fp64(P, m, A.Hm64.data());
+  H.resize(3 * (size_t)n); hm.resize(3 * (size_t)n); hj.resize(3 * (size_t)n); hk.resize(3 * (size_t)n);
+  for (size_t k = 0; k < 3 * (size_t)n; ++k) {
+    hm[k] = P.demag_fp64 ? A.Hm64[k] : (double)P.Hm[k];
+    hj[k] = P.Hj[k];
+    hk[k] = P.Hk[k];
+    H[k] = -A.hext[k / n] + hj[k] + hm[k] + hk[k];
+  }
+}
+
+void rhs_adj(AdjCtx& A, const double* m, double* out) {
+  Problem& P = *A.P;
+  int n = P.n;
+  std::vector<double> H, hm, hj, hk;
+  total_field(A, m, H, hm, hj, hk);
+  for (int r = 0; r < n; ++r) {
+    double mx = m[r], my = m[n + r], mz = m[2 * n + r];
+    double Hx = H[r], Hy = H[n + r], Hz = H[2 * n + r];
+    double cx = -(my * Hz - mz * Hy), cy = -(mz * Hx - mx * Hz), cz = -(mx * Hy - my * Hx);
+    out[r] = -P.gamma * cx - P.alpha0 * (my * cz - mz * cy);
+    out[n + r] = -P.gamma * cy - P.alpha0 * (mz * cx - mx * cz);
+    out[2 * n + r] = -P.gamma * cz - P.alpha0 * (mx * cy - my * cx);
+  }
+}
+
+// mu = (df/dm)^T u ; gh += dL/dhext ; gms += dL/dMs
+void vjp_adj(AdjCtx& A, const double* m, const double* u, double* mu, double* gh, double* gms) {
+  Problem& P = *A.P;
+  int n = P.n;
+  std::vector<double> H, hm, hj, hk;
+  total_field(A, m, H, hm, hj, hk);
+  std::vector<double> hb(3 * (size_t)n);
+  for (int r = 0; r < n; ++r) {
+    double mm[3] = {m[r], m[n + r], m[2 * n + r]}, uu[3] = {u[r], u[n + r], u[2 * n + r]};
+    double HH[3] = {H[r], H[n + r], H[2 * n + r]};
+    double c[3] = {-(mm[1] * HH[2] - mm[2] * HH[1]), -(mm[2] * HH[0] - mm[0] * HH[2]), -(mm[0] * HH[1] - mm[1] * HH[0])};
+    double uxm[3] = {uu[1] * mm[2] - uu[2] * mm[1], uu[2] * mm[0] - uu[0] * mm[2], uu[0] * mm[1] - uu[1] * mm[0]};
+    double cb[3], cxu[3], hxc[3];
+    for (int k = 0; k < 3; ++k) cb[k] = -P.gamma * uu[k] - P.alpha0 * uxm[k];
+    cxu[0] = c[1] * uu[2] - c[2] * uu[1]; cxu[1] = c[2] * uu[0] - c[0] * uu[2]; cxu[2] = c[0] * uu[1] - c[1] * uu[0];
+    hxc[0] = HH[1] * cb[2] - HH[2] * cb[1]; hxc[1] = HH[2] * cb[0] - HH[0] * cb[2]; hxc[2] = HH[0] * cb[1] - HH[1] * cb[0];
+    double hbr[3] = {-(cb[1] * mm[2] - cb[2] * mm[1]), -(cb[2] * mm[0] - cb[0] * mm[2]), -(cb[0] * mm[1] - cb[1] * mm[0])};
+    double ms = (double)P.Mfact[r];
+    double g = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      mu[(size_t)k * n + r] = -P.alpha0 * cxu[k] - hxc[k];
+      hb[(size_t)k * n + r] = hbr[k];
+      gh[k] += -hbr[k];
+      g += hbr[k] * (hm[(size_t)k * n + r] - hj[(size_t)k * n + r] - hk[(size_t)k * n + r]);
+    }
+    gms[r] += g / ms;
+  }
+  // exchange transpose: H_j = -2 Jfact (A m)  ->  mu += A^T (-2 Jfact hb)
+  for (int c3 = 0; c3 < 3; ++c3)
+    for (int r = 0; r < n; ++r) {
+      double w = -2.0 * P.Jfact[r] * hb[(size_t)c3 * n + r];
+      for (int q = P.Aex.rowptr[r]; q < P.Aex.rowptr[r + 1]; ++q) mu[(size_t)c3 * n + P.Aex.col[q]] += P.Aex.val[q] * w;
+    }
+  // demag transpose: H_m,a(i) = -Ms_i sum_b K_ab(i,j) m_b(j)  ->  mu_b(j) += -sum_a,i K_ab(i,j) Ms_i hb_a(i)
+  static const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  size_t nn = (size_t)n * n;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      int c6 = comp[a][b];
+      for (int i = 0; i < n; ++i) {
+        double w = -(double)P.Mfact[i] * hb[(size_t)a * n + i];
+        if (P.demag_fp64) {
+          const double* Krow = P.Kd.data() + c6 * nn + (size_t)i * n;
+          for (int j = 0; j < n; ++j) mu[(size_t)b * n + j] += Krow[j] * w;
+        } else {
+          const float* Krow = P.K.data() + c6 * nn + (size_t)i * n;
+          for (int j = 0; j < n; ++j) mu[(size_t)b * n + j] += (double)Krow[j] * w;
+        }
+      }
+    }
+  // anisotropy transpose (symmetric 3x3 per cell): mu += -2 Kfact A_ani hb
+  for (int r = 0; r < n; ++r) {
+    double f = -2.0 * P.Kfact[r];
+    double h0 = hb[r], h1 = hb[n + r], h2 = hb[2 * n + r];
+    mu[r] += f * (P.Axx[r] * h0 + P.Axy[r] * h1 + P.Axz[r] * h2);
+    mu[n + r] += f * (P.Axy[r] * h0 + P.Ayy[r] * h1 + P.Ayz[r] * h2);
+    mu[2 * n + r] += f * (P.Axz[r] * h0 + P.Ayz[r] * h1 + P.Azz[r] * h2);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// RK4 forward (mT) and, when dL_dmT is given, the discrete adjoint.  Uses
+// q's grid, materials, alpha0, gamma and demag_fp64; the applied field is hext3.
+int or_rk4_adjoint(const or_problem* q, double dt, long long nsteps, const double* hext3, const double* m0, double* mT,
+                   const double* dL_dmT, double* grad_m0, double* grad_hext, double* grad_ms) {
+  Problem P;
+  int rc = build_problem(q, P);
+  if (rc) return rc;
+  int n = P.n;
+  size_t n3 = 3 * (size_t)n;
+  AdjCtx A;
+  A.P = &P;
+  A.hext[0] = hext3[0]; A.hext[1] = hext3[1]; A.hext[2] = hext3[2];
+  A.Hm64.assign(n3, 0.0);
+  std::vector<double> Y((size_t)(nsteps + 1) * n3);
+  std::vector<double> k1(n3), k2(n3), k3(n3), k4(n3), tmp(n3);
+  std::memcpy(Y.data(), m0, n3 * sizeof(double));
+  for (long long s = 0; s < nsteps; ++s) {
+    const double* y = Y.data() + (size_t)s * n3;
+    double* yn = Y.data() + (size_t)(s + 1) * n3;
+    rhs_adj(A, y, k1.data());
+    for (size_t k = 0; k < n3; ++k) tmp[k] = y[k] + 0.5 * dt * k1[k];
+    rhs_adj(A, tmp.data(), k2.data());
+    for (size_t k = 0; k < n3; ++k) tmp[k] = y[k] + 0.5 * dt * k2[k];
+    rhs_adj(A, tmp.data(), k3.data());
+    for (size_t k = 0; k < n3; ++k) tmp[k] = y[k] + dt * k3[k];
+    rhs_adj(A, tmp.data(), k4.data());
+    for (size_t k = 0; k < n3; ++k) yn[k] = y[k] + dt / 6.0 * (k1[k] + 2.0 * k2[k] + 2.0 * k3[k] + k4[k]);
+  }
+  if (mT) std::memcpy(mT, Y.data() + (size_t)nsteps * n3, n3 * sizeof(double));
+  if (!dL_dmT) return 0;
+  std::vector<double> lam(dL_dmT, dL_dmT + n3), kb(n3), mu1(n3), mu2(n3), mu3(n3), mu4(n3);
+  std::vector<double> Y2(n3), Y3(n3), Y4(n3);
+  std::vector<double> gh(3, 0.0), gms(n, 0.0);
+  for (long long s = nsteps - 1; s >= 0; --s) {
+    const double* y = Y.data() + (size_t)s * n3;
+    rhs_adj(A, y, k1.data());
+    for (size_t k = 0; k < n3; ++k) Y2[k] = y[k] + 0.5 * dt * k1[k];
+    rhs_adj(A, Y2.data(), k2.data());
+    for (size_t k = 0; k < n3; ++k) Y3[k] = y[k] + 0.5 * dt * k2[k];
+    rhs_adj(A, Y3.data(), k3.data());
+    for (size_t k = 0; k < n3; ++k) Y4[k] = y[k] + dt * k3[k];
+    for (size_t k = 0; k < n3; ++k) kb[k] = dt / 6.0 * lam[k];
+    vjp_adj(A, Y4.data(), kb.data(), mu4.data(), gh.data(), gms.data());
+    for (size_t k = 0; k < n3; ++k) kb[k] = dt / 3.0 * lam[k] + dt * mu4[k];
+    vjp_adj(A, Y3.data(), kb.data(), mu3.data(), gh.data(), gms.data());
+    for (size_t k = 0; k < n3; ++k) kb[k] = dt / 3.0 * lam[k] + 0.5 * dt * mu3[k];
+    vjp_adj(A, Y2.data(), kb.data(), mu2.data(), gh.data(), gms.data());
+    for (size_t k = 0; k < n3; ++k) kb[k] = dt / 6.0 * lam[k] + 0.5 * dt * mu2[k];
+    vjp_adj(A, y, kb.data(), mu1.data(), gh.data(), gms.data());
+    for (size_t k = 0; k < n3; ++k) lam[k] += mu1[k] + mu2[k] + mu3[k] + mu4[k];
+  }
+  if (grad_m0) std::memcpy(grad_m0, lam.data(), n3 * sizeof(double));
+  if (grad_hext) std::memcpy(grad_hext, gh.data(), 3 * sizeof(double));
+  if (grad_ms) std::memcpy(grad_ms, gms.data(), n * sizeof(double));
+  return 0;
+}
+
+}  // extern "C"

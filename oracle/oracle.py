@@ -70,6 +70,8 @@ def lib():
                                          _dp, _dp, C.POINTER(OrStats)]
         _lib.or_rk_tables.argtypes = [_dp] * 5
         _lib.or_demag_rows.argtypes = [C.c_int, C.c_int, _fp, _fp, _dp, _dp]
+        _lib.or_rk4_adjoint.argtypes = [C.POINTER(OrProblem), C.c_double, C.c_longlong, _dp, _dp, _dp, _dp, _dp,
+                                        _dp, _dp]
     return _lib
 
 
@@ -172,6 +174,23 @@ class Problem:
         if rc:
             raise RuntimeError(f"or_dmdt failed: {rc}")
         return out, H  # H = [H_exc, H_ext, H_dem, H_ani] (negated fields, SoA)
+
+    def rk4_adjoint(self, dt, nsteps, hext3, m0, dL_dmT=None):
+        """RK4 forward (and discrete adjoint when dL_dmT is given); SoA vectors."""
+        hext3 = _c64(hext3)
+        m0 = _c64(m0).reshape(-1)
+        mT = np.zeros_like(m0)
+        out = [None, None, None]
+        if dL_dmT is not None:
+            g = _c64(dL_dmT).reshape(-1)
+            out = [np.zeros_like(m0), np.zeros(3), np.zeros(self.n)]
+            rc = lib().or_rk4_adjoint(C.byref(self.q), dt, nsteps, _d(hext3), _d(m0), _d(mT), _d(g),
+                                      *(_d(o) for o in out))
+        else:
+            rc = lib().or_rk4_adjoint(C.byref(self.q), dt, nsteps, _d(hext3), _d(m0), _d(mT), None, None, None, None)
+        if rc:
+            raise RuntimeError(f"or_rk4_adjoint failed: {rc}")
+        return mT, out[0], out[1], out[2]
 
     def run(self):
         nt = self.q.nt

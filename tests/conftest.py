@@ -18,3 +18,20 @@ def oracle():
 
     O.lib()
     return O
+
+
+def _init_torch_gpu_first():
+    """torch ships its own HIP runtime, which must initialise before
+    libmagtense_hip's when both drive the device in one process."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.zeros(1, device="cuda")
+    except Exception:  # pragma: no cover - no usable device
+        pass
+
+
+_init_torch_gpu_first()

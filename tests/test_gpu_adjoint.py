@@ -1,0 +1,90 @@
+"""GPU discrete adjoint of RK4 LL integration vs the FP64 oracle adjoint and vs
+central finite differences (h = 1e-4 relative), through torch.autograd."""
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+def _unit(rng, n):
+    m = rng.normal(size=(n, 3))
+    return m / np.linalg.norm(m, axis=1, keepdims=True)
+
+
+def _loss(mT, target):
+    return mT[..., 2].mean() + torch.linalg.norm(mT - target)
+
+
+def test_gpu_adjoint_matches_oracle_adjoint(oracle):
+    from magtense_amd.adjoint import ll_rk4
+
+    res, L = [8, 4, 2], [40e-9, 20e-9, 6e-9]
+    n = 64
+    rng = np.random.default_rng(1)
+    ms = rng.uniform(7e5, 9e5, size=n)
+    m0 = _unit(rng, n)
+    target = _unit(rng, n)
+    h = np.array([1.5e4, -3e3, 2e3])
+    dt, nsteps, ck = 5e-14, 50, 16  # the last checkpoint segment is partial
+    m0_t = torch.tensor(m0, requires_grad=True)
+    h_t = torch.tensor(h, requires_grad=True)
+    ms_t = torch.tensor(ms, requires_grad=True)
+    mT = ll_rk4(m0_t, h_t, ms_t, res, L, dt, nsteps, checkpoint_every=ck)
+    loss = _loss(mT, torch.tensor(target))
+    loss.backward()
+    # oracle with the same FP32 demag semantics
+    q = oracle.Problem(res, L, Ms=ms, gamma=2.21e5, alpha=0.02 * 2.21e5)
+    soa = lambda a: np.concatenate([a[:, 0], a[:, 1], a[:, 2]])  # noqa: E731
+    mT_o, *_ = q.rk4_adjoint(dt, nsteps, h, soa(m0))
+    mT_o_aos = mT_o.reshape(3, n).T
+    assert np.abs(mT.detach().numpy() - mT_o_aos).max() < 1e-6
+    d = (mT_o_aos - target) / np.linalg.norm(mT_o_aos - target)
+    d[:, 2] += 1.0 / n
+    _, gm0, gh, gms = q.rk4_adjoint(dt, nsteps, h, soa(m0), soa(d))
+    assert np.allclose(h_t.grad.numpy(), gh, rtol=1e-5, atol=1e-6 * np.abs(gh).max())
+    assert np.abs(ms_t.grad.numpy() - gms).max() <= 1e-5 * np.abs(gms).max()
+    assert np.abs(m0_t.grad.numpy() - gm0.reshape(3, n).T).max() <= 1e-5 * np.abs(gm0).max()
+
+
+def test_gpu_adjoint_vs_finite_differences_c5():
+    """128x32x1 film, 2000 RK4 steps of 0.1 ps, checkpoints every 100 steps;
+    loss = mean m_z(T) + |m(T) - m_target|; gradients w.r.t. H_ext (3) and Ms (4096)."""
+    from magtense_amd.adjoint import ll_rk4
+
+    res, L = [128, 32, 1], [500e-9, 125e-9, 3e-9]
+    n = 4096
+    rng = np.random.default_rng(2024)
+    ms = rng.uniform(7e5, 9e5, size=n)
+    m0 = _unit(rng, n)
+    target = torch.tensor(_unit(rng, n))
+    h = np.array([-24.6, 4.3, 0.0]) / 1000 / (4e-7 * np.pi)
+    dt, nsteps = 1e-13, 2000
+
+    def loss_of(h_, ms_):
+        with torch.no_grad():
+            mT = ll_rk4(torch.tensor(m0), torch.tensor(h_), torch.tensor(ms_), res, L, dt, nsteps)
+            return float(_loss(mT, target))
+
+    h_t = torch.tensor(h, requires_grad=True)
+    ms_t = torch.tensor(ms, requires_grad=True)
+    mT = ll_rk4(torch.tensor(m0), h_t, ms_t, res, L, dt, nsteps, checkpoint_every=100)
+    _loss(mT, target).backward()
+    gh = h_t.grad.numpy()
+    gms = ms_t.grad.numpy()
+    scale = np.abs(h).max()
+    errs = []
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = 1e-4 * scale
+        fd = (loss_of(h + e, ms) - loss_of(h - e, ms)) / (2 * e[k])
+        errs.append(abs(fd - gh[k]) / max(abs(fd), 1e-3 * np.abs(gh).max()))
+    v = rng.normal(size=n)
+    v /= np.linalg.norm(v)
+    step = 1e-4 * ms.mean()
+    fd = (loss_of(h, ms + step * v) - loss_of(h, ms - step * v)) / (2 * step)
+    errs.append(abs(fd - gms @ v) / abs(fd))
+    print("relative adjoint-vs-FD errors", errs)
+    assert max(errs) < 1e-4, errs
