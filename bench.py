@@ -132,11 +132,11 @@ def field_error_vs_fp64(W, n_rows=1024, n_states=8, seed=7):
     dsub = torch.as_tensor(np.ascontiguousarray(pts[rows])).cuda()
     dabc = torch.as_tensor(np.array(d)).cuda()
     K64 = torch.empty(6 * n_rows * n, dtype=torch.float64, device="cuda")
-    _lib.check(L.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, dsub.data_ptr(), n, n_rows, None,
+    _lib.check(L.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, None, dsub.data_ptr(), n, n_rows, None,
                                  K64.data_ptr(), None))
     K32 = torch.empty(6 * n * n, dtype=torch.float32, device="cuda")
     split = torch.empty(L.mt_split_tensor_bytes(n), dtype=torch.uint8, device="cuda")
-    _lib.check(L.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, dp.data_ptr(), n, n, K32.data_ptr(), None,
+    _lib.check(L.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, None, dp.data_ptr(), n, n, K32.data_ptr(), None,
                                  split.data_ptr()))
     m = W["m0"][:n_states].reshape(n_states, 3, n)
     dm = torch.as_tensor(m.astype(np.float32)).cuda()

@@ -18,7 +18,9 @@
  *   - prism tensor getNFromTiles / ComputeDemagfieldTensor
  *       python/src/magtense/lib/FortranToPythonIO.f90:25-93,
  *       source/MagTenseMicroMag/LandauLifshitzEquationSolver.f90:883-1164
- *     -> mt_prism_tensor
+ *     -> mt_prism_tensor (device arrays), mt_tiles_demag_tensor (host arrays)
+ *   - getHFromTiles  FortranToPythonIO.f90:106-223
+ *     -> mt_tiles_h_field
  *   - demag product updateDemagfield / cu_MVMult_GetH
  *       LandauLifshitzEquationSolver.f90:558-779, MagTenseCudaBlas.cu:367-407
  *     -> mt_demag_apply_f32, mt_demag_apply_bf16x2
@@ -50,16 +52,49 @@ int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* 
 
 /* ---- (a) prism tensor ----------------------------------------------------------
  * K = -N_standard (MagTense sign) of prism j (centre src[3j..], full edges
- * abc[abc_stride*j ..]) at point i (pts[3i..]); all arrays are device pointers,
- * row-major.  Outputs (any may be null):
+ * abc[abc_stride*j ..]) at point i (pts[3i..]), evaluated at rot_j (p_i - c_j)
+ * when rot (9 doubles per prism, row-major rotMat of getRotationMatrices,
+ * DemagFieldGetSolution.f90:925-948) is given, else at p_i - c_j.  All arrays
+ * are device pointers, row-major.  Outputs (any may be null):
  *   out32[c*n_pts*n_src + i*n_src + j]  FP32, c = xx,xy,xz,yy,yz,zz
  *   out64                               same layout, FP64
  *   split                               fragment-tiled BF16 hi/lo (square only,
  *                                       mt_split_tensor_bytes(n) bytes)
  * Replaces getN_prism_3D over all pairs (TileNComponents.f90:629-724 and
  * LandauLifshitzEquationSolver.f90:935-984 / 1044-1133). */
-int mt_prism_tensor(const double* src, const double* abc, int abc_stride, const double* pts, int n_src, int n_pts,
-                    float* out32, double* out64, void* split);
+int mt_prism_tensor(const double* src, const double* abc, int abc_stride, const double* rot, const double* pts,
+                    int n_src, int n_pts, float* out32, double* out64, void* split);
+
+/* ---- magnetostatic prism tiles (host arrays) --------------------------------
+ * Mirrors the f2py arguments of getNFromTiles / getHFromTiles
+ * (FortranToPythonIO.f90:25-93,106-223) that matter for rectangular prisms.
+ * Every (n,3) array is Fortran-ordered as f2py passes it: element (i,k) at
+ * i + n*k.  Optional members may be null (rot: no rotation; tile_type: all
+ * prisms; use_sym: 0; sym_ops: 1; M: only needed for the field).  A tile
+ * whose tile_type is not 2 (prism) fails with MT_ERR_UNSUPPORTED. */
+typedef struct {
+  int n_tiles;
+  const double* size;    /* tile_size (a,b,c), full edges */
+  const double* offset;  /* tile centre */
+  const double* rot;     /* rotAngles, getRotationMatrices (DemagFieldGetSolution.f90:925-948) */
+  const double* M;       /* Mag */
+  const int* tile_type;  /* tileType */
+  const int* use_sym;    /* exploitSymmetry (1 = eight mirrored evaluations, :744-822) */
+  const double* sym_ops; /* symmetryOps */
+} mt_tiles;
+
+#define MT_ERR_UNSUPPORTED (-6)
+
+/* N(n_tiles, n_pts, 3, 3), Fortran order ((i,p,a,b) at i + nt*(p + np*(a + 3b))),
+ * of every tile at every point (pts: (n_pts,3) Fortran order), in each tile's
+ * rotated frame, MagTense sign (getFieldFromTile, DemagFieldGetSolution.f90:685-730). */
+int mt_tiles_demag_tensor(const mt_tiles* tiles, int n_pts, const double* pts, double* N);
+
+/* H(n_pts,3), Fortran order: sum over tiles of Rinv (N (R M)) (getHFromTiles).
+ * When N is non-null it is the stored tensor of mt_tiles_demag_tensor
+ * (useStoredN); the sum over tiles is a fixed-order tree, not the reference's
+ * sequential loop (differences at FP64 rounding level). */
+int mt_tiles_h_field(const mt_tiles* tiles, int n_pts, const double* pts, const double* N, double* H);
 long long mt_split_tensor_bytes(int n);
 long long mt_packed_m_bytes(int n);
 /* FP32 receiver-major tensor (device) -> split BF16 fragment layout (device). */

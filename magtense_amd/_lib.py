@@ -53,14 +53,22 @@ class MtStats(C.Structure):
     ]
 
 
+class MtTiles(C.Structure):
+    _fields_ = [
+        ("n_tiles", C.c_int),
+        ("size", _dp), ("offset", _dp), ("rot", _dp), ("M", _dp),
+        ("tile_type", C.POINTER(C.c_int)), ("use_sym", C.POINTER(C.c_int)), ("sym_ops", _dp),
+    ]
+
+
 # every symbol declared in include/magtense_hip.h
 EXPORTS = [
     "mt_last_error", "mt_version", "mt_device_count", "mt_measure_bandwidth", "mt_prism_tensor", "mt_split_tensor_bytes",
     "mt_packed_m_bytes", "mt_split_tensor", "mt_demag_apply_f32", "mt_demag_apply_bf16x2", "mt_ll_rhs",
     "mt_solver_create", "mt_solver_advance", "mt_solver_run", "mt_solver_advance_total", "mt_solver_steps_total", "mt_solver_results", "mt_solver_state",
     "mt_solver_set_timing", "mt_solver_destroy", "mt_run_micromag", "mt_rk4_create", "mt_rk4_forward",
-    "mt_rk4_backward", "mt_rk4_destroy", "icl_initDemagMatrices", "icl_MVMult_GetH",
-    "icl_destroy",
+    "mt_rk4_backward", "mt_rk4_destroy", "mt_tiles_demag_tensor", "mt_tiles_h_field", "icl_initDemagMatrices",
+    "icl_MVMult_GetH", "icl_destroy",
 ]
 
 _lib = None
@@ -89,7 +97,7 @@ def load():
     lib.mt_packed_m_bytes.restype = C.c_longlong
     vp = C.c_void_p
     lib.mt_measure_bandwidth.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-    lib.mt_prism_tensor.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp]
+    lib.mt_prism_tensor.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp]
     lib.mt_split_tensor.argtypes = [vp, C.c_int, vp]
     lib.mt_demag_apply_f32.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_demag_apply_bf16x2.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_float]
@@ -112,6 +120,8 @@ def load():
     lib.mt_rk4_backward.argtypes = [vp, vp, vp, vp, vp]
     lib.mt_rk4_destroy.argtypes = [vp]
     lib.mt_rk4_destroy.restype = None
+    lib.mt_tiles_demag_tensor.argtypes = [C.POINTER(MtTiles), C.c_int, vp, vp]
+    lib.mt_tiles_h_field.argtypes = [C.POINTER(MtTiles), C.c_int, vp, vp, vp]
     lib.icl_initDemagMatrices.argtypes = [vp] * 6 + [C.POINTER(C.c_int)]
     lib.icl_initDemagMatrices.restype = None
     lib.icl_MVMult_GetH.argtypes = [vp] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_float)]

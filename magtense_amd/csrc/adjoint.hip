@@ -565,7 +565,7 @@ int mt_rk4_create(const mt_micromag_problem* q, double dt, long long nsteps, int
     MT_HIP(hipMalloc(&da, sizeof(abc)));
     MT_HIP(hipMemcpyAsync(dp, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
     MT_HIP(hipMemcpyAsync(da, abc, sizeof(abc), hipMemcpyHostToDevice, P.stream));
-    int rc = launch_assemble(dp, da, 0, dp, n, n, P.K32, nullptr, nullptr, P.stream);
+    int rc = launch_assemble(dp, da, 0, nullptr, dp, n, n, P.K32, nullptr, nullptr, P.stream);
     if (rc) return rc;
     MT_HIP(hipStreamSynchronize(P.stream));
     (void)hipFree(dp);

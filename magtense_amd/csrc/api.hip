@@ -1,6 +1,7 @@
 // C ABI: library utilities, the stand-alone kernel entry points and the
 // reference plug-in ABI (icl_*, MagTenseCudaBlasICLWrapper.cxx:13-59).
 
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -174,13 +175,13 @@ int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* 
   return kOk;
 }
 
-int mt_prism_tensor(const double* src, const double* abc, int abc_stride, const double* pts, int n_src, int n_pts,
-                    float* out32, double* out64, void* split) {
+int mt_prism_tensor(const double* src, const double* abc, int abc_stride, const double* rot, const double* pts,
+                    int n_src, int n_pts, float* out32, double* out64, void* split) {
   int rc = device_ok();
   if (rc) return rc;
   MT_CHECK(n_src > 0 && n_pts > 0 && src && abc && pts, "invalid tensor arguments");
   MT_CHECK(abc_stride == 0 || abc_stride == 3, "abc_stride must be 0 (one size for all) or 3");
-  rc = launch_assemble(src, abc, abc_stride, pts, n_src, n_pts, out32, out64, (uint8_t*)split, nullptr);
+  rc = launch_assemble(src, abc, abc_stride, rot, pts, n_src, n_pts, out32, out64, (uint8_t*)split, nullptr);
   if (rc) return rc;
   MT_HIP(hipStreamSynchronize(nullptr));
   return kOk;
@@ -334,6 +335,118 @@ int mt_ll_rhs(int nx, int ny, int nz, double Lx, double Ly, double Lz, const dou
   (void)hipFree(dd); (void)hipFree(scratch); (void)hipFree(dvd);
   if (dan) (void)hipFree(dan);
   return rc;
+}
+
+// ---- magnetostatic prism tiles (getNFromTiles / getHFromTiles) ---------------
+}  // extern "C"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+void mat3_mul(const double* A, const double* B, double* C) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
+}
+
+// getRotationMatrices (DemagFieldGetSolution.f90:925-948): R = Rz(-t3) Ry(-t2)
+// Rx(-t1), Rinv = Rx^T Ry^T Rz^T; row-major.
+void rotation_matrices(const double ang[3], double* R, double* Ri) {
+  double ax = -ang[0], ay = -ang[1], az = -ang[2];
+  const double X[9] = {1, 0, 0, 0, std::cos(ax), -std::sin(ax), 0, std::sin(ax), std::cos(ax)};
+  const double Y[9] = {std::cos(ay), 0, std::sin(ay), 0, 1, 0, -std::sin(ay), 0, std::cos(ay)};
+  const double Z[9] = {std::cos(az), -std::sin(az), 0, std::sin(az), std::cos(az), 0, 0, 0, 1};
+  double ZY[9], Xt[9], Yt[9], Zt[9], XtYt[9];
+  mat3_mul(Z, Y, ZY);
+  mat3_mul(ZY, X, R);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) { Xt[3 * r + c] = X[3 * c + r]; Yt[3 * r + c] = Y[3 * c + r]; Zt[3 * r + c] = Z[3 * c + r]; }
+  mat3_mul(Xt, Yt, XtYt);
+  mat3_mul(XtYt, Zt, Ri);
+}
+
+// Fortran-ordered (n,3) host arrays -> device tile records of 32 doubles.
+int tile_records(const mt_tiles* t, bool need_m, std::vector<double>& rec) {
+  MT_CHECK(t && t->n_tiles > 0 && t->size && t->offset, "mt_tiles needs n_tiles > 0, size and offset");
+  MT_CHECK(!need_m || t->M, "mt_tiles_h_field needs the tile magnetisation M");
+  const int nt = t->n_tiles;
+  rec.assign((size_t)nt * 32, 0.0);
+  for (int i = 0; i < nt; ++i) {
+    if (t->tile_type && t->tile_type[i] != 2) {
+      set_error("tile " + std::to_string(i) + " has tile_type " + std::to_string(t->tile_type[i]) +
+                "; only rectangular prisms (tile_type 2) are supported");
+      return kErrUnsupported;
+    }
+    double* r = rec.data() + (size_t)i * 32;
+    double ang[3], M[3];
+    for (int k = 0; k < 3; ++k) {
+      r[k] = t->size[i + (size_t)nt * k];
+      r[3 + k] = t->offset[i + (size_t)nt * k];
+      ang[k] = t->rot ? t->rot[i + (size_t)nt * k] : 0.0;
+      M[k] = t->M ? t->M[i + (size_t)nt * k] : 0.0;
+      r[27 + k] = t->sym_ops ? t->sym_ops[i + (size_t)nt * k] : 1.0;
+    }
+    rotation_matrices(ang, r + 6, r + 15);
+    for (int a = 0; a < 3; ++a) r[24 + a] = r[6 + 3 * a] * M[0] + r[7 + 3 * a] * M[1] + r[8 + 3 * a] * M[2];
+    r[30] = (t->use_sym && t->use_sym[i] == 1) ? 1.0 : 0.0;
+  }
+  return kOk;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int mt_tiles_demag_tensor(const mt_tiles* tiles, int n_pts, const double* pts, double* N) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n_pts > 0 && pts && N, "invalid points or output");
+  std::vector<double> rec;
+  rc = tile_records(tiles, false, rec);
+  if (rc) return rc;
+  const int nt = tiles->n_tiles;
+  const size_t nbytes = (size_t)nt * n_pts * 9 * sizeof(double);
+  DevBuf drec, dpts, dN;
+  MT_HIP(hipMalloc(&drec.p, rec.size() * sizeof(double)));
+  MT_HIP(hipMalloc(&dpts.p, (size_t)n_pts * 3 * sizeof(double)));
+  MT_HIP(hipMalloc(&dN.p, nbytes));
+  MT_HIP(hipMemcpy(drec.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP(hipMemcpy(dpts.p, pts, (size_t)n_pts * 3 * sizeof(double), hipMemcpyHostToDevice));
+  rc = launch_tiles_tensor((const double*)drec.p, nt, (const double*)dpts.p, n_pts, (double*)dN.p, nullptr);
+  if (rc) return rc;
+  MT_HIP(hipMemcpy(N, dN.p, nbytes, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+int mt_tiles_h_field(const mt_tiles* tiles, int n_pts, const double* pts, const double* N, double* H) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n_pts > 0 && pts && H, "invalid points or output");
+  std::vector<double> rec;
+  rc = tile_records(tiles, true, rec);
+  if (rc) return rc;
+  const int nt = tiles->n_tiles;
+  DevBuf drec, dpts, dN, dH;
+  MT_HIP(hipMalloc(&drec.p, rec.size() * sizeof(double)));
+  MT_HIP(hipMalloc(&dpts.p, (size_t)n_pts * 3 * sizeof(double)));
+  MT_HIP(hipMalloc(&dH.p, (size_t)n_pts * 3 * sizeof(double)));
+  MT_HIP(hipMemcpy(drec.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP(hipMemcpy(dpts.p, pts, (size_t)n_pts * 3 * sizeof(double), hipMemcpyHostToDevice));
+  if (N) {
+    const size_t nbytes = (size_t)nt * n_pts * 9 * sizeof(double);
+    MT_HIP(hipMalloc(&dN.p, nbytes));
+    MT_HIP(hipMemcpy(dN.p, N, nbytes, hipMemcpyHostToDevice));
+  }
+  rc = launch_tiles_field((const double*)drec.p, nt, (const double*)dpts.p, n_pts, (const double*)dN.p,
+                          (double*)dH.p, nullptr);
+  if (rc) return rc;
+  MT_HIP(hipMemcpy(H, dH.p, (size_t)n_pts * 3 * sizeof(double), hipMemcpyDeviceToHost));
+  return kOk;
 }
 
 // ---- reference plug-in ABI: one global instance, host arrays, PCIe per call ----

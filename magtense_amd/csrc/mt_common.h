@@ -20,6 +20,7 @@ enum Status {
   kErrSolver = -3,    // RKSUITE fatal condition
   kErrAlloc = -4,     // device allocation failure
   kErrState = -5,     // API used in the wrong order
+  kErrUnsupported = -6,  // input outside the supported subset (e.g. a non-prism tile)
 };
 
 #define MT_HIP(expr)                                                                              \

@@ -6,8 +6,13 @@
 namespace mt {
 
 // ---- (a) tensor assembly ------------------------------------------------------
-int launch_assemble(const double* src, const double* abc, int abc_stride, const double* pts, int n_src, int n_pts,
-                    float* out32, double* out64, uint8_t* split, hipStream_t stream);
+int launch_assemble(const double* src, const double* abc, int abc_stride, const double* rot, const double* pts,
+                    int n_src, int n_pts, float* out32, double* out64, uint8_t* split, hipStream_t stream);
+// Prism tiles: tile records of 32 doubles (abc, offset, R, Rinv, R*M, symmetry
+// ops, symmetry flag, pad); points Fortran-ordered (npt,3).
+int launch_tiles_tensor(const double* tiles, int nt, const double* pts, int npt, double* out, hipStream_t stream);
+int launch_tiles_field(const double* tiles, int nt, const double* pts, int npt, const double* Nin, double* H,
+                       hipStream_t stream);
 int launch_split(const float* K, int n, uint8_t* split, hipStream_t stream);
 
 // ---- (b) demag product ------------------------------------------------------------

@@ -170,7 +170,8 @@ __device__ inline int64_t frag_offset(int64_t np, int64_t i, int64_t j, int c, i
 }
 
 __global__ void __launch_bounds__(256) assemble_kernel(const double* __restrict__ src, const double* __restrict__ abc,
-                                                       int abc_stride, const double* __restrict__ pts, int n_src,
+                                                       int abc_stride, const double* __restrict__ rot,
+                                                       const double* __restrict__ pts, int n_src,
                                                        int n_pts, int64_t np, float* __restrict__ out32,
                                                        double* __restrict__ out64, uint8_t* __restrict__ split) {
   // Thread -> (row i, 8 columns starting at j0); grid covers the padded square for the split layout.
@@ -195,7 +196,15 @@ __global__ void __launch_bounds__(256) assemble_kernel(const double* __restrict_
     if (ok) {
       const double* s = src + j * 3;
       const double* d = abc + j * abc_stride;
-      prism_k6(d[0], d[1], d[2], px - s[0], py - s[1], pz - s[2], k6);
+      double dx = px - s[0], dy = py - s[1], dz = pz - s[2];
+      if (rot) {  // diffPos = rotMat (p - offset), DemagFieldGetSolution.f90:705-709
+        const double* R = rot + j * 9;
+        double rx = R[0] * dx + R[1] * dy + R[2] * dz;
+        double ry = R[3] * dx + R[4] * dy + R[5] * dz;
+        double rz = R[6] * dx + R[7] * dy + R[8] * dz;
+        dx = rx; dy = ry; dz = rz;
+      }
+      prism_k6(d[0], d[1], d[2], dx, dy, dz, k6);
       if (out64) {
         size_t plane = (size_t)n_pts * n_src;
         for (int c = 0; c < 6; ++c) out64[c * plane + (size_t)i * n_src + j] = k6[c];
@@ -254,12 +263,126 @@ __global__ void __launch_bounds__(256) split_kernel(const float* __restrict__ K,
   }
 }
 
+// ---- magnetostatic tiles (getNFromTiles / getHFromTiles, prism tiles) -------
+// Tile record (TileRec, 32 doubles): abc[3] off[3] R[9] Rinv[9] RM[3] ops[3]
+// sym pad, with R/Rinv of getRotationMatrices (DemagFieldGetSolution.f90:925-948)
+// and RM = R M (the rotated magnetisation of getFieldFromTile, :718).
+
+// Tensor of one tile at point (px,py,pz): getFieldFromTile (:685-730), or the
+// eight mirrored evaluations of getFieldFromTile_symm (:744-822).  The symmetry
+// matrices of getSymmOpMatrices (:833-896) are a signed identity (for M) and a
+// diagonal mirror (for H and the point), applied here as sign products.
+__device__ void tile_tensor(const double* __restrict__ T, double px, double py, double pz, double N[9]) {
+  const double* R = T + 6;
+  int sym = T[30] != 0.0;
+  int cases = sym ? 8 : 1;
+  double o0 = T[27], o1 = T[28], o2 = T[29];
+  // c[j]: sign applied to M; (sx,sy,sz)[j]: mirror applied to the point and rows
+  const double cs[8] = {1.0, o2, o1, o0, o1 * o2, o0 * o2, o0 * o1, o2 * (o1 * o0)};
+  const signed char mir[8][3] = {{1, 1, 1}, {-1, 1, 1}, {1, -1, 1}, {1, 1, -1},
+                                 {-1, -1, 1}, {-1, 1, -1}, {1, -1, -1}, {-1, -1, -1}};
+  for (int q = 0; q < 9; ++q) N[q] = 0.0;
+  for (int j = 0; j < cases; ++j) {
+    double s0 = mir[j][0], s1 = mir[j][1], s2 = mir[j][2];
+    double dx = (sym ? s0 * px : px) - T[3];
+    double dy = (sym ? s1 * py : py) - T[4];
+    double dz = (sym ? s2 * pz : pz) - T[5];
+    double rx = R[0] * dx + R[1] * dy + R[2] * dz;
+    double ry = R[3] * dx + R[4] * dy + R[5] * dz;
+    double rz = R[6] * dx + R[7] * dy + R[8] * dz;
+    double k[6];
+    prism_k6(T[0], T[1], T[2], rx, ry, rz, k);
+    double full[9] = {k[0], k[1], k[2], k[1], k[3], k[4], k[2], k[4], k[5]};
+    if (!sym) {
+      for (int q = 0; q < 9; ++q) N[q] = full[q];
+    } else {
+      double srow[3] = {s0, s1, s2};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) N[a * 3 + b] = srow[a] * (full[a * 3 + b] * cs[j]) + N[a * 3 + b];
+    }
+  }
+}
+
+// out: N(n_tiles, n_pts, 3, 3) in Fortran order, (i,p,a,b) at i + nt*(p + np*(a + 3b)).
+__global__ void __launch_bounds__(256) tiles_tensor_kernel(const double* __restrict__ tiles, int nt,
+                                                           const double* __restrict__ pts, int npt,
+                                                           double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const double* T = tiles + (size_t)i * 32;
+  for (int p = blockIdx.y; p < npt; p += gridDim.y) {
+    double N[9];
+    tile_tensor(T, pts[p], pts[p + (size_t)npt], pts[p + 2 * (size_t)npt], N);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) out[i + (size_t)nt * (p + (size_t)npt * (a + 3 * b))] = N[a * 3 + b];
+  }
+}
+
+// H(p) = sum_i Rinv_i (N_ip (R_i M_i)) (getDotProd, :909-918, and the
+// back-rotation, :721-726).  One block per point, threads stride over tiles,
+// fixed-order tree reduction.  Nin (Fortran-ordered as above) replaces the
+// tensor evaluation when given (useStoredN).
+__global__ void __launch_bounds__(256) tiles_field_kernel(const double* __restrict__ tiles, int nt,
+                                                          const double* __restrict__ pts, int npt,
+                                                          const double* __restrict__ Nin, double* __restrict__ H) {
+  __shared__ double red[3][256];
+  for (int p = blockIdx.x; p < npt; p += gridDim.x) {
+    double px = pts[p], py = pts[p + (size_t)npt], pz = pts[p + 2 * (size_t)npt];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+      const double* T = tiles + (size_t)i * 32;
+      double N[9];
+      if (Nin) {
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) N[a * 3 + b] = Nin[i + (size_t)nt * (p + (size_t)npt * (a + 3 * b))];
+      } else {
+        tile_tensor(T, px, py, pz, N);
+      }
+      const double* Ri = T + 15;
+      const double* RM = T + 24;
+      double d0 = N[0] * RM[0] + N[1] * RM[1] + N[2] * RM[2];
+      double d1 = N[3] * RM[0] + N[4] * RM[1] + N[5] * RM[2];
+      double d2 = N[6] * RM[0] + N[7] * RM[1] + N[8] * RM[2];
+      acc[0] += Ri[0] * d0 + Ri[1] * d1 + Ri[2] * d2;
+      acc[1] += Ri[3] * d0 + Ri[4] * d1 + Ri[5] * d2;
+      acc[2] += Ri[6] * d0 + Ri[7] * d1 + Ri[8] * d2;
+    }
+    for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = acc[a];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w)
+        for (int a = 0; a < 3; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0)
+      for (int a = 0; a < 3; ++a) H[p + (size_t)npt * a] = red[a][0];
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 // Host launchers -------------------------------------------------------------
 
-int launch_assemble(const double* src, const double* abc, int abc_stride, const double* pts, int n_src, int n_pts,
-                    float* out32, double* out64, uint8_t* split, hipStream_t stream) {
+int launch_tiles_tensor(const double* tiles, int nt, const double* pts, int npt, double* out, hipStream_t stream) {
+  if (nt <= 0 || npt <= 0) return kOk;
+  dim3 grid((unsigned)((nt + 255) / 256), (unsigned)(npt < 65535 ? npt : 65535));
+  hipLaunchKernelGGL(tiles_tensor_kernel, grid, dim3(256), 0, stream, tiles, nt, pts, npt, out);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_tiles_field(const double* tiles, int nt, const double* pts, int npt, const double* Nin, double* H,
+                       hipStream_t stream) {
+  if (npt <= 0) return kOk;
+  unsigned blocks = (unsigned)(npt < 65535 ? npt : 65535);
+  hipLaunchKernelGGL(tiles_field_kernel, dim3(blocks), dim3(256), 0, stream, tiles, nt, pts, npt, Nin, H);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_assemble(const double* src, const double* abc, int abc_stride, const double* rot, const double* pts,
+                    int n_src, int n_pts, float* out32, double* out64, uint8_t* split, hipStream_t stream) {
   if (split && n_src != n_pts) {
     set_error("split tensor layout needs a square tensor (n_src == n_pts)");
     return kErrArg;
@@ -269,8 +392,8 @@ int launch_assemble(const double* src, const double* abc, int abc_stride, const 
   int64_t rows = split ? np : n_pts;
   int64_t threads = rows * groups_per_row;
   int64_t blocks = (threads + 255) / 256;
-  hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, abc, abc_stride, pts, n_src,
-                     n_pts, np, out32, out64, split);
+  hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, abc, abc_stride, rot, pts,
+                     n_src, n_pts, np, out32, out64, split);
   MT_HIP(hipGetLastError());
   return kOk;
 }

@@ -1196,7 +1196,7 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
     double abc[3] = {P.dx, P.dy, P.dz};
     MT_HIP(hipMemcpyAsync(dpts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
     MT_HIP(hipMemcpyAsync(dabc, abc, sizeof(abc), hipMemcpyHostToDevice, P.stream));
-    int rc = launch_assemble(dpts, dabc, 0, dpts, n, n, P.K32, nullptr, P.Ksplit, P.stream);
+    int rc = launch_assemble(dpts, dabc, 0, nullptr, dpts, n, n, P.K32, nullptr, P.Ksplit, P.stream);
     if (rc) return rc;
     MT_HIP(hipStreamSynchronize(P.stream));
     MT_HIP(hipFree(dpts));

@@ -1522,3 +1522,176 @@ int or_rk4_adjoint(const or_problem* q, double dt, long long nsteps, const doubl
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------------------------
+// Magnetostatic prism tiles (getNFromTiles / getHFromTiles restricted to
+// rectangular prisms, python/src/magtense/lib/FortranToPythonIO.f90:25-223).
+// Arrays are Fortran-ordered as f2py passes them: a (n,3) array holds element
+// (i,k) at i + n*k; the tensor N(n_tiles,n_pts,3,3) holds (i,p,a,b) at
+// i + nt*(p + np*(a + 3b)).
+// ----------------------------------------------------------------------------
+namespace {
+
+typedef double Mat3[3][3];
+
+void matmul3(const Mat3 A, const Mat3 B, Mat3 C) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) C[r][c] = A[r][0] * B[0][c] + A[r][1] * B[1][c] + A[r][2] * B[2][c];
+}
+
+// DemagFieldGetSolution.f90:925-1011 (getRotationMatrices, getRotX/Y/Z)
+void rotation_matrices(const double ang[3], Mat3 R, Mat3 Rinv) {
+  double ax = -ang[0], ay = -ang[1], az = -ang[2];
+  Mat3 X = {{1, 0, 0}, {0, std::cos(ax), -std::sin(ax)}, {0, std::sin(ax), std::cos(ax)}};
+  Mat3 Y = {{std::cos(ay), 0, std::sin(ay)}, {0, 1, 0}, {-std::sin(ay), 0, std::cos(ay)}};
+  Mat3 Z = {{std::cos(az), -std::sin(az), 0}, {std::sin(az), std::cos(az), 0}, {0, 0, 1}};
+  Mat3 ZY, XtYt, Xt, Yt, Zt;
+  matmul3(Z, Y, ZY);
+  matmul3(ZY, X, R);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) { Xt[r][c] = X[c][r]; Yt[r][c] = Y[c][r]; Zt[r][c] = Z[c][r]; }
+  matmul3(Xt, Yt, XtYt);
+  matmul3(XtYt, Zt, Rinv);
+}
+
+// DemagFieldGetSolution.f90:833-896 (getSymmOpMatrices).  Every symm_M is a
+// signed identity and every symm_H a diagonal mirror, so both are returned as
+// the scalar sign c[j] and the diagonal s[j][3].
+void symmetry_ops(const double ops[3], double c[8], double s[8][3]) {
+  const double sx[3] = {-1, 1, 1}, sy[3] = {1, -1, 1}, sz[3] = {1, 1, -1};
+  c[0] = 1; c[1] = ops[2]; c[2] = ops[1]; c[3] = ops[0];
+  c[4] = c[2] * c[1]; c[5] = c[3] * c[1]; c[6] = c[3] * c[2]; c[7] = c[1] * (c[2] * c[3]);
+  for (int k = 0; k < 3; ++k) {
+    s[0][k] = 1; s[1][k] = sx[k]; s[2][k] = sy[k]; s[3][k] = sz[k];
+    s[4][k] = sy[k] * sx[k]; s[5][k] = sz[k] * sx[k]; s[6][k] = sz[k] * sy[k];
+    s[7][k] = sx[k] * (sy[k] * sz[k]);
+  }
+}
+
+void k6_to_mat(const double k[6], Mat3 N) {
+  N[0][0] = k[0]; N[0][1] = k[1]; N[0][2] = k[2];
+  N[1][0] = k[1]; N[1][1] = k[3]; N[1][2] = k[4];
+  N[2][0] = k[2]; N[2][1] = k[4]; N[2][2] = k[5];
+}
+
+struct PrismTile {
+  double abc[3], off[3], M[3], ops[3];
+  int sym;
+  Mat3 R, Rinv;
+};
+
+// getFieldFromTile (:685-730) and getFieldFromTile_symm (:744-822): tensor at p
+void tile_tensor(const PrismTile& T, const double p[3], Mat3 N) {
+  if (!T.sym) {
+    double d[3] = {p[0] - T.off[0], p[1] - T.off[1], p[2] - T.off[2]}, r[3], k[6];
+    for (int a = 0; a < 3; ++a) r[a] = T.R[a][0] * d[0] + T.R[a][1] * d[1] + T.R[a][2] * d[2];
+    prism_k6(T.abc[0], T.abc[1], T.abc[2], r[0], r[1], r[2], k);
+    k6_to_mat(k, N);
+    return;
+  }
+  double c[8], s[8][3];
+  symmetry_ops(T.ops, c, s);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) N[a][b] = 0.0;
+  for (int j = 0; j < 8; ++j) {
+    double d[3], r[3], k[6];
+    Mat3 Nj;
+    for (int a = 0; a < 3; ++a) d[a] = s[j][a] * p[a] - T.off[a];
+    for (int a = 0; a < 3; ++a) r[a] = T.R[a][0] * d[0] + T.R[a][1] * d[1] + T.R[a][2] * d[2];
+    prism_k6(T.abc[0], T.abc[1], T.abc[2], r[0], r[1], r[2], k);
+    k6_to_mat(k, Nj);
+    // N = symm_H * (N * symm_M): row a scaled by s[j][a], all scaled by c[j]
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) N[a][b] = s[j][a] * (Nj[a][b] * c[j]) + N[a][b];
+  }
+}
+
+// H contribution of one tile at p given its tensor (getDotProd + rotations)
+void tile_field(const PrismTile& T, const Mat3 N, double h[3]) {
+  double rm[3], dp[3];
+  for (int a = 0; a < 3; ++a) rm[a] = T.R[a][0] * T.M[0] + T.R[a][1] * T.M[1] + T.R[a][2] * T.M[2];
+  for (int a = 0; a < 3; ++a) dp[a] = N[a][0] * rm[0] + N[a][1] * rm[1] + N[a][2] * rm[2];
+  for (int a = 0; a < 3; ++a) h[a] = T.Rinv[a][0] * dp[0] + T.Rinv[a][1] * dp[1] + T.Rinv[a][2] * dp[2];
+}
+
+int load_prism_tiles(int nt, const double* size, const double* offset, const double* rot, const double* M,
+                     const int* tile_type, const int* use_sym, const double* sym_ops, std::vector<PrismTile>& T) {
+  T.resize(nt);
+  for (int i = 0; i < nt; ++i) {
+    if (tile_type && tile_type[i] != 2) return -2;
+    PrismTile& t = T[i];
+    double ang[3];
+    for (int k = 0; k < 3; ++k) {
+      t.abc[k] = size[i + (size_t)nt * k];
+      t.off[k] = offset[i + (size_t)nt * k];
+      t.M[k] = M ? M[i + (size_t)nt * k] : 0.0;
+      t.ops[k] = sym_ops ? sym_ops[i + (size_t)nt * k] : 1.0;
+      ang[k] = rot ? rot[i + (size_t)nt * k] : 0.0;
+    }
+    t.sym = use_sym ? (use_sym[i] == 1) : 0;
+    rotation_matrices(ang, t.R, t.Rinv);
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// getRotationMatrices for one set of angles; R and Rinv row-major 3x3
+void or_rotation_matrices(const double* ang, double* R, double* Rinv) {
+  Mat3 a, b;
+  rotation_matrices(ang, a, b);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) { R[3 * r + c] = a[r][c]; Rinv[3 * r + c] = b[r][c]; }
+}
+
+// getNFromTiles for prism tiles; returns -2 if a tile is not a prism
+int or_tiles_tensor(int nt, const double* size, const double* offset, const double* rot, const int* tile_type,
+                    const int* use_sym, const double* sym_ops, int np, const double* pts, double* N) {
+  std::vector<PrismTile> T;
+  int rc = load_prism_tiles(nt, size, offset, rot, nullptr, tile_type, use_sym, sym_ops, T);
+  if (rc) return rc;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int p = 0; p < np; ++p) {
+    double q[3] = {pts[p], pts[p + (size_t)np], pts[p + 2 * (size_t)np]};
+    for (int i = 0; i < nt; ++i) {
+      Mat3 Nm;
+      tile_tensor(T[i], q, Nm);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) N[i + (size_t)nt * (p + (size_t)np * (a + 3 * b))] = Nm[a][b];
+    }
+  }
+  return 0;
+}
+
+// getHFromTiles for prism tiles (H summed over tiles in tile order); N_in,
+// when given, replaces the tensor evaluation (useStoredN)
+int or_tiles_field(int nt, const double* size, const double* offset, const double* rot, const double* M,
+                   const int* tile_type, const int* use_sym, const double* sym_ops, int np, const double* pts,
+                   const double* N_in, double* H) {
+  std::vector<PrismTile> T;
+  int rc = load_prism_tiles(nt, size, offset, rot, M, tile_type, use_sym, sym_ops, T);
+  if (rc) return rc;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int p = 0; p < np; ++p) {
+    double q[3] = {pts[p], pts[p + (size_t)np], pts[p + 2 * (size_t)np]};
+    double acc[3] = {0, 0, 0};
+    for (int i = 0; i < nt; ++i) {
+      Mat3 Nm;
+      if (N_in) {
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) Nm[a][b] = N_in[i + (size_t)nt * (p + (size_t)np * (a + 3 * b))];
+      } else {
+        tile_tensor(T[i], q, Nm);
+      }
+      double h[3];
+      tile_field(T[i], Nm, h);
+      for (int a = 0; a < 3; ++a) acc[a] += h[a];
+    }
+    for (int a = 0; a < 3; ++a) H[p + (size_t)np * a] = acc[a];
+  }
+  return 0;
+}
+
+}  // extern "C"

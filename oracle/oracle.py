@@ -72,6 +72,9 @@ def lib():
         _lib.or_demag_rows.argtypes = [C.c_int, C.c_int, _fp, _fp, _dp, _dp]
         _lib.or_rk4_adjoint.argtypes = [C.POINTER(OrProblem), C.c_double, C.c_longlong, _dp, _dp, _dp, _dp, _dp,
                                         _dp, _dp]
+        _lib.or_rotation_matrices.argtypes = [_dp, _dp, _dp]
+        _lib.or_tiles_tensor.argtypes = [C.c_int, _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, _dp, _dp]
+        _lib.or_tiles_field.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp]
     return _lib
 
 
@@ -245,3 +248,53 @@ def demag_rows(Krows, ms_rows, m):
     out = np.zeros(3 * R)
     lib().or_demag_rows(n, R, Krows.ctypes.data_as(_fp), ms_rows.ctypes.data_as(_fp), _d(m), _d(out))
     return out.reshape(3, R)
+
+
+def rotation_matrices(angles):
+    """(rotMat, rotMatInv) of DemagFieldGetSolution.f90:925-948, row-major."""
+    R = np.zeros((3, 3))
+    Ri = np.zeros((3, 3))
+    lib().or_rotation_matrices(_d(_c64(np.asarray(angles, dtype=np.float64))), _d(R), _d(Ri))
+    return R, Ri
+
+
+def _tile_args(size, offset, rot, M, tile_type, use_sym, sym_ops):
+    nt = np.asarray(size).reshape(-1, 3).shape[0]
+
+    def f3(a, default):
+        a = np.full((nt, 3), default, dtype=np.float64) if a is None else np.asarray(a, dtype=np.float64)
+        return np.asfortranarray(a.reshape(nt, 3))
+
+    def i1(a, default):
+        a = np.full(nt, default, dtype=np.int32) if a is None else np.asarray(a, dtype=np.int32)
+        return np.ascontiguousarray(a.reshape(nt))
+
+    return nt, f3(size, 0.0), f3(offset, 0.0), f3(rot, 0.0), f3(M, 0.0), i1(tile_type, 2), i1(use_sym, 0), \
+        f3(sym_ops, 1.0)
+
+
+def tiles_tensor(size, offset, pts, rot=None, tile_type=None, use_sym=None, sym_ops=None):
+    """getNFromTiles for prism tiles: N(n_tiles, n_pts, 3, 3), Fortran order."""
+    nt, sz, off, rt, _, tt, us, so = _tile_args(size, offset, rot, None, tile_type, use_sym, sym_ops)
+    P = np.asfortranarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
+    npt = P.shape[0]
+    N = np.zeros((nt, npt, 3, 3), order="F")
+    rc = lib().or_tiles_tensor(nt, _d(sz), _d(off), _d(rt), tt.ctypes.data_as(_ip), us.ctypes.data_as(_ip),
+                               _d(so), npt, _d(P), _d(N))
+    if rc:
+        raise ValueError("only prism tiles (tile_type 2) are restated")
+    return N
+
+
+def tiles_field(size, offset, M, pts, rot=None, tile_type=None, use_sym=None, sym_ops=None, N=None):
+    """getHFromTiles for prism tiles: H(n_pts, 3), Fortran order."""
+    nt, sz, off, rt, mm, tt, us, so = _tile_args(size, offset, rot, M, tile_type, use_sym, sym_ops)
+    P = np.asfortranarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
+    npt = P.shape[0]
+    H = np.zeros((npt, 3), order="F")
+    Nin = None if N is None else np.asfortranarray(np.asarray(N, dtype=np.float64))
+    rc = lib().or_tiles_field(nt, _d(sz), _d(off), _d(rt), _d(mm), tt.ctypes.data_as(_ip), us.ctypes.data_as(_ip),
+                              _d(so), npt, _d(P), None if Nin is None else _d(Nin), _d(H))
+    if rc:
+        raise ValueError("only prism tiles (tile_type 2) are restated")
+    return H

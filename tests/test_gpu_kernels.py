@@ -44,7 +44,7 @@ def grid_tensor_gpu(res, L, want64=False, want_split=False):
     K32 = torch.empty(6 * n * n, dtype=torch.float32, device="cuda")
     K64 = torch.empty(6 * n * n, dtype=torch.float64, device="cuda") if want64 else None
     split = torch.empty(L_.mt_split_tensor_bytes(n), dtype=torch.uint8, device="cuda") if want_split else None
-    rc = L_.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, dp.data_ptr(), n, n, K32.data_ptr(),
+    rc = L_.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, None, dp.data_ptr(), n, n, K32.data_ptr(),
                             K64.data_ptr() if want64 else None, split.data_ptr() if want_split else None)
     lib.check(rc)
     return n, K32, K64, split
@@ -66,7 +66,7 @@ def test_prism_tensor_random_pairs(oracle):
     out64 = torch.empty(6 * npt * ns, dtype=torch.float64, device="cuda")
     out32 = torch.empty(6 * npt * ns, dtype=torch.float32, device="cuda")
     ds, da, dpt = _dev(src), _dev(abc), _dev(pts)
-    lib.check(L_.mt_prism_tensor(ds.data_ptr(), da.data_ptr(), 3, dpt.data_ptr(), ns, npt, out32.data_ptr(),
+    lib.check(L_.mt_prism_tensor(ds.data_ptr(), da.data_ptr(), 3, None, dpt.data_ptr(), ns, npt, out32.data_ptr(),
                                  out64.data_ptr(), None))
     g64 = out64.cpu().numpy().reshape(6, npt, ns)
     g32 = out32.cpu().numpy().reshape(6, npt, ns)

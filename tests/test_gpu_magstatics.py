@@ -1,0 +1,115 @@
+"""GPU parity of the prism-tile API (get_demag_tensor / get_H_field, the
+prism part of getNFromTiles / getHFromTiles) and of the tensor kernel on the
+16384-prism jittered lattice, against the FP64 oracle.
+
+Tolerances: tensors |N_gpu - N_oracle| <= 1e-12 * max|N| (libm ulp
+differences; the arithmetic order is the same); fields 1e-12 relative
+(norm-wise; the GPU sums tiles in a fixed tree order instead of sequentially);
+FP32 lattice tensor within 2 ulp of the oracle's FP64 value rounded.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from test_oracle_tiles import comsol_case, comsol_check
+
+
+def _random_tiles(rng, nt, sym_every=0):
+    from magtense_amd import magstatics as ms
+
+    t = ms.Tiles(nt, size=rng.uniform(0.05, 0.4, (nt, 3)).tolist(), offset=rng.uniform(-1, 1, (nt, 3)).tolist(),
+                 tile_type=2, M_rem=1.1 / (4e-7 * math.pi), easy_axis=rng.normal(size=(nt, 3)).tolist())
+    t.rot = rng.uniform(-math.pi, math.pi, (nt, 3)).tolist()
+    if sym_every:
+        t.use_sym[::sym_every] = 1
+        t.sym_op[::sym_every] = [1.0, -1.0, 1.0]
+    return t
+
+
+def _oracle_args(t):
+    return dict(rot=t.rot, use_sym=t.use_sym, sym_ops=t.sym_op)
+
+
+@pytest.mark.parametrize("nt,npt,sym", [(1, 300, 0), (37, 129, 0), (300, 260, 5)])
+def test_tiles_tensor_and_field(oracle, nt, npt, sym):
+    from magtense_amd import magstatics as ms
+
+    rng = np.random.default_rng(nt + npt)
+    t = _random_tiles(rng, nt, sym)
+    pts = rng.uniform(-1.5, 1.5, (npt, 3))
+    pts[0] = t.offset[0]  # the centre of a tile
+    N = ms.get_demag_tensor(t, pts)
+    Nref = oracle.tiles_tensor(t.size, t.offset, pts, **_oracle_args(t))
+    assert N.shape == (nt, npt, 3, 3) and N.flags.f_contiguous
+    assert np.all(np.isfinite(N))
+    assert np.abs(N - Nref).max() <= 1e-12 * np.abs(Nref).max()
+    H = ms.get_H_field(t, pts)
+    Href = oracle.tiles_field(t.size, t.offset, t.M, pts, **_oracle_args(t))
+    rel = np.linalg.norm(H - Href) / np.linalg.norm(Href)
+    assert rel <= 1e-12, rel
+    Hs = ms.get_H_field(t, pts, N)
+    rel_s = np.linalg.norm(Hs - Href) / np.linalg.norm(Href)
+    assert rel_s <= 1e-12, rel_s
+    print(f"tiles={nt} pts={npt} sym={sym}: tensor max err {np.abs(N - Nref).max():.2e}, field rel {rel:.2e}")
+
+
+def test_comsol_prism_lines_gpu(oracle):
+    from magtense_amd import magstatics as ms
+
+    def fn(case, pts):
+        t = ms.Tiles(1, size=case["size"], offset=case["offset"], rot=case["rot"], tile_type=2,
+                     M_rem=1.2 / (4e-7 * math.pi), easy_axis=[0.35355339, 0.61237244, 0.70710678])
+        assert np.allclose(t.M[0], case["M"], rtol=0, atol=0)
+        return ms.get_H_field(t, pts)
+
+    worst = comsol_check(fn, oracle)
+    print("GPU vs COMSOL", worst)
+
+
+def _lattice(seed=4):
+    """16384 prisms: 128 x 128 lattice of pitch 5 nm, centres jittered by
+    U[-0.5, 0.5] nm in x and y, edges U[0.8, 1.2] * (5, 5, 2) nm."""
+    rng = np.random.default_rng(seed)
+    ix, iy = np.meshgrid(np.arange(128), np.arange(128), indexing="ij")
+    c = np.zeros((128 * 128, 3))
+    c[:, 0] = ix.reshape(-1) * 5e-9 + rng.uniform(-0.5e-9, 0.5e-9, 128 * 128)
+    c[:, 1] = iy.reshape(-1) * 5e-9 + rng.uniform(-0.5e-9, 0.5e-9, 128 * 128)
+    abc = rng.uniform(0.8, 1.2, (128 * 128, 3)) * np.array([5e-9, 5e-9, 2e-9])
+    return c, abc
+
+
+def test_jittered_lattice_tensor_rows(oracle):
+    import torch
+    from magtense_amd import _lib
+
+    L = _lib.lib()
+    c, abc = _lattice()
+    n = c.shape[0]
+    dc = torch.as_tensor(c).cuda()
+    da = torch.as_tensor(abc).cuda()
+    K32 = torch.empty(6 * n * n, dtype=torch.float32, device="cuda")
+    _lib.check(L.mt_prism_tensor(dc.data_ptr(), da.data_ptr(), 3, None, dc.data_ptr(), n, n, K32.data_ptr(), None,
+                                 None))
+    rng = np.random.default_rng(7)
+    rows = np.sort(rng.choice(n, size=48, replace=False))
+    rows[0] = 0
+    rows[-1] = n - 1
+    ref = oracle.tensor_points(c, abc, c[rows])  # (6, 48, n)
+    K = K32.view(6, n, n)
+    got = K[:, torch.as_tensor(rows).cuda(), :].cpu().numpy()
+    r32 = ref.astype(np.float32)
+    ulp = np.spacing(np.abs(r32))
+    bad = np.abs(got - r32) > 2 * ulp
+    assert not bad.any(), (bad.sum(), np.abs(got - r32)[bad].max())
+    # near pairs (< 22.05 nm) of the sampled rows are all covered and finite
+    d = np.linalg.norm(c[rows][:, None, :] - c[None, :, :], axis=2)
+    near = d < 3 * math.sqrt(5 ** 2 + 5 ** 2 + 2 ** 2) * 1e-9
+    assert near.sum() >= 48 * 20
+    assert np.all(np.isfinite(got))
+    # the self term: trace of K at a prism's own centre is -1
+    diag = got[0, np.arange(48), rows] + got[3, np.arange(48), rows] + got[5, np.arange(48), rows]
+    assert np.allclose(diag, -1.0, atol=1e-6)
