@@ -160,10 +160,10 @@ def field_error_vs_fp64(W, n_rows=1024, n_states=8, seed=7):
                 bf16x2_max_rel=float(np.abs(gbf - ref).max() / np.abs(ref).max()))
 
 
-def cpu_baseline(W, rows=2048, reps=5):
-    """FP64 oracle (FP32 demag semantics, 9 gemv-equivalents per RHS) on the host:
-    time the demag product of `rows` receiver rows for one state and extrapolate
-    to a full 64-state batch step (7 RHS per accepted step)."""
+def cpu_baseline(W, rows=2048):
+    """FP64 oracle (FP32 demag semantics, 9 gemv-equivalents per RHS) on the host
+    cores: the demag products of one full batch step (7 RHS x all B states) for
+    `rows` of the N receiver rows, extrapolated to all rows."""
     from oracle import oracle as O
 
     n = W["n"]
@@ -171,26 +171,72 @@ def cpu_baseline(W, rows=2048, reps=5):
     pts, d = O.setup_grid(list(res), list(Lg))
     rng = np.random.default_rng(3)
     sel = np.sort(rng.choice(n, rows, replace=False))
-    src = pts
     abc = np.tile(d, (n, 1))
     t0 = time.perf_counter()
-    Krows = O.tensor_points(src, abc, pts[sel]).astype(np.float32)
+    Krows = O.tensor_points(pts, abc, pts[sel]).astype(np.float32)
     t_tensor = time.perf_counter() - t0
     ms = np.full(rows, 8e5, dtype=np.float32)
+    t0 = time.perf_counter()
+    for _rhs in range(7):
+        for b in range(W["B"]):
+            O.demag_rows(Krows, ms, W["m0"][b])
+    t_sample = time.perf_counter() - t0
+    t_batch_step = t_sample * n / rows
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=1.0 / t_batch_step, unit="LL batch-steps/s (64 states)", cores=cores, kind="port",
+                sample=(f"FP64 oracle demag products (FP32 tensor, FP64-exact sums, reference gemv order) of one "
+                        f"batch step (7 RHS x {W['B']} states) for {rows} of {n} receiver rows: {t_sample:.2f} s "
+                        f"on {cores} threads; extrapolated x{n / rows:g} rows (tensor rows took {t_tensor:.1f} s, "
+                        f"not counted)"))
+
+
+def lattice_prisms(seed=4, side=128):
+    """16384 prisms: side x side lattice of pitch 5 nm, centres jittered by
+    U[-0.5, 0.5] nm in x and y, edges U[0.8, 1.2] * (5, 5, 2) nm."""
+    rng = np.random.default_rng(seed)
+    ix, iy = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+    c = np.zeros((side * side, 3))
+    c[:, 0] = ix.reshape(-1) * 5e-9 + rng.uniform(-0.5e-9, 0.5e-9, side * side)
+    c[:, 1] = iy.reshape(-1) * 5e-9 + rng.uniform(-0.5e-9, 0.5e-9, side * side)
+    abc = rng.uniform(0.8, 1.2, (side * side, 3)) * np.array([5e-9, 5e-9, 2e-9])
+    return c, abc
+
+
+def tensor_assembly(reps=3, cpu_rows=64):
+    """Prism-tensor assembly on the 16384-prism jittered lattice: all N^2 pairs,
+    six FP32 components stored (6.44 GB); pairs/s on the GPU and, for a sample
+    of receiver rows, in the FP64 oracle on the host cores."""
+    import torch
+    from magtense_amd import _lib
+
+    L = _lib.lib()
+    c, abc = lattice_prisms()
+    n = c.shape[0]
+    dc = torch.as_tensor(c).cuda()
+    da = torch.as_tensor(abc).cuda()
+    K32 = torch.empty(6 * n * n, dtype=torch.float32, device="cuda")
+    args = (dc.data_ptr(), da.data_ptr(), 3, None, dc.data_ptr(), n, n, K32.data_ptr(), None, None)
+    _lib.check(L.mt_prism_tensor(*args))
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        O.demag_rows(Krows, ms, W["m0"][0])
+        _lib.check(L.mt_prism_tensor(*args))  # synchronous
         times.append(time.perf_counter() - t0)
-    t_rows = float(np.median(times))
-    t_rhs_state = t_rows * n / rows
-    t_batch_step = t_rhs_state * 7 * W["B"]
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=1.0 / t_batch_step, unit="LL batch-steps/s (64 states)", cores=cores, kind="port",
-                sample=(f"FP64 oracle demag product (FP32 tensor, FP64-exact sums, reference gemv order) for "
-                        f"{rows} of {n} receiver rows x 1 state, median of {reps}: {t_rows:.3f} s; extrapolated "
-                        f"x{n // rows} rows x7 RHS x{W['B']} states (extrapolated); tensor rows took "
-                        f"{t_tensor:.1f} s"))
+    t = float(np.median(times))
+    del K32
+    out = dict(workload="16384 prisms, 128x128 lattice pitch 5 nm, jittered; 6 x N^2 FP32 out",
+               pairs=n * n, ms=1e3 * t, pairs_per_s=n * n / t, store_bytes=6.0 * n * n * 4,
+               store_floor_ms=6.0 * n * n * 4 / (HBM_PEAK_GBS * 1e9) * 1e3, bound="fp64 valu (transcendentals)")
+    if cpu_rows:
+        from oracle import oracle as O
+
+        rows = np.linspace(0, n - 1, cpu_rows).astype(int)
+        t0 = time.perf_counter()
+        O.tensor_points(c, abc, c[rows])
+        tc = time.perf_counter() - t0
+        out["cpu_pairs_per_s"] = cpu_rows * n / tc
+        out["cpu_sample"] = f"oracle prism tensor, {cpu_rows} rows x {n} sources, {tc:.2f} s"
+    return out
 
 
 def main():
@@ -203,6 +249,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-field-error", action="store_true")
     ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--no-assembly", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}); "
@@ -261,6 +308,8 @@ def main():
     log(f"measured streaming bandwidth: read {rd.value:.0f} GB/s, copy {cp.value:.0f} GB/s")
     ferr = None if args.no_field_error else field_error_vs_fp64(W)
     log(f"field error vs FP64: {ferr}")
+    asm = None if args.no_assembly else tensor_assembly(cpu_rows=0 if args.no_cpu else 64)
+    log(f"tensor assembly: {asm}")
     cpu = None if args.no_cpu else cpu_baseline(W)
     log(f"cpu baseline: {cpu}")
     line = {
@@ -285,6 +334,7 @@ def main():
         "speedup_vs_plain_fp32": (steps_per_s / plain) if plain else None,
         "setup_s": t_setup,
         "field_error_vs_fp64": ferr,
+        "tensor_assembly": asm,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "gemm_bf16x2_lds_kernel", "kernel_ms": gemm_ms,

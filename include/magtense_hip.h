@@ -36,6 +36,12 @@ extern "C" {
 
 #define MT_PREC_FP32 0   /* plain per-state FP32 mat-vec (reference semantics) */
 #define MT_PREC_BF16X2 1 /* split BF16 hi+lo tensor, batched MFMA GEMM, FP32 accumulation */
+#define MT_PREC_FP64 2   /* FP64 tensor, product and Ms scaling (mt_rk4 only: gradient checks) */
+
+/* demag approximations (dem_appr of MicromagProblem, micromag.py:61-68) */
+#define MT_DEMAG_DENSE 1
+#define MT_DEMAG_THRESHOLD 2
+#define MT_DEMAG_THRESHOLD_FRACTION 4
 
 /* ---- library --------------------------------------------------------------- */
 const char* mt_last_error(void);
@@ -109,6 +115,21 @@ int mt_demag_apply_f32(const float* K32, int n, const float* m, float* h, int ns
  * fragments internally; nstates <= 64 per call (larger batches are chunked). */
 int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, int nstates, float pref);
 
+/* Threshold-sparse tensor (ApplyThresholdDense, DemagAuxFunctions.f90:20-122;
+ * ConvertDenseToSparse_s, util_micromag.f90:81-138): six CSR matrices built on
+ * the device from the FP32 receiver-major tensor K32 (device), keeping entries
+ * with |K| > threshold.  approx MT_DEMAG_THRESHOLD uses `threshold` directly;
+ * MT_DEMAG_THRESHOLD_FRACTION treats it as the fraction of positive entries to
+ * drop (FindThresholdFraction, DemagAuxFunctions.f90:353-428).  K32 may be
+ * freed afterwards. */
+int mt_demag_sparse_create(const float* K32, int n, int approx, double threshold, void** out);
+/* nnz per component (xx xy xz yy yz zz) and the FP32 threshold used. */
+int mt_demag_sparse_info(void* sp, long long* nnz6, float* threshold);
+/* h = pref * K_sparse m for nstates (<= 64 per call) states, [s][3n] device layout
+ * (the product of updateDemagfield's sparse branch, LandauLifshitzEquationSolver.f90:581-624). */
+int mt_demag_apply_sparse(void* sp, const float* m, float* h, int nstates, float pref);
+void mt_demag_sparse_destroy(void* sp);
+
 /* ---- (c) right-hand side ------------------------------------------------------------
  * One dm/dt evaluation per state on a uniform grid (device pointers):
  *   m, dmdt: nstates x 3n doubles (SoA per state); ms, k0: n doubles (host);
@@ -152,6 +173,9 @@ typedef struct {
   float* K_out;          /* optional host buffer receiving the FP32 tensor */
   int keep_fp32_tensor;  /* keep an FP32 copy beside the split tensor */
   int ksplit;            /* 0: automatic k-split of the MFMA GEMM */
+  int demag_approx;      /* MT_DEMAG_DENSE (0 or 1), MT_DEMAG_THRESHOLD (2), MT_DEMAG_THRESHOLD_FRACTION (4);
+                            the sparse modes need precision MT_PREC_FP32 */
+  double demag_threshold; /* dem_thres: |K| threshold (mode 2) or fraction of the positive entries (mode 4) */
 } mt_micromag_problem;
 
 typedef struct {
@@ -177,6 +201,9 @@ long long mt_solver_steps_total(mt_solver* s);
 int mt_solver_results(mt_solver* s, double* t_out, double* y_out, double* fields, mt_stats* stats);
 int mt_solver_state(mt_solver* s, double* y);
 int mt_solver_set_timing(mt_solver* s, int on);
+/* Sparse demag of a solver created with demag_approx 2 or 4: nnz per component
+ * and the FP32 threshold used (an argument error when the solver is dense). */
+int mt_solver_demag_info(mt_solver* s, long long* nnz6, float* threshold);
 void mt_solver_destroy(mt_solver* s);
 int mt_run_micromag(const mt_micromag_problem* problem, double* t_out, double* y_out, double* fields,
                     mt_stats* stats);
@@ -184,7 +211,9 @@ int mt_run_micromag(const mt_micromag_problem* problem, double* t_out, double* y
 /* ---- differentiable integration (no reference counterpart) -------------------------
  * Fixed-step RK4 of the same LL right-hand side for n_states states (constant
  * applied field per state: the first row of hext; alpha0, gamma, Ms, K0, A0,
- * u_ea and the tensor as in mt_micromag_problem; plain FP32 demag product).
+ * u_ea and the tensor as in mt_micromag_problem; demag product in FP32 as the
+ * reference (precision MT_PREC_FP32) or fully in FP64 (MT_PREC_FP64: FP64 tensor,
+ * product and Ms scaling, for gradient checks against finite differences).
  * mt_rk4_forward stores checkpoints every checkpoint_every steps;
  * mt_rk4_backward runs the hand-written discrete adjoint (recomputing each
  * checkpoint segment) for the cotangent dL/dm(T) and returns dL/dm0
@@ -204,6 +233,14 @@ void icl_initDemagMatrices(const float* Kxx, const float* Kxy, const float* Kxz,
 void icl_MVMult_GetH(const float* Mx, const float* My, const float* Mz, float* Hx, float* Hy, float* Hz, int* n,
                      float* pref);
 void icl_destroy(void);
+/* Sparse plug-in entry points (MagTenseCudaBlasICLWrapper.cxx:13-37,
+ * MagTenseCudaBlas.cu:106-297): component mat_no (1..6 = xx xy xz yy yz zz)
+ * as a CSR of n rows with nnz entries, 1-based columns and row starts
+ * (rowInds has n+1 entries, the last equal to nnz + rowInds[0]); then
+ * H = pref * K M over all six components, host arrays. */
+void icl_initDemagMatrices_sparse(int* n, int* nnz, int* mat_no, float* values, int* colInds, int* rowInds);
+void icl_MVMult_GetH_sparse(const float* Mx, const float* My, const float* Mz, float* Hx, float* Hy, float* Hz,
+                            const int* n, const float* pref);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,10 @@ CSRC = os.path.join(_HERE, "csrc")
 
 PREC_FP32 = 0
 PREC_BF16X2 = 1
+PREC_FP64 = 2
+DEMAG_DENSE = 1
+DEMAG_THRESHOLD = 2
+DEMAG_THRESHOLD_FRACTION = 4
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -40,6 +44,8 @@ class MtProblem(C.Structure):
         ("K_in", _fp), ("K_out", _fp),
         ("keep_fp32_tensor", C.c_int),
         ("ksplit", C.c_int),
+        ("demag_approx", C.c_int),
+        ("demag_threshold", C.c_double),
     ]
 
 
@@ -67,8 +73,9 @@ EXPORTS = [
     "mt_packed_m_bytes", "mt_split_tensor", "mt_demag_apply_f32", "mt_demag_apply_bf16x2", "mt_ll_rhs",
     "mt_solver_create", "mt_solver_advance", "mt_solver_run", "mt_solver_advance_total", "mt_solver_steps_total", "mt_solver_results", "mt_solver_state",
     "mt_solver_set_timing", "mt_solver_destroy", "mt_run_micromag", "mt_rk4_create", "mt_rk4_forward",
-    "mt_rk4_backward", "mt_rk4_destroy", "mt_tiles_demag_tensor", "mt_tiles_h_field", "icl_initDemagMatrices",
-    "icl_MVMult_GetH", "icl_destroy",
+    "mt_rk4_backward", "mt_rk4_destroy", "mt_tiles_demag_tensor", "mt_tiles_h_field", "mt_demag_sparse_create",
+    "mt_demag_sparse_info", "mt_demag_apply_sparse", "mt_demag_sparse_destroy", "mt_solver_demag_info",
+    "icl_initDemagMatrices", "icl_MVMult_GetH", "icl_destroy", "icl_initDemagMatrices_sparse", "icl_MVMult_GetH_sparse",
 ]
 
 _lib = None
@@ -122,6 +129,16 @@ def load():
     lib.mt_rk4_destroy.restype = None
     lib.mt_tiles_demag_tensor.argtypes = [C.POINTER(MtTiles), C.c_int, vp, vp]
     lib.mt_tiles_h_field.argtypes = [C.POINTER(MtTiles), C.c_int, vp, vp, vp]
+    lib.mt_demag_sparse_create.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.POINTER(vp)]
+    lib.mt_demag_sparse_info.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_float)]
+    lib.mt_demag_apply_sparse.argtypes = [vp, vp, vp, C.c_int, C.c_float]
+    lib.mt_demag_sparse_destroy.argtypes = [vp]
+    lib.mt_demag_sparse_destroy.restype = None
+    lib.mt_solver_demag_info.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_float)]
+    lib.icl_initDemagMatrices_sparse.argtypes = [C.POINTER(C.c_int)] * 3 + [vp, vp, vp]
+    lib.icl_initDemagMatrices_sparse.restype = None
+    lib.icl_MVMult_GetH_sparse.argtypes = [vp] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_float)]
+    lib.icl_MVMult_GetH_sparse.restype = None
     lib.icl_initDemagMatrices.argtypes = [vp] * 6 + [C.POINTER(C.c_int)]
     lib.icl_initDemagMatrices.restype = None
     lib.icl_MVMult_GetH.argtypes = [vp] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_float)]

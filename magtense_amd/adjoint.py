@@ -11,6 +11,10 @@ vector-Jacobian products per step.  Gradients flow to m0, to the applied field
 counterpart (MagTense has no gradients); this extends the drop-in.
 
 Tensors live on the host (float64); the integration itself runs on the GPU.
+``demag_precision='fp32'`` (default) forms the demag field as the reference
+does (FP32 tensor and product, FP32 Ms scaling); ``'fp64'`` keeps the tensor,
+product and Ms scaling in FP64, which makes the integration a smooth function
+of its inputs down to FP64 rounding (what finite-difference checks need).
 """
 
 import ctypes as C
@@ -22,7 +26,8 @@ from . import _lib
 
 
 class _Rk4Handle:
-    def __init__(self, res, L, ms, h_ext, dt, nsteps, checkpoint_every, A0, alpha, gamma, K0, u_ea, n_states):
+    def __init__(self, res, L, ms, h_ext, dt, nsteps, checkpoint_every, A0, alpha, gamma, K0, u_ea, n_states,
+                 demag_precision="fp32"):
         L_ = _lib.lib()
         n = int(np.prod(res))
         self.n, self.B = n, n_states
@@ -46,6 +51,10 @@ class _Rk4Handle:
         q.nt_hext = 1
         q.hext = dp(self.keep["hext"])
         q.hext_per_state = 1
+        try:
+            q.precision = {"fp32": _lib.PREC_FP32, "fp64": _lib.PREC_FP64}[demag_precision]
+        except KeyError:
+            raise ValueError("demag_precision must be 'fp32' or 'fp64'") from None
         h = C.c_void_p()
         _lib.check(L_.mt_rk4_create(C.byref(q), float(dt), int(nsteps), int(checkpoint_every), C.byref(h)))
         self.h = h
@@ -87,7 +96,7 @@ class LLRK4(torch.autograd.Function):
         B = m0.shape[0]
         handle = _Rk4Handle(cfg["res"], cfg["L"], ms.detach().cpu().numpy(), h_ext.detach().cpu().numpy(),
                             cfg["dt"], cfg["nsteps"], cfg["checkpoint_every"], cfg["A0"], cfg["alpha"],
-                            cfg["gamma"], cfg["K0"], cfg["u_ea"], B)
+                            cfg["gamma"], cfg["K0"], cfg["u_ea"], B, cfg["demag_precision"])
         mT = handle.forward(_soa(m0.detach().cpu().numpy()))
         ctx.handle = handle
         ctx.ms_shape = ms.shape
@@ -104,12 +113,13 @@ class LLRK4(torch.autograd.Function):
 
 
 def ll_rk4(m0, h_ext, ms, res, L, dt, nsteps, checkpoint_every=100, A0=1.3e-11, alpha=0.02 * 2.21e5,
-           gamma=2.21e5, K0=0.0, u_ea=None):
+           gamma=2.21e5, K0=0.0, u_ea=None, demag_precision="fp32"):
     """Differentiable RK4 LL integration.
 
     m0: (B, N, 3) or (N, 3) float64 tensor; h_ext: (B, 3) or (3,) applied field
     in A/m (constant in time); ms: (N,) or scalar Ms in A/m.  ``alpha`` is the
-    API's alpha (alpha_LL * gamma), as in MicromagProblem.
+    API's alpha (alpha_LL * gamma), as in MicromagProblem.  demag_precision:
+    'fp32' (the reference's demag arithmetic) or 'fp64'.
     Returns m(T) with the shape of m0.
     """
     single = m0.dim() == 2
@@ -117,6 +127,6 @@ def ll_rk4(m0, h_ext, ms, res, L, dt, nsteps, checkpoint_every=100, A0=1.3e-11, 
         m0 = m0.unsqueeze(0)
         h_ext = h_ext.reshape(1, 3)
     cfg = dict(res=list(res), L=list(L), dt=float(dt), nsteps=int(nsteps), checkpoint_every=int(checkpoint_every),
-               A0=A0, alpha=alpha, gamma=gamma, K0=K0, u_ea=u_ea)
+               A0=A0, alpha=alpha, gamma=gamma, K0=K0, u_ea=u_ea, demag_precision=demag_precision)
     out = LLRK4.apply(m0, h_ext.reshape(m0.shape[0], 3), ms, cfg)
     return out[0] if single else out

@@ -38,11 +38,11 @@ constexpr double kMu0 = 4.0 * 3.141592653589793 * 1e-7;
 
 struct VjpDesc {
   const double* m;      // stage input Y
-  const float* S;       // K Y (positive demag sums)
+  const void* S;        // K Y (positive demag sums), FP32 or FP64
   const double* u;      // cotangent of k = f(Y)
   double* mbar;         // out: direct part of the input cotangent
   double* hbar;         // out: field cotangent
-  float* packed;        // out: FP32 (Ms * Hbar) for the transpose demag pass
+  void* packed;         // out: (Ms * Hbar) for the transpose demag pass, FP32 or FP64
   double* grad_ms;      // accumulate: per-cell Ms gradient
   double* hext_part;    // accumulate: per-block partial sums of -Hbar (3 per block)
   double hext[3];
@@ -62,6 +62,9 @@ __device__ inline double lap_at(const double* v, int i, int ix, int iy, int iz, 
   return lap;
 }
 
+// T: demag precision (float: the reference's FP32 product and FP32 Ms scaling;
+// double: FP64 tensor, product and Ms)
+template <typename T>
 __global__ void __launch_bounds__(256) vjp_kernel(const LlgGrid g, const VjpDesc* __restrict__ descs) {
   const VjpDesc& D = descs[blockIdx.y];
   const int n = g.n;
@@ -74,8 +77,13 @@ __global__ void __launch_bounds__(256) vjp_kernel(const LlgGrid g, const VjpDesc
       m[c] = D.m[(size_t)c * n + i];
       u[c] = D.u[(size_t)c * n + i];
       hj[c] = (-2.0 * lap_at(D.m + (size_t)c * n, i, ix, iy, iz, g)) * g.jfact[i];
-      float hf = (-D.S[(size_t)c * n + i]) * g.ms32[i];
-      hm[c] = (double)hf;
+      const T sv = static_cast<const T*>(D.S)[(size_t)c * n + i];
+      if constexpr (sizeof(T) == 8) {
+        hm[c] = (-sv) * g.ms64[i];
+      } else {
+        float hf = (-sv) * g.ms32[i];
+        hm[c] = (double)hf;
+      }
     }
     if (g.aniso) {
       const double* A = g.aniso + (size_t)i * 6;
@@ -98,12 +106,12 @@ __global__ void __launch_bounds__(256) vjp_kernel(const LlgGrid g, const VjpDesc
     double hxc[3] = {H[1] * cb[2] - H[2] * cb[1], H[2] * cb[0] - H[0] * cb[2], H[0] * cb[1] - H[1] * cb[0]};
     // Hbar = -(cbar x m)
     double hb[3] = {-(cb[1] * m[2] - cb[2] * m[1]), -(cb[2] * m[0] - cb[0] * m[2]), -(cb[0] * m[1] - cb[1] * m[0])};
-    double ms = (double)g.ms32[i];
+    double ms = sizeof(T) == 8 ? g.ms64[i] : (double)g.ms32[i];
     double gms = 0.0;
     for (int c = 0; c < 3; ++c) {
       D.mbar[(size_t)c * n + i] = -D.alpha * cxu[c] - hxc[c];
       D.hbar[(size_t)c * n + i] = hb[c];
-      D.packed[(size_t)c * n + i] = (float)(ms * hb[c]);
+      static_cast<T*>(D.packed)[(size_t)c * n + i] = (T)(ms * hb[c]);
       gms += hb[c] * (hm[c] - hj[c] - hk[c]);
       hx_part[c] = -hb[c];
     }
@@ -127,12 +135,12 @@ __global__ void __launch_bounds__(256) vjp_kernel(const LlgGrid g, const VjpDesc
 struct FinishDesc {
   const double* mbar;   // direct part
   const double* hbar;   // field cotangent
-  const float* T;       // K (Ms Hbar)
+  const void* T;        // K (Ms Hbar), FP32 or FP64
   double* mu;           // out: vjp w.r.t. the stage input
 };
 
-__global__ void __launch_bounds__(256) vjp_finish_kernel(const LlgGrid g, const FinishDesc* __restrict__ descs,
-                                                         double* __restrict__ wbuf) {
+template <typename TT>
+__global__ void __launch_bounds__(256) vjp_finish_kernel(const LlgGrid g, const FinishDesc* __restrict__ descs) {
   const FinishDesc& D = descs[blockIdx.y];
   const int n = g.n;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -162,9 +170,9 @@ __global__ void __launch_bounds__(256) vjp_finish_kernel(const LlgGrid g, const 
       double lz = iz == 0 ? (w(i + sz) - wi) : (iz == g.nz - 1 ? (w(i - sz) - wi) : (w(i - sz) - 2.0 * wi + w(i + sz)));
       lap += lz * g.idz2;
     }
-    D.mu[(size_t)c * n + i] = D.mbar[(size_t)c * n + i] - 2.0 * lap - (double)D.T[(size_t)c * n + i] + ak[c];
+    const double tv = (double)static_cast<const TT*>(D.T)[(size_t)c * n + i];
+    D.mu[(size_t)c * n + i] = D.mbar[(size_t)c * n + i] - 2.0 * lap - tv + ak[c];
   }
-  (void)wbuf;
 }
 
 }  // namespace
@@ -181,14 +189,18 @@ struct Rk4 {
   std::vector<double> hext;  // B*3
   hipStream_t stream = nullptr;
   LlgGrid grid{};
+  bool fp64 = false;       // FP64 tensor, product and Ms scaling
+  size_t es = 4;           // bytes per demag element (4 or 8)
   float* K32 = nullptr;
+  double* K64 = nullptr;
   double *jfact = nullptr, *kfact = nullptr, *aniso = nullptr;
   float* ms32 = nullptr;
-  float* packf = nullptr;  // B x 3n
-  float* hbuf = nullptr;   // B x 3n
+  double* ms64 = nullptr;
+  void* packd = nullptr;   // B x 3n demag input (FP32 or FP64)
+  void* hbuf = nullptr;    // B x 3n demag sums (FP32 or FP64)
   double* ckpt = nullptr;  // (nck+1) x B x 3n
   double* seg = nullptr;   // ck+1 steps x B x (y + 4 stage inputs) x 3n
-  float* segS = nullptr;   // ck steps x 4 stages x B x 3n
+  uint8_t* segS = nullptr; // ck steps x 4 stages x B x 3n demag sums
   double* work = nullptr;  // B x 3n x 12 scratch vectors
   double* grad_ms = nullptr;   // B x n
   double* hext_part = nullptr; // B x nblocks x 3
@@ -205,13 +217,24 @@ struct Rk4 {
     auto f = [](void* p) {
       if (p) (void)hipFree(p);
     };
-    f(K32); f(jfact); f(kfact); f(aniso); f(ms32); f(packf); f(hbuf); f(ckpt); f(seg); f(segS); f(work);
+    f(K32); f(K64); f(jfact); f(kfact); f(aniso); f(ms32); f(ms64); f(packd); f(hbuf); f(ckpt); f(seg); f(segS);
+    f(work);
     f(grad_ms); f(hext_part); f(desc_d);
     if (desc_h) (void)hipHostFree(desc_h);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
   size_t n3() const { return (size_t)3 * n; }
+  void* hcol(void* base, int b) const { return static_cast<uint8_t*>(base) + (size_t)b * n3() * es; }
+
+  // hbuf = K * packd for all B states
+  int demag_pass() {
+    if (fp64)
+      return launch_matvec_f64(K64, n, static_cast<const double*>(packd), (int64_t)n3(), static_cast<double*>(hbuf),
+                               (int64_t)n3(), B, stream);
+    return launch_matvec_f32(K32, n, static_cast<const float*>(packd), (int64_t)n3(), static_cast<float*>(hbuf),
+                             (int64_t)n3(), B, stream);
+  }
   int nblocks() const { return (n + 255) / 256; }
 
   // Descriptors go through a ring of pinned host slots copied asynchronously;
@@ -236,24 +259,25 @@ struct Rk4 {
   }
 
   // k[b] = f(Y[b]) for every state; S (K Y) is written to S_out[b] when given
-  int rhs(double* const* Y, double* const* k, float* const* S_out) {
+  int rhs(double* const* Y, double* const* k, void* const* S_out) {
     std::vector<VecDesc> vd(B);
     for (int b = 0; b < B; ++b) {
       VecDesc& d = vd[b];
       std::memset(&d, 0, sizeof(d));
-      d.dst = Y[b];  // identity combination, used for the FP32 pack only
+      // FP32: identity combination used for the pack only; FP64: copy into the contiguous input
+      d.dst = fp64 ? static_cast<double*>(hcol(packd, b)) : Y[b];
       d.base = Y[b];
       d.bcoef = 1.0;
       d.post_scale = 1.0;
-      d.pack_col = b;
+      d.pack_col = fp64 ? -1 : b;
     }
     const VecDesc* dvd = nullptr;
     int rc = upload(vd, &dvd, 0);
     if (rc) return rc;
-    PackSpec pk{kPackF32, packf, (int64_t)n3(), nullptr};
+    PackSpec pk{fp64 ? kPackNone : kPackF32, fp64 ? nullptr : static_cast<float*>(packd), (int64_t)n3(), nullptr};
     rc = launch_vec(dvd, B, n, nullptr, nullptr, pk, false, stream);
     if (rc) return rc;
-    rc = launch_matvec_f32(K32, n, packf, (int64_t)n3(), hbuf, (int64_t)n3(), B, stream);
+    rc = demag_pass();
     if (rc) return rc;
     std::vector<LlgDesc> ld(B);
     for (int b = 0; b < B; ++b) {
@@ -261,7 +285,8 @@ struct Rk4 {
       std::memset(&L, 0, sizeof(L));
       L.m = Y[b];
       L.out = k[b];
-      L.hpart = hbuf + (size_t)b * n3();
+      if (fp64) L.hpart64 = static_cast<const double*>(hcol(hbuf, b));
+      else L.hpart = static_cast<const float*>(hcol(hbuf, b));
       L.nsplit = 1;
       for (int c = 0; c < 3; ++c) L.hext[c] = hext[b * 3 + c];
       L.alpha = alpha;
@@ -274,8 +299,7 @@ struct Rk4 {
     if (rc) return rc;
     if (S_out)
       for (int b = 0; b < B; ++b)
-        MT_HIP(hipMemcpyAsync(S_out[b], hbuf + (size_t)b * n3(), n3() * sizeof(float), hipMemcpyDeviceToDevice,
-                              stream));
+        MT_HIP(hipMemcpyAsync(S_out[b], hcol(hbuf, b), n3() * es, hipMemcpyDeviceToDevice, stream));
     return kOk;
   }
 
@@ -313,10 +337,10 @@ struct Rk4 {
 
   double* seg_y(long long j) { return seg + (size_t)j * B * 5 * n3(); }                        // B x 3n
   double* seg_stage(long long j, int s) { return seg_y(j) + (size_t)(1 + s) * B * n3(); }    // stage input s
-  float* seg_S(long long j, int s) { return segS + ((size_t)j * 4 + s) * B * n3(); }
+  uint8_t* seg_S(long long j, int s) { return segS + ((size_t)j * 4 + s) * B * n3() * es; }
 
   // One RK4 step from y (B vectors) to ynext; records stage inputs / demag sums when asked.
-  int step(double* const* y, double* const* ynext, double* const* Yst /*4*B*/, float* const* Sst /*4*B*/) {
+  int step(double* const* y, double* const* ynext, double* const* Yst /*4*B*/, void* const* Sst /*4*B*/) {
     const size_t N3 = n3();
     auto W = [&](int k) { return vecs(work + (size_t)k * B * N3, N3); };
     auto k1 = W(0), k2 = W(1), k3 = W(2), k4 = W(3), tmp = W(4);
@@ -376,7 +400,7 @@ struct Rk4 {
   }
 
   // mu[b] = vjp of f at stage input Y with cotangent u; accumulates parameter gradients
-  int vjp(double* const* Y, float* const* S, double* const* u, double* const* mu) {
+  int vjp(double* const* Y, void* const* S, double* const* u, double* const* mu) {
     const size_t N3 = n3();
     auto mbar = vecs(work + (size_t)10 * B * N3, N3);
     auto hbar = vecs(work + (size_t)11 * B * N3, N3);
@@ -388,7 +412,7 @@ struct Rk4 {
       d.u = u[b];
       d.mbar = mbar[b];
       d.hbar = hbar[b];
-      d.packed = packf + (size_t)b * N3;
+      d.packed = hcol(packd, b);
       d.grad_ms = grad_ms + (size_t)b * n;
       d.hext_part = hext_part + (size_t)b * nblocks() * 3;
       for (int c = 0; c < 3; ++c) d.hext[c] = hext[b * 3 + c];
@@ -398,21 +422,23 @@ struct Rk4 {
     const VjpDesc* dvd = nullptr;
     int rc = upload(vd, &dvd, 3);
     if (rc) return rc;
-    hipLaunchKernelGGL(vjp_kernel, dim3(nblocks(), B), dim3(256), 0, stream, grid, dvd);
+    if (fp64) hipLaunchKernelGGL(vjp_kernel<double>, dim3(nblocks(), B), dim3(256), 0, stream, grid, dvd);
+    else hipLaunchKernelGGL(vjp_kernel<float>, dim3(nblocks(), B), dim3(256), 0, stream, grid, dvd);
     MT_HIP(hipGetLastError());
-    rc = launch_matvec_f32(K32, n, packf, (int64_t)N3, hbuf, (int64_t)N3, B, stream);
+    rc = demag_pass();
     if (rc) return rc;
     std::vector<FinishDesc> fd(B);
     for (int b = 0; b < B; ++b) {
       fd[b].mbar = mbar[b];
       fd[b].hbar = hbar[b];
-      fd[b].T = hbuf + (size_t)b * N3;
+      fd[b].T = hcol(hbuf, b);
       fd[b].mu = mu[b];
     }
     const FinishDesc* dfd = nullptr;
     rc = upload(fd, &dfd, 4);
     if (rc) return rc;
-    hipLaunchKernelGGL(vjp_finish_kernel, dim3(nblocks(), B), dim3(256), 0, stream, grid, dfd, nullptr);
+    if (fp64) hipLaunchKernelGGL(vjp_finish_kernel<double>, dim3(nblocks(), B), dim3(256), 0, stream, grid, dfd);
+    else hipLaunchKernelGGL(vjp_finish_kernel<float>, dim3(nblocks(), B), dim3(256), 0, stream, grid, dfd);
     MT_HIP(hipGetLastError());
     return kOk;
   }
@@ -442,11 +468,11 @@ struct Rk4 {
       if (rc) return rc;
       for (long long j = 0; j < len; ++j) {
         std::vector<double*> Yst(4 * B);
-        std::vector<float*> Sst(4 * B);
+        std::vector<void*> Sst(4 * B);
         for (int s = 0; s < 4; ++s)
           for (int b = 0; b < B; ++b) {
             Yst[s * B + b] = seg_stage(j, s) + (size_t)b * N3;
-            Sst[s * B + b] = seg_S(j, s) + (size_t)b * N3;
+            Sst[s * B + b] = seg_S(j, s) + (size_t)b * N3 * es;
           }
         auto yj = vecs(seg_y(j), N3);
         auto yn = vecs(seg_y(j + 1), N3);
@@ -461,8 +487,8 @@ struct Rk4 {
           return v;
         };
         auto sS = [&](int s) {
-          std::vector<float*> v(B);
-          for (int b = 0; b < B; ++b) v[b] = seg_S(j, s) + (size_t)b * N3;
+          std::vector<void*> v(B);
+          for (int b = 0; b < B; ++b) v[b] = seg_S(j, s) + (size_t)b * N3 * es;
           return v;
         };
         auto Y1 = st(0), Y2 = st(1), Y3 = st(2), Y4 = st(3);
@@ -528,11 +554,16 @@ int mt_rk4_create(const mt_micromag_problem* q, double dt, long long nsteps, int
   MT_CHECK(q->n_states >= 1 && q->n_states <= 1024, "n_states must be in [1, 1024]");
   MT_CHECK(dt > 0 && nsteps >= 1 && checkpoint_every >= 1, "need dt > 0, nsteps >= 1, checkpoint_every >= 1");
   MT_CHECK(q->Ms && q->hext && q->nt_hext >= 1, "Ms and an applied field are required");
+  MT_CHECK(q->precision == MT_PREC_FP32 || q->precision == MT_PREC_FP64,
+           "the RK4 adjoint runs the demag product in FP32 (MT_PREC_FP32) or FP64 (MT_PREC_FP64)");
+  MT_CHECK(q->precision == MT_PREC_FP32 || !q->K_in, "K_in is an FP32 tensor: use MT_PREC_FP32 with it");
   std::unique_ptr<Rk4> R(new Rk4());
   Rk4& P = *R;
   P.nx = q->nx; P.ny = q->ny; P.nz = q->nz; P.n = q->nx * q->ny * q->nz; P.B = q->n_states;
   P.dt = dt; P.nsteps = nsteps; P.ck = checkpoint_every;
   P.gamma = q->gamma; P.alpha = q->alpha0;
+  P.fp64 = q->precision == MT_PREC_FP64;
+  P.es = P.fp64 ? sizeof(double) : sizeof(float);
   P.hext.resize((size_t)P.B * 3);
   for (int b = 0; b < P.B; ++b) {
     const double* row = q->hext + (size_t)(q->hext_per_state ? b : 0) * q->nt_hext * 4;
@@ -545,7 +576,8 @@ int mt_rk4_create(const mt_micromag_problem* q, double dt, long long nsteps, int
   MT_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   // tensor (FP32, receiver-major) from the assembly kernel or the caller
   const size_t nn = (size_t)n * n;
-  MT_HIP(hipMalloc(&P.K32, 6 * nn * sizeof(float)));
+  if (P.fp64) MT_HIP(hipMalloc(&P.K64, 6 * nn * sizeof(double)));
+  else MT_HIP(hipMalloc(&P.K32, 6 * nn * sizeof(float)));
   if (q->K_in) {
     MT_HIP(hipMemcpyAsync(P.K32, q->K_in, 6 * nn * sizeof(float), hipMemcpyHostToDevice, P.stream));
   } else {
@@ -565,7 +597,7 @@ int mt_rk4_create(const mt_micromag_problem* q, double dt, long long nsteps, int
     MT_HIP(hipMalloc(&da, sizeof(abc)));
     MT_HIP(hipMemcpyAsync(dp, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
     MT_HIP(hipMemcpyAsync(da, abc, sizeof(abc), hipMemcpyHostToDevice, P.stream));
-    int rc = launch_assemble(dp, da, 0, nullptr, dp, n, n, P.K32, nullptr, nullptr, P.stream);
+    int rc = launch_assemble(dp, da, 0, nullptr, dp, n, n, P.K32, P.K64, nullptr, P.stream);
     if (rc) return rc;
     MT_HIP(hipStreamSynchronize(P.stream));
     (void)hipFree(dp);
@@ -584,6 +616,8 @@ int mt_rk4_create(const mt_micromag_problem* q, double dt, long long nsteps, int
   MT_HIP(hipMalloc(&P.jfact, n * sizeof(double)));
   MT_HIP(hipMalloc(&P.kfact, n * sizeof(double)));
   MT_HIP(hipMalloc(&P.ms32, n * sizeof(float)));
+  MT_HIP(hipMalloc(&P.ms64, n * sizeof(double)));
+  MT_HIP(hipMemcpyAsync(P.ms64, q->Ms, n * sizeof(double), hipMemcpyHostToDevice, P.stream));
   MT_HIP(hipMemcpyAsync(P.jfact, jf.data(), n * sizeof(double), hipMemcpyHostToDevice, P.stream));
   MT_HIP(hipMemcpyAsync(P.kfact, kf.data(), n * sizeof(double), hipMemcpyHostToDevice, P.stream));
   MT_HIP(hipMemcpyAsync(P.ms32, ms.data(), n * sizeof(float), hipMemcpyHostToDevice, P.stream));
@@ -599,13 +633,14 @@ int mt_rk4_create(const mt_micromag_problem* q, double dt, long long nsteps, int
   }
   P.grid.n = n; P.grid.nx = P.nx; P.grid.ny = P.ny; P.grid.nz = P.nz;
   P.grid.idx2 = 1.0 / (dx * dx); P.grid.idy2 = 1.0 / (dy * dy); P.grid.idz2 = 1.0 / (dz * dz);
-  P.grid.jfact = P.jfact; P.grid.kfact = P.kfact; P.grid.ms32 = P.ms32; P.grid.aniso = P.aniso;
+  P.grid.jfact = P.jfact; P.grid.kfact = P.kfact; P.grid.ms32 = P.ms32; P.grid.ms64 = P.ms64;
+  P.grid.aniso = P.aniso;
   P.nck = (int)((nsteps + P.ck - 1) / P.ck);
-  MT_HIP(hipMalloc(&P.packf, (size_t)P.B * N3 * sizeof(float)));
-  MT_HIP(hipMalloc(&P.hbuf, (size_t)P.B * N3 * sizeof(float)));
+  MT_HIP(hipMalloc(&P.packd, (size_t)P.B * N3 * P.es));
+  MT_HIP(hipMalloc(&P.hbuf, (size_t)P.B * N3 * P.es));
   MT_HIP(hipMalloc(&P.ckpt, (size_t)P.nck * P.B * N3 * sizeof(double)));
   MT_HIP(hipMalloc(&P.seg, (size_t)(P.ck + 1) * P.B * 5 * N3 * sizeof(double)));
-  MT_HIP(hipMalloc(&P.segS, (size_t)P.ck * 4 * P.B * N3 * sizeof(float)));
+  MT_HIP(hipMalloc(&P.segS, (size_t)P.ck * 4 * P.B * N3 * P.es));
   MT_HIP(hipMalloc(&P.work, (size_t)13 * P.B * N3 * sizeof(double)));
   MT_HIP(hipMalloc(&P.grad_ms, (size_t)P.B * n * sizeof(double)));
   MT_HIP(hipMalloc(&P.hext_part, (size_t)P.B * P.nblocks() * 3 * sizeof(double)));

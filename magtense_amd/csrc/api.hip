@@ -324,7 +324,7 @@ int mt_ll_rhs(int nx, int ny, int nz, double Lx, double Ly, double Lz, const dou
       L.fields = fields ? fields + (size_t)s * 12 * n : nullptr;
     }
     MT_HIP(hipMemcpy(dd, ld.data(), nstates * sizeof(LlgDesc), hipMemcpyHostToDevice));
-    LlgGrid g;
+    LlgGrid g{};
     g.n = n; g.nx = nx; g.ny = ny; g.nz = nz;
     g.idx2 = 1.0 / (dx * dx); g.idy2 = 1.0 / (dy * dy); g.idz2 = 1.0 / (dz * dz);
     g.jfact = djf; g.kfact = dkf; g.ms32 = dms; g.aniso = dan;
@@ -336,6 +336,55 @@ int mt_ll_rhs(int nx, int ny, int nz, double Lx, double Ly, double Lz, const dou
   if (dan) (void)hipFree(dan);
   return rc;
 }
+
+// ---- threshold-sparse demag tensor -----------------------------------------------
+int mt_demag_sparse_create(const float* K32, int n, int approx, double threshold, void** out) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(out && K32 && n > 0, "invalid sparse tensor arguments");
+  MT_CHECK(approx == MT_DEMAG_THRESHOLD || approx == MT_DEMAG_THRESHOLD_FRACTION,
+           "approx must be MT_DEMAG_THRESHOLD or MT_DEMAG_THRESHOLD_FRACTION");
+  *out = nullptr;
+  float thr = (float)threshold;
+  if (approx == MT_DEMAG_THRESHOLD_FRACTION) {
+    rc = threshold_for_fraction(K32, n, threshold, &thr, nullptr);
+    if (rc) return rc;
+  }
+  SparseDemag* sp = new SparseDemag();
+  rc = sp->build(K32, n, thr, nullptr);
+  if (rc) {
+    delete sp;
+    return rc;
+  }
+  *out = sp;
+  return kOk;
+}
+
+int mt_demag_sparse_info(void* h, long long* nnz6, float* threshold) {
+  MT_CHECK(h, "null sparse tensor");
+  SparseDemag* sp = static_cast<SparseDemag*>(h);
+  for (int c = 0; c < 6; ++c)
+    if (nnz6) nnz6[c] = sp->nnz[c];
+  if (threshold) *threshold = sp->threshold;
+  return kOk;
+}
+
+int mt_demag_apply_sparse(void* h, const float* m, float* hout, int nstates, float pref) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(h && m && hout && nstates > 0, "invalid sparse product arguments");
+  SparseDemag* sp = static_cast<SparseDemag*>(h);
+  const int64_t n3 = (int64_t)3 * sp->n;
+  for (int s0 = 0; s0 < nstates; s0 += kMaxCols) {
+    int S = nstates - s0 < kMaxCols ? nstates - s0 : kMaxCols;
+    rc = sp->apply(m + s0 * n3, n3, S, pref, hout + s0 * n3, nullptr);
+    if (rc) return rc;
+  }
+  MT_HIP(hipStreamSynchronize(nullptr));
+  return kOk;
+}
+
+void mt_demag_sparse_destroy(void* h) { delete static_cast<SparseDemag*>(h); }
 
 // ---- magnetostatic prism tiles (getNFromTiles / getHFromTiles) ---------------
 }  // extern "C"
@@ -455,6 +504,8 @@ std::mutex g_icl_mutex;
 float* g_icl_K = nullptr;  // receiver-major FP32 planes
 float* g_icl_io = nullptr; // 6n floats: m (3n) then h (3n)
 int g_icl_n = 0;
+SparseDemag* g_icl_sparse = nullptr;  // sparse plug-in tensor (components loaded one by one)
+float* g_icl_sp_io = nullptr;         // 6n floats: m (3n) then h (3n)
 }  // namespace
 
 void icl_initDemagMatrices(const float* Kxx, const float* Kxy, const float* Kxz, const float* Kyy, const float* Kyz,
@@ -497,12 +548,50 @@ void icl_MVMult_GetH(const float* Mx, const float* My, const float* Mz, float* H
   (void)hipMemcpy(Hz, dh + 2 * (size_t)N, N * sizeof(float), hipMemcpyDeviceToHost);
 }
 
+void icl_initDemagMatrices_sparse(int* n, int* nnz, int* mat_no, float* values, int* colInds, int* rowInds) {
+  std::lock_guard<std::mutex> lock(g_icl_mutex);
+  if (device_ok() != kOk || !n || !nnz || !mat_no || !rowInds || *n <= 0) return;
+  if (!g_icl_sparse) g_icl_sparse = new SparseDemag();
+  if (g_icl_sparse->n != 0 && g_icl_sparse->n != *n) {
+    set_error("icl_initDemagMatrices_sparse: size differs from the loaded components");
+    return;
+  }
+  if (!g_icl_sp_io && hipMalloc(&g_icl_sp_io, 6 * (size_t)(*n) * sizeof(float)) != hipSuccess) {
+    set_error("icl_initDemagMatrices_sparse: allocation failed");
+    return;
+  }
+  (void)g_icl_sparse->load_component(*mat_no - 1, *n, *nnz, values, colInds, rowInds, nullptr);
+}
+
+void icl_MVMult_GetH_sparse(const float* Mx, const float* My, const float* Mz, float* Hx, float* Hy, float* Hz,
+                            const int* n, const float* pref) {
+  std::lock_guard<std::mutex> lock(g_icl_mutex);
+  if (!g_icl_sparse || !g_icl_sp_io || !n || *n != g_icl_sparse->n) {
+    set_error("icl_MVMult_GetH_sparse: icl_initDemagMatrices_sparse not called for this size");
+    return;
+  }
+  const int N = *n;
+  float* dm = g_icl_sp_io;
+  float* dh = g_icl_sp_io + 3 * (size_t)N;
+  (void)hipMemcpy(dm, Mx, N * sizeof(float), hipMemcpyHostToDevice);
+  (void)hipMemcpy(dm + N, My, N * sizeof(float), hipMemcpyHostToDevice);
+  (void)hipMemcpy(dm + 2 * (size_t)N, Mz, N * sizeof(float), hipMemcpyHostToDevice);
+  if (g_icl_sparse->apply(dm, 3 * (int64_t)N, 1, *pref, dh, nullptr) != kOk) return;
+  (void)hipMemcpy(Hx, dh, N * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipMemcpy(Hy, dh + N, N * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipMemcpy(Hz, dh + 2 * (size_t)N, N * sizeof(float), hipMemcpyDeviceToHost);
+}
+
 void icl_destroy(void) {
   std::lock_guard<std::mutex> lock(g_icl_mutex);
   if (g_icl_K) (void)hipFree(g_icl_K);
   if (g_icl_io) (void)hipFree(g_icl_io);
+  if (g_icl_sp_io) (void)hipFree(g_icl_sp_io);
+  delete g_icl_sparse;
   g_icl_K = nullptr;
   g_icl_io = nullptr;
+  g_icl_sp_io = nullptr;
+  g_icl_sparse = nullptr;
   g_icl_n = 0;
 }
 

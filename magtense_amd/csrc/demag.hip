@@ -375,6 +375,51 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
 
 }  // namespace
 
+// FP64 product for the gradient-check mode of the RK4 adjoint: one wavefront
+// per (receiver row, state), lanes stride over the sources, the six planes'
+// rows are read once for the three output components.
+__global__ void __launch_bounds__(256) matvec_f64_kernel(const double* __restrict__ K, int n,
+                                                         const double* __restrict__ m, int64_t m_stride,
+                                                         double* __restrict__ out, int64_t out_stride) {
+  const int s = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const size_t nn = (size_t)n * n;
+  const double* row = K + (size_t)i * n;
+  const double* mx = m + (int64_t)s * m_stride;
+  const double* my = mx + n;
+  const double* mz = my + n;
+  double hx = 0.0, hy = 0.0, hz = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const double xx = row[j], xy = row[nn + j], xz = row[2 * nn + j];
+    const double yy = row[3 * nn + j], yz = row[4 * nn + j], zz = row[5 * nn + j];
+    const double a = mx[j], b = my[j], c = mz[j];
+    hx += xx * a + xy * b + xz * c;
+    hy += xy * a + yy * b + yz * c;
+    hz += xz * a + yz * b + zz * c;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    hx += __shfl_xor(hx, o, 64);
+    hy += __shfl_xor(hy, o, 64);
+    hz += __shfl_xor(hz, o, 64);
+  }
+  if (lane == 0) {
+    double* o = out + (int64_t)s * out_stride;
+    o[i] = hx;
+    o[n + i] = hy;
+    o[2 * n + i] = hz;
+  }
+}
+
+int launch_matvec_f64(const double* K, int n, const double* m, int64_t m_stride, double* out, int64_t out_stride,
+                      int nstates, hipStream_t stream) {
+  dim3 grid((unsigned)((n + 3) / 4), (unsigned)nstates);
+  hipLaunchKernelGGL(matvec_f64_kernel, grid, dim3(256), 0, stream, K, n, m, m_stride, out, out_stride);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
 int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, float* out, int64_t out_stride,
                       int nstates, hipStream_t stream) {
   constexpr int RB = 4;

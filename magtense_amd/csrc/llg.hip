@@ -56,11 +56,15 @@ __global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc
     hj[c] = (-2.0 * lap) * g.jfact[i];
   }
   // demag: S = sum_k partial (K m); field = fl32(fl32(-S) * Ms_i)
-  for (int c = 0; c < 3; ++c) {
-    float s = 0.0f;
-    for (int k = 0; k < D.nsplit; ++k) s += D.hpart[(int64_t)k * D.split_stride + (int64_t)c * n + i];
-    float h = (-s) * g.ms32[i];
-    hm[c] = (double)h;
+  if (D.hpart64) {
+    for (int c = 0; c < 3; ++c) hm[c] = (-D.hpart64[(int64_t)c * n + i]) * g.ms64[i];
+  } else {
+    for (int c = 0; c < 3; ++c) {
+      float s = 0.0f;
+      for (int k = 0; k < D.nsplit; ++k) s += D.hpart[(int64_t)k * D.split_stride + (int64_t)c * n + i];
+      float h = (-s) * g.ms32[i];
+      hm[c] = (double)h;
+    }
   }
   if (g.aniso) {
     const double* A = g.aniso + (size_t)i * 6;  // xx xy xz yy yz zz

@@ -1,6 +1,8 @@
 // Kernel descriptors and launchers shared by the host runtime.
 #pragma once
 
+#include <vector>
+
 #include "mt_common.h"
 
 namespace mt {
@@ -19,10 +21,39 @@ int launch_split(const float* K, int n, uint8_t* split, hipStream_t stream);
 // out[s*out_stride + c*n + i] = sum_j K_c.(i,j) m_s,.(j)   (positive K m)
 int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, float* out, int64_t out_stride,
                       int nstates, hipStream_t stream);
+// FP64 form of the same product (FP64 tensor planes and states)
+int launch_matvec_f64(const double* K, int n, const double* m, int64_t m_stride, double* out, int64_t out_stride,
+                      int nstates, hipStream_t stream);
 int gemm_ksplit_for(int n, int waves);
 // out[ks*split_stride + col*col_stride + c*n + i] = k-split partial of K m
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
+
+// Threshold-sparse demag tensor: six CSR matrices (xx xy xz yy yz zz), rows =
+// receivers, FP32 values, entries with |K| > threshold.
+struct SparseDemag {
+  int n = 0;
+  float threshold = 0.0f;
+  int64_t nnz[6] = {0, 0, 0, 0, 0, 0};
+  int64_t* rowptr = nullptr;  // 6 x (n+1), per component
+  int64_t* coff = nullptr;    // 6 component offsets into cols/vals (device)
+  int* cols = nullptr;
+  float* vals = nullptr;
+  float* mT = nullptr;        // transposed staging for the many-state product
+  float* hT = nullptr;
+  int tcols = 0;
+  std::vector<std::vector<int>> host_cols;    // components loaded from host CSR
+  std::vector<std::vector<float>> host_vals;
+  int build(const float* K32, int n, float thr, hipStream_t stream);
+  int load_component(int c, int n, int nnz, const float* values, const int* colInds, const int* rowInds,
+                     hipStream_t stream);
+  // h[s*ld + a*n + i] = pref * (K m_s)_a(i), s < S <= 64, m and h device [S][3n] with row stride ld
+  int apply(const float* m, int64_t ld, int S, float pref, float* h, hipStream_t stream);
+  void release();
+  ~SparseDemag() { release(); }
+};
+// FindThresholdFraction over the positive entries of the FP32 receiver-major tensor.
+int threshold_for_fraction(const float* K32, int n, double fraction, float* thr, hipStream_t stream);
 
 // ---- (c) LL right-hand side ------------------------------------------------------
 struct LlgGrid {
@@ -31,6 +62,7 @@ struct LlgGrid {
   const double* jfact;       // A0 / (mu0 Ms_i)
   const double* kfact;       // K0_i / (mu0 Ms_i)
   const float* ms32;         // Ms_i in FP32 (demag scaling)
+  const double* ms64;        // Ms_i in FP64 (FP64 demag mode; may be null)
   const double* aniso;       // 6 per cell (u u^T), or null when K0 == 0 everywhere
 };
 
@@ -40,6 +72,7 @@ struct LlgDesc {
   const float* hpart;        // demag partial sums for this state's column
   int64_t split_stride;      // distance between k-split partials
   int nsplit;
+  const double* hpart64;     // FP64 demag sums (3n, replaces hpart when set): H_m = -S * Ms (FP64)
   double hext[3];            // applied field (A/m) at the stage time
   double alpha, gamma;
   double* fields;            // optional 12n: H_exc, H_ext, H_dem, H_ani (negated, SoA)

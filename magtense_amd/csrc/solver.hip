@@ -283,6 +283,8 @@ struct Solver {
   hipStream_t stream = nullptr;
   float* K32 = nullptr;
   uint8_t* Ksplit = nullptr;
+  int demag_approx = 1;       // 1 dense, 2 threshold, 4 threshold fraction
+  SparseDemag sparse;
   double *jfact = nullptr, *kfact = nullptr, *aniso = nullptr;
   float* ms32 = nullptr;
   double* pool = nullptr;     // state vectors
@@ -318,6 +320,7 @@ struct Solver {
     };
     f(K32); f(Ksplit); f(jfact); f(kfact); f(aniso); f(ms32); f(pool); f(yout); f(fout); f(packf); f(packb);
     f(hbuf); f(partials); f(results); f(arena_d);
+    sparse.release();
     if (h_results) (void)hipHostFree(h_results);
     if (arena_h) (void)hipHostFree(arena_h);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -1005,6 +1008,8 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     if (timing) MT_HIP(hipEventRecord(ev0, stream));
     if (precision == MT_PREC_BF16X2) {
       rc = launch_gemm_bf16x2(Ksplit, packb, n, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit, stream);
+    } else if (demag_approx != MT_DEMAG_DENSE) {
+      rc = sparse.apply(packf, (int64_t)N3, S, 1.0f, hbuf, stream);
     } else {
       rc = launch_matvec_f32(K32, n, packf, (int64_t)N3, hbuf, (int64_t)N3, S, stream);
     }
@@ -1126,6 +1131,11 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   MT_CHECK(q->nt_hext >= 1 && q->hext, "need an applied-field table");
   MT_CHECK(q->solver == 1 || q->solver == 2, "solver must be 1 (explicit) or 2 (dynamic)");
   MT_CHECK(q->precision == MT_PREC_FP32 || q->precision == MT_PREC_BF16X2, "unknown precision");
+  const int approx = q->demag_approx == 0 ? MT_DEMAG_DENSE : q->demag_approx;
+  MT_CHECK(approx == MT_DEMAG_DENSE || approx == MT_DEMAG_THRESHOLD || approx == MT_DEMAG_THRESHOLD_FRACTION,
+           "demag_approx must be 1 (dense), 2 (threshold) or 4 (threshold fraction)");
+  MT_CHECK(approx == MT_DEMAG_DENSE || q->precision == MT_PREC_FP32,
+           "the threshold-sparse demag product runs in FP32 (precision MT_PREC_FP32)");
   MT_CHECK(q->Ms && q->m0, "Ms and m0 are required");
   std::unique_ptr<Solver> S(new Solver());
   Solver& P = *S;
@@ -1134,6 +1144,7 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   P.B = q->n_states;
   P.solver = q->solver;
   P.precision = q->precision;
+  P.demag_approx = approx;
   P.gamma = q->gamma; P.alpha0 = q->alpha0;
   P.tol = q->tol; P.thres = q->thres; P.conv_tol = q->conv_tol;
   for (int k = 0; k < q->nt_alpha; ++k) {
@@ -1205,6 +1216,21 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   if (q->K_out) {
     MT_CHECK(P.K32 != nullptr, "K_out needs the FP32 tensor (precision fp32 or keep_fp32_tensor)");
     MT_HIP(hipMemcpyAsync(q->K_out, P.K32, 6 * nn * sizeof(float), hipMemcpyDeviceToHost, P.stream));
+  }
+  if (approx != MT_DEMAG_DENSE) {
+    // ApplyThresholdDense (DemagAuxFunctions.f90:20-122) after the tensor (LandauLifshitzEquationSolver.f90:1157)
+    float thr = (float)q->demag_threshold;
+    if (approx == MT_DEMAG_THRESHOLD_FRACTION) {
+      int rc = threshold_for_fraction(P.K32, n, q->demag_threshold, &thr, P.stream);
+      if (rc) return rc;
+    }
+    int rc = P.sparse.build(P.K32, n, thr, P.stream);
+    if (rc) return rc;
+    if (!q->keep_fp32_tensor) {
+      MT_HIP(hipStreamSynchronize(P.stream));
+      MT_HIP(hipFree(P.K32));
+      P.K32 = nullptr;
+    }
   }
   // per-cell factors (LandauLifshitzEquationSolver.f90:416-425)
   std::vector<double> jf(n), kf(n), an;
@@ -1383,6 +1409,16 @@ int mt_solver_set_timing(mt_solver* h, int on) {
   P.timing = on != 0;
   P.gemm_ms = 0.0;
   return 0;
+}
+
+int mt_solver_demag_info(mt_solver* h, long long* nnz6, float* threshold) {
+  MT_CHECK(h, "null solver");
+  mt::Solver& P = *reinterpret_cast<mt::Solver*>(h);
+  MT_CHECK(P.demag_approx != MT_DEMAG_DENSE, "the solver uses the dense demag tensor");
+  for (int c = 0; c < 6; ++c)
+    if (nnz6) nnz6[c] = P.sparse.nnz[c];
+  if (threshold) *threshold = P.sparse.threshold;
+  return mt::kOk;
 }
 
 void mt_solver_destroy(mt_solver* h) { delete reinterpret_cast<Solver*>(h); }

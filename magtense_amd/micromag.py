@@ -20,9 +20,10 @@ Additive extensions (no reference counterpart):
     (the reference's f2py wrapper declares only one row).
 
 Behaviour that differs on purpose: ``cuda`` is accepted but ignored (this
-backend always runs on the GPU and raises when none is visible); the
-``demag_approx`` modes other than ``None`` and the non-uniform grid types raise
-``NotImplementedError``; the demag tensor file (``filename`` longer than two
+backend always runs on the GPU and raises when none is visible); the FFT
+``demag_approx`` modes ('fft_thres', 'fft_threshold_fraction') and the
+non-uniform grid types raise ``NotImplementedError`` ('threshold' and
+'threshold_fraction' build the CSR tensor on the device); the demag tensor file (``filename`` longer than two
 characters) is loaded when it exists and written otherwise.
 """
 
@@ -210,8 +211,11 @@ class MicromagProblem:
     def _check_supported(self):
         if self._grid_type != 1:
             raise NotImplementedError("only grid_type='uniform' is implemented on the MI355X backend")
-        if self._dem_appr != 1:
-            raise NotImplementedError("only the dense demag tensor (demag_approx=None) is implemented")
+        if self._dem_appr in (3, 5):
+            raise NotImplementedError("the FFT-threshold demag approximations are not implemented "
+                                      "(demag_approx None, 'threshold' and 'threshold_fraction' are)")
+        if self._dem_appr in (2, 4) and self._precision_code() != _lib.PREC_FP32:
+            raise ValueError("the threshold-sparse demag product runs in FP32: use precision='fp32'")
         if self._solver not in (1, 2):
             raise NotImplementedError("solver must be 'explicit' or 'dynamic'")
         if self.CV > 0:
@@ -270,6 +274,8 @@ class MicromagProblem:
         q.precision = self._precision_code()
         q.return_fields = int(bool(return_fields))
         q.max_steps = int(max_steps)
+        q.demag_approx = int(self._dem_appr)
+        q.demag_threshold = float(self.dem_thres)
         path = self._tensor_file()
         Kout = None
         if path is not None and os.path.exists(path):
@@ -283,9 +289,21 @@ class MicromagProblem:
         y_out = np.zeros((B, n_h, nt, 3, n))
         fields = np.zeros((B, n_h, nt, 4, 3, n)) if return_fields else None
         stats = (_lib.MtStats * B)()
-        rc = L.mt_run_micromag(C.byref(q), t_out.ctypes.data, y_out.ctypes.data,
-                               fields.ctypes.data if fields is not None else None, C.cast(stats, C.c_void_p))
-        _lib.check(rc)
+        h = C.c_void_p()
+        _lib.check(L.mt_solver_create(C.byref(q), C.byref(h)))
+        try:
+            _lib.check(L.mt_solver_run(h))
+            _lib.check(L.mt_solver_results(h, t_out.ctypes.data, y_out.ctypes.data,
+                                           fields.ctypes.data if fields is not None else None,
+                                           C.cast(stats, C.c_void_p)))
+            self.last_demag_info = None
+            if self._dem_appr in (2, 4):
+                nnz = (C.c_longlong * 6)()
+                thr = C.c_float()
+                _lib.check(L.mt_solver_demag_info(h, nnz, C.byref(thr)))
+                self.last_demag_info = dict(nnz=[int(v) for v in nnz], threshold=float(thr.value))
+        finally:
+            L.mt_solver_destroy(h)
         if Kout is not None:
             save_demag_tensor(path, Kout)
         self.last_stats = [dict(f_calls=s.f_calls, steps=s.steps, bad_steps=s.bad_steps, flag3=s.flag3,

@@ -292,6 +292,7 @@ struct Problem {
   std::vector<double> Axx, Axy, Axz, Ayy, Ayz, Azz;
   std::vector<double> Jfact, Kfact;
   std::vector<float> Mfact;
+  std::vector<double> Ms64;  // FP64 Ms: the demag scaling of the FP64 mode
   double gamma, alpha0;
   std::vector<double> alpha_t, alpha_v;
   int solver;
@@ -301,6 +302,7 @@ struct Problem {
   std::vector<double> Hj, Hh, Hk;
   std::vector<float> Hm, ms;
   long long rhs_calls = 0;
+  float demag_threshold = 0.0f;  // threshold applied to K (sparse modes)
 };
 
 double alpha_at(const Problem& P, double t) {
@@ -363,7 +365,7 @@ void update_fields(Problem& P, const double* m, double t) {
   }
 }
 
-// FP64 demag field (used when demag_fp64): H = -Ms_i * sum_j K m
+// FP64 demag field (used when demag_fp64): H = -Ms_i * sum_j K m, all in FP64
 void demag_fp64(const Problem& P, const double* m, double* H) {
   int n = P.n;
   size_t nn = (size_t)n * n;
@@ -377,7 +379,7 @@ void demag_fp64(const Problem& P, const double* m, double* H) {
         const double* v = m + (size_t)b * n;
         for (int j = 0; j < n; ++j) s += Krow[j] * v[j];
       }
-      H[(size_t)a * n + i] = -s * (double)P.Mfact[i];
+      H[(size_t)a * n + i] = -s * P.Ms64[i];
     }
 }
 
@@ -1125,6 +1127,8 @@ typedef struct {
   int demag_fp64;        // 0: FP32 demag product as the reference; 1: FP64 product from FP64 tensor
   const float* K_in;     // optional 6*n*n FP32 tensor (receiver-major); null: assemble
   int return_fields;     // 1: fill H_exc/H_ext/H_dem/H_ani
+  int demag_approx;      // 0/1 dense, 2 threshold, 4 threshold fraction
+  double demag_threshold;
 } or_problem;
 
 typedef struct {
@@ -1135,6 +1139,44 @@ typedef struct {
 }  // extern "C"
 
 namespace {
+
+// FindThresholdFraction (DemagAuxFunctions.f90:353-428) over the positive
+// entries of the FP32 tensor (ApplyThresholdDense packs K > 0, :35-105):
+// FP32 bisection; counts in 64-bit integers.
+float threshold_fraction(const float* K, size_t total, double fraction) {
+  const float thr = (float)fraction;  // sngl(dem_thres), MagTenseMicroMagPyIO.f90:117-118
+  if (thr >= 1.0f) return FLT_MAX;
+  long long npos = 0;
+  float f_large = 0.0f, f_small = FLT_MAX;
+  for (size_t k = 0; k < total; ++k)
+    if (K[k] > 0.0f) {
+      ++npos;
+      f_large = std::max(f_large, K[k]);
+      f_small = std::min(f_small, K[k]);
+    }
+  if (npos == 0) return -1.0f;
+  const float nf = (float)npos;
+  float f_middle = f_small;
+  for (int k_do = 1;; ++k_do) {
+    f_middle = (f_large - f_small) / 2.0f + f_small;
+    long long cm = 0;
+#pragma omp parallel for reduction(+ : cm)
+    for (long long k = 0; k < (long long)total; ++k) cm += (K[k] > 0.0f && K[k] <= f_middle);
+    const float cmf = (float)cm;
+    if (cmf > thr * nf) f_large = f_middle;
+    else f_small = f_middle;
+    if (cmf >= (thr - 0.01f) * nf && cmf <= (thr + 0.01f) * nf) break;
+    if (k_do > 1000) break;
+  }
+  return f_middle;
+}
+
+// ConvertDenseToSparse_s (util_micromag.f90:81-138): entries with |K| <= thr
+// are dropped; the dense product over the remaining entries equals the CSR one.
+void apply_threshold(std::vector<float>& K, float thr) {
+  for (auto& v : K)
+    if (!(std::fabs(v) > thr)) v = 0.0f;
+}
 
 int build_problem(const or_problem* q, Problem& P) {
   if (q->nx < 2 || q->ny < 2) return -1;
@@ -1154,12 +1196,22 @@ int build_problem(const or_problem* q, Problem& P) {
   } else {
     P.K.assign(q->K_in, q->K_in + 6 * nn);
   }
+  if (q->demag_approx == 2 || q->demag_approx == 4) {
+    if (P.demag_fp64) return -3;
+    float thr = (float)q->demag_threshold;
+    if (q->demag_approx == 4) thr = threshold_fraction(P.K.data(), P.K.size(), q->demag_threshold);
+    if (thr < 0.0f) return -3;
+    apply_threshold(P.K, thr);
+    P.demag_threshold = thr;
+  } else if (q->demag_approx != 0 && q->demag_approx != 1) {
+    return -3;
+  }
   P.Axx.resize(n); P.Axy.resize(n); P.Axz.resize(n); P.Ayy.resize(n); P.Ayz.resize(n); P.Azz.resize(n);
   for (int i = 0; i < n; ++i) {
     double ux = q->u_ea ? q->u_ea[i * 3] : 0, uy = q->u_ea ? q->u_ea[i * 3 + 1] : 0, uz = q->u_ea ? q->u_ea[i * 3 + 2] : 0;
     P.Axx[i] = ux * ux; P.Axy[i] = ux * uy; P.Axz[i] = ux * uz; P.Ayy[i] = uy * uy; P.Ayz[i] = uy * uz; P.Azz[i] = uz * uz;
   }
-  P.Jfact.resize(n); P.Kfact.resize(n); P.Mfact.resize(n);
+  P.Jfact.resize(n); P.Kfact.resize(n); P.Mfact.resize(n); P.Ms64.assign(q->Ms, q->Ms + n);
   for (int i = 0; i < n; ++i) {
     P.Jfact[i] = q->A0 / (kMu0 * q->Ms[i]);
     P.Mfact[i] = (float)q->Ms[i];
@@ -1417,7 +1469,7 @@ void vjp_adj(AdjCtx& A, const double* m, const double* u, double* mu, double* gh
     cxu[0] = c[1] * uu[2] - c[2] * uu[1]; cxu[1] = c[2] * uu[0] - c[0] * uu[2]; cxu[2] = c[0] * uu[1] - c[1] * uu[0];
     hxc[0] = HH[1] * cb[2] - HH[2] * cb[1]; hxc[1] = HH[2] * cb[0] - HH[0] * cb[2]; hxc[2] = HH[0] * cb[1] - HH[1] * cb[0];
     double hbr[3] = {-(cb[1] * mm[2] - cb[2] * mm[1]), -(cb[2] * mm[0] - cb[0] * mm[2]), -(cb[0] * mm[1] - cb[1] * mm[0])};
-    double ms = (double)P.Mfact[r];
+    double ms = P.demag_fp64 ? P.Ms64[r] : (double)P.Mfact[r];
     double g = 0.0;
     for (int k = 0; k < 3; ++k) {
       mu[(size_t)k * n + r] = -P.alpha0 * cxu[k] - hxc[k];
@@ -1440,7 +1492,7 @@ void vjp_adj(AdjCtx& A, const double* m, const double* u, double* mu, double* gh
     for (int b = 0; b < 3; ++b) {
       int c6 = comp[a][b];
       for (int i = 0; i < n; ++i) {
-        double w = -(double)P.Mfact[i] * hb[(size_t)a * n + i];
+        double w = -(P.demag_fp64 ? P.Ms64[i] : (double)P.Mfact[i]) * hb[(size_t)a * n + i];
         if (P.demag_fp64) {
           const double* Krow = P.Kd.data() + c6 * nn + (size_t)i * n;
           for (int j = 0; j < n; ++j) mu[(size_t)b * n + j] += Krow[j] * w;
@@ -1692,6 +1744,15 @@ int or_tiles_field(int nt, const double* size, const double* offset, const doubl
     for (int a = 0; a < 3; ++a) H[p + (size_t)np * a] = acc[a];
   }
   return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// FindThresholdFraction over the positive entries of a 6*n*n FP32 tensor
+float or_threshold_fraction(const float* K, long long total, double fraction) {
+  return threshold_fraction(K, (size_t)total, fraction);
 }
 
 }  // extern "C"

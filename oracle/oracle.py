@@ -37,6 +37,8 @@ class OrProblem(C.Structure):
         ("demag_fp64", C.c_int),
         ("K_in", _fp),
         ("return_fields", C.c_int),
+        ("demag_approx", C.c_int),
+        ("demag_threshold", C.c_double),
     ]
 
 
@@ -73,6 +75,8 @@ def lib():
         _lib.or_rk4_adjoint.argtypes = [C.POINTER(OrProblem), C.c_double, C.c_longlong, _dp, _dp, _dp, _dp, _dp,
                                         _dp, _dp]
         _lib.or_rotation_matrices.argtypes = [_dp, _dp, _dp]
+        _lib.or_threshold_fraction.argtypes = [_fp, C.c_longlong, C.c_double]
+        _lib.or_threshold_fraction.restype = C.c_float
         _lib.or_tiles_tensor.argtypes = [C.c_int, _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, _dp, _dp]
         _lib.or_tiles_field.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp]
     return _lib
@@ -138,7 +142,7 @@ class Problem:
 
     def __init__(self, res, L, Ms=8e5, A0=1.3e-11, K0=0.0, gamma=0.0, alpha=0.02, u_ea=None, solver=2,
                  hext=None, alphat=None, t=None, m0=None, tol=1e-4, thres=1e-6, t_conv=None, conv_tol=1e-4,
-                 demag_fp64=False, K_in=None, return_fields=True):
+                 demag_fp64=False, K_in=None, return_fields=True, demag_approx=1, dem_thres=0.0):
         n = int(np.prod(res))
         self.n = n
         self.res = list(res)
@@ -167,6 +171,8 @@ class Problem:
         q.demag_fp64 = int(bool(demag_fp64))
         q.K_in = self._K.ctypes.data_as(_fp) if self._K is not None else None
         q.return_fields = int(bool(return_fields))
+        q.demag_approx = int(demag_approx)
+        q.demag_threshold = float(dem_thres)
         self.q = q
 
     def dmdt(self, t, m):
@@ -298,3 +304,9 @@ def tiles_field(size, offset, M, pts, rot=None, tile_type=None, use_sym=None, sy
     if rc:
         raise ValueError("only prism tiles (tile_type 2) are restated")
     return H
+
+
+def threshold_fraction(K, fraction):
+    """FindThresholdFraction over the positive entries of an FP32 tensor (any shape)."""
+    K = np.ascontiguousarray(K, dtype=np.float32)
+    return float(lib().or_threshold_fraction(K.ctypes.data_as(_fp), K.size, float(fraction)))

@@ -43,8 +43,12 @@ def test_m0_setter_forms():
 
 
 def test_unsupported_modes_raise_before_any_gpu_call():
-    p = MicromagProblem([4, 4, 1], demag_approx="threshold")
+    p = MicromagProblem([4, 4, 1], demag_approx="fft_thres")
     with pytest.raises(NotImplementedError):
+        p._check_supported()
+    MicromagProblem([4, 4, 1], demag_approx="threshold", dem_thres=1e-3)._check_supported()
+    p = MicromagProblem([4, 4, 1], demag_approx="threshold_fraction", dem_thres=0.5, precision="bf16x2")
+    with pytest.raises(ValueError):
         p._check_supported()
     p = MicromagProblem([1, 4, 1])
     with pytest.raises(ValueError):

@@ -1,0 +1,138 @@
+"""GPU parity of the threshold-sparse demag path against the oracle.
+
+* CSR build: per-component entry counts equal numpy's count of |K| > t on the
+  same FP32 tensor (integer, exact); the fraction threshold equals the
+  oracle's FP32 bisection exactly (same arithmetic, integer counts).
+* Product: 1e-6 relative (norm-wise) to the FP64 product over the kept
+  entries, for 1, 3, 5 and 64 states (both kernel shapes).
+* Reference plug-in ABI (icl_*_sparse, 1-based host CSR): same bar.
+* Solver with demag_approx 'threshold' / 'threshold_fraction': trajectories
+  within 1e-4 of the oracle run with the same threshold.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MU0 = 4e-7 * np.pi
+COMP = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+
+
+def _tensor(res, L):
+    import torch
+    from oracle import oracle as O
+    from magtense_amd import _lib
+
+    Lb = _lib.lib()
+    pts, d = O.setup_grid(res, L)
+    n = pts.shape[0]
+    dp = torch.as_tensor(pts).cuda()
+    da = torch.as_tensor(d).cuda()
+    K32 = torch.empty(6 * n * n, dtype=torch.float32, device="cuda")
+    _lib.check(Lb.mt_prism_tensor(dp.data_ptr(), da.data_ptr(), 0, None, dp.data_ptr(), n, n, K32.data_ptr(), None,
+                                  None))
+    return n, K32
+
+
+def _masked_product(K, thr, m):
+    S, _, n = m.shape
+    Km = np.where(np.abs(K) > thr, K, 0).astype(np.float64)
+    out = np.zeros((S, 3, n))
+    for a in range(3):
+        for b in range(3):
+            out[:, a] += m[:, b] @ Km[COMP[a][b]].T
+    return out
+
+
+@pytest.mark.parametrize("approx,value", [(2, 2e-3), (4, 0.9), (4, 0.5)])
+def test_sparse_build_and_product(oracle, approx, value):
+    import torch
+    from magtense_amd import _lib
+
+    Lb = _lib.lib()
+    res, L = [16, 8, 2], [32e-9, 16e-9, 4e-9]
+    n, K32 = _tensor(res, L)
+    K = K32.cpu().numpy().reshape(6, n, n)
+    h = C.c_void_p()
+    _lib.check(Lb.mt_demag_sparse_create(K32.data_ptr(), n, approx, value, C.byref(h)))
+    try:
+        nnz = (C.c_longlong * 6)()
+        thr = C.c_float()
+        _lib.check(Lb.mt_demag_sparse_info(h, nnz, C.byref(thr)))
+        t = np.float32(value) if approx == 2 else np.float32(oracle.threshold_fraction(K, value))
+        assert thr.value == t
+        expect = [int(np.count_nonzero(np.abs(K[c]) > t)) for c in range(6)]
+        assert list(nnz) == expect
+        print(f"approx={approx} value={value}: threshold {thr.value:.3e}, nnz {list(nnz)} of {n * n} each")
+        for S in (1, 3, 5, 64):
+            rng = np.random.default_rng(S)
+            m = rng.normal(size=(S, 3, n)).astype(np.float32)
+            ref = -_masked_product(K, t, m.astype(np.float64))
+            dm = torch.as_tensor(m).cuda()
+            out = torch.empty(S * 3 * n, dtype=torch.float32, device="cuda")
+            _lib.check(Lb.mt_demag_apply_sparse(h, dm.data_ptr(), out.data_ptr(), S, -1.0))
+            got = out.cpu().numpy().reshape(S, 3, n)
+            rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            assert rel < 1e-6, (S, rel)
+    finally:
+        Lb.mt_demag_sparse_destroy(h)
+
+
+def test_icl_sparse_plugin_abi(oracle):
+    from magtense_amd import _lib
+
+    Lb = _lib.lib()
+    res, L = [8, 8, 1], [16e-9, 16e-9, 2e-9]
+    n, K32 = _tensor(res, L)
+    K = K32.cpu().numpy().reshape(6, n, n)
+    thr = np.float32(1e-3)
+    keep = []
+    for c in range(6):
+        D = np.where(np.abs(K[c]) > thr, K[c], 0)
+        rows, cols = np.nonzero(D)
+        vals = np.ascontiguousarray(D[rows, cols], dtype=np.float32)
+        colInds = np.ascontiguousarray(cols + 1, dtype=np.int32)
+        rowInds = np.ascontiguousarray(np.concatenate([[1], 1 + np.cumsum(np.bincount(rows, minlength=n))]),
+                                       dtype=np.int32)
+        keep += [vals, colInds, rowInds]
+        nn, nnz, mat = C.c_int(n), C.c_int(vals.size), C.c_int(c + 1)
+        Lb.icl_initDemagMatrices_sparse(C.byref(nn), C.byref(nnz), C.byref(mat), vals.ctypes.data,
+                                        colInds.ctypes.data, rowInds.ctypes.data)
+    rng = np.random.default_rng(9)
+    M = rng.normal(size=(3, n)).astype(np.float32)
+    H = np.zeros((3, n), dtype=np.float32)
+    nn, pref = C.c_int(n), C.c_float(-1.0)
+    Lb.icl_MVMult_GetH_sparse(M[0].ctypes.data, M[1].ctypes.data, M[2].ctypes.data, H[0].ctypes.data,
+                              H[1].ctypes.data, H[2].ctypes.data, C.byref(nn), C.byref(pref))
+    Lb.icl_destroy()
+    ref = -_masked_product(K, thr, M[None].astype(np.float64))[0]
+    rel = np.linalg.norm(H - ref) / np.linalg.norm(ref)
+    assert rel < 1e-6, rel
+
+
+@pytest.mark.parametrize("mode,value", [("threshold", 1e-3), ("threshold_fraction", 0.8)])
+def test_solver_threshold_matches_oracle(oracle, mode, value):
+    from magtense_amd.micromag import MicromagProblem
+
+    res, L = [16, 4, 1], [250e-9, 62.5e-9, 3e-9]
+    rng = np.random.default_rng(21)
+    m0 = rng.normal(size=(64, 3))
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    hfield = np.array([-24.6, 4.3, 0.0]) / 1000 / MU0
+    p = MicromagProblem(res, L, m0=m0, alpha=4.42e3, gamma=2.21e5, tol=1e-6, demag_approx=mode, dem_thres=value)
+    t_out, M, *_ = p.run_simulation(1e-10, 11, lambda t: np.ones((len(t), 1)) * hfield, 20)
+    t = np.linspace(0, 1e-10, 11)
+    hext = np.zeros((20, 4))
+    hext[:, 0] = np.linspace(0, 1e-10, 20)
+    hext[:, 1:] = hfield
+    q = oracle.Problem(res, L, hext=hext, t=t, m0=np.concatenate([m0[:, 0], m0[:, 1], m0[:, 2]]), alpha=4.42e3,
+                       gamma=2.21e5, tol=1e-6, demag_approx=2 if mode == "threshold" else 4, dem_thres=value)
+    _, y, _, st = q.run()
+    yr = np.transpose(y[0], (0, 2, 1))
+    err = np.abs(M[:, :, 0, :] - yr).max()
+    print(mode, value, "threshold info", p.last_demag_info, "max traj err", err)
+    assert err < 1e-4
+    assert sum(p.last_demag_info["nnz"]) < 6 * 64 * 64
