@@ -47,7 +47,7 @@ def workload(B=64, seed=1234, res=(256, 64, 2), L=(512e-9, 128e-9, 3e-9)):
     return dict(res=res, L=L, n=n, B=B, m0=m0, fields=fields)
 
 
-def make_solver(W, precision, t_end=1e-9):
+def make_solver(W, precision, t_end=1e-9, toeplitz=0):
     from magtense_amd import _lib
 
     L = _lib.lib()
@@ -88,6 +88,7 @@ def make_solver(W, precision, t_end=1e-9):
     q.t_conv = dp(keep["tconv"])
     q.conv_tol = 1e-4
     q.precision = precision
+    q.toeplitz = toeplitz
     q.return_fields = 0
     h = C.c_void_p()
     _lib.check(L.mt_solver_create(C.byref(q), C.byref(h)))
@@ -250,6 +251,10 @@ def main():
     ap.add_argument("--no-field-error", action="store_true")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-assembly", action="store_true")
+    ap.add_argument("--dense-steps", type=int, default=10,
+                    help="steps of the dense fragment-tiled tensor path timed beside the table path (0: skip)")
+    ap.add_argument("--dense-tensor", action="store_true",
+                    help="time the dense fragment-tiled split tensor as the main path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}); "
@@ -270,7 +275,7 @@ def main():
     n, B = W["n"], W["B"]
     # ---- split-BF16 MFMA path (the measured value) ---------------------------------
     t0 = time.perf_counter()
-    L, h, keep = make_solver(W, _lib.PREC_BF16X2)
+    L, h, keep = make_solver(W, _lib.PREC_BF16X2, toeplitz=-1 if args.dense_tensor else 0)
     t_setup = time.perf_counter() - t0
     time_steps(L, h, args.warmup, B)
     st0 = stats(L, h, B)
@@ -293,6 +298,22 @@ def main():
         f"demag passes {passes}, rejected {rejected}, gemm {gemm_ms:.3f} ms/launch")
     alg_bytes = 6.0 * n * n * 4 + 2 * 3 * n * B * 4
     achieved = alg_bytes / (gemm_ms * 1e-3) / 1e9
+    # ---- the dense fragment-tiled tensor (25.8 GB streamed per pass) beside it ---------
+    dense = None
+    if args.dense_steps > 0 and not args.dense_tensor:
+        Ld, hd, keepd = make_solver(W, _lib.PREC_BF16X2, toeplitz=-1)
+        time_steps(Ld, hd, 1, B)
+        Ld.mt_solver_set_timing(hd, 1)
+        sa = stats(Ld, hd, B)
+        walld, bsd = time_steps(Ld, hd, args.dense_steps, B)
+        sb = stats(Ld, hd, B)
+        Ld.mt_solver_set_timing(hd, 0)
+        Ld.mt_solver_destroy(hd)
+        del keepd
+        dms = sb[0].demag_ms / max(sb[0].demag_passes - sa[0].demag_passes, 1)
+        dense = dict(steps_per_s=bsd / walld, gemm_ms=dms, gemm_gbps=alg_bytes / (dms * 1e-3) / 1e9,
+                     note="timed with per-pass HIP events (adds a sync per pass)")
+        log(f"dense tensor: {bsd / walld:.2f} steps/s, gemm {dms:.3f} ms = {alg_bytes / (dms * 1e-3) / 1e9:.0f} GB/s")
     # ---- plain FP32 per-state path ------------------------------------------------------
     plain = None
     if not args.no_plain:
@@ -335,6 +356,7 @@ def main():
         "setup_s": t_setup,
         "field_error_vs_fp64": ferr,
         "tensor_assembly": asm,
+        "dense_tensor_path": dense,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "gemm_bf16x2_lds_kernel", "kernel_ms": gemm_ms,

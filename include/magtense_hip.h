@@ -106,6 +106,16 @@ long long mt_packed_m_bytes(int n);
 /* FP32 receiver-major tensor (device) -> split BF16 fragment layout (device). */
 int mt_split_tensor(const float* K32, int n, void* split);
 
+/* Translation-invariant split-BF16 table of a uniform nx x ny x nz grid (nx % 32
+ * == 0; cell sizes and centres as setupGrid, LandauLifshitzEquationSolver.f90:
+ * 815-872): every tensor entry depends only on the cell offset, so the fragments
+ * of all (row tile, k-chunk) pairs with the same offset are stored once
+ * (mt_toeplitz_tensor_bytes bytes, device) instead of the dense N x N tensor.
+ * No reference counterpart: a storage form of the dense tensor of
+ * ComputeDemagfieldTensor (:883-986) for the batched product. */
+long long mt_toeplitz_tensor_bytes(int nx, int ny, int nz);
+int mt_toeplitz_tensor(int nx, int ny, int nz, double Lx, double Ly, double Lz, void* table);
+
 /* ---- (b) demag product -----------------------------------------------------------
  * h[s*3n + c*n + i] = pref * sum_j K_c(i,j) m[s*3n + c'*n + j] for each of
  * nstates states (SoA per state, device pointers).  pref = -1 reproduces
@@ -114,6 +124,9 @@ int mt_demag_apply_f32(const float* K32, int n, const float* m, float* h, int ns
 /* Same product from the split tensor: packs m (FP32, device) into BF16 hi/lo
  * fragments internally; nstates <= 64 per call (larger batches are chunked). */
 int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, int nstates, float pref);
+/* Same product from the translation-invariant table of mt_toeplitz_tensor. */
+int mt_demag_apply_bf16x2_toeplitz(const void* table, int nx, int ny, int nz, const float* m, float* h, int nstates,
+                                   float pref);
 
 /* Threshold-sparse tensor (ApplyThresholdDense, DemagAuxFunctions.f90:20-122;
  * ConvertDenseToSparse_s, util_micromag.f90:81-138): six CSR matrices built on
@@ -176,6 +189,8 @@ typedef struct {
   int demag_approx;      /* MT_DEMAG_DENSE (0 or 1), MT_DEMAG_THRESHOLD (2), MT_DEMAG_THRESHOLD_FRACTION (4);
                             the sparse modes need precision MT_PREC_FP32 */
   double demag_threshold; /* dem_thres: |K| threshold (mode 2) or fraction of the positive entries (mode 4) */
+  int toeplitz;          /* MT_PREC_BF16X2 tensor storage: 0 automatic (the translation-invariant fragment
+                            table when nx % 32 == 0 and no K_in), 1 require it, -1 dense fragments */
 } mt_micromag_problem;
 
 typedef struct {

@@ -46,6 +46,7 @@ class MtProblem(C.Structure):
         ("ksplit", C.c_int),
         ("demag_approx", C.c_int),
         ("demag_threshold", C.c_double),
+        ("toeplitz", C.c_int),
     ]
 
 
@@ -75,6 +76,7 @@ EXPORTS = [
     "mt_solver_set_timing", "mt_solver_destroy", "mt_run_micromag", "mt_rk4_create", "mt_rk4_forward",
     "mt_rk4_backward", "mt_rk4_destroy", "mt_tiles_demag_tensor", "mt_tiles_h_field", "mt_demag_sparse_create",
     "mt_demag_sparse_info", "mt_demag_apply_sparse", "mt_demag_sparse_destroy", "mt_solver_demag_info",
+    "mt_toeplitz_tensor_bytes", "mt_toeplitz_tensor", "mt_demag_apply_bf16x2_toeplitz",
     "icl_initDemagMatrices", "icl_MVMult_GetH", "icl_destroy", "icl_initDemagMatrices_sparse", "icl_MVMult_GetH_sparse",
 ]
 
@@ -129,6 +131,10 @@ def load():
     lib.mt_rk4_destroy.restype = None
     lib.mt_tiles_demag_tensor.argtypes = [C.POINTER(MtTiles), C.c_int, vp, vp]
     lib.mt_tiles_h_field.argtypes = [C.POINTER(MtTiles), C.c_int, vp, vp, vp]
+    lib.mt_toeplitz_tensor_bytes.argtypes = [C.c_int] * 3
+    lib.mt_toeplitz_tensor_bytes.restype = C.c_longlong
+    lib.mt_toeplitz_tensor.argtypes = [C.c_int] * 3 + [C.c_double] * 3 + [vp]
+    lib.mt_demag_apply_bf16x2_toeplitz.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_demag_sparse_create.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.POINTER(vp)]
     lib.mt_demag_sparse_info.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_float)]
     lib.mt_demag_apply_sparse.argtypes = [vp, vp, vp, C.c_int, C.c_float]

@@ -207,10 +207,25 @@ int mt_demag_apply_f32(const float* K32, int n, const float* m, float* h, int ns
   return kOk;
 }
 
-int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, int nstates, float pref) {
+long long mt_toeplitz_tensor_bytes(int nx, int ny, int nz) {
+  if (nx < 32 || nx % 32 != 0 || ny < 1 || nz < 1) return 0;
+  return toeplitz_tensor_bytes(nx, ny, nz);
+}
+
+int mt_toeplitz_tensor(int nx, int ny, int nz, double Lx, double Ly, double Lz, void* table) {
   int rc = device_ok();
   if (rc) return rc;
-  MT_CHECK(n > 0 && nstates > 0 && split && m && h, "invalid demag arguments");
+  MT_CHECK(nx >= 32 && nx % 32 == 0 && ny >= 1 && nz >= 1 && table, "the table needs nx % 32 == 0 (and nx >= 32)");
+  rc = launch_toeplitz_assemble(nx, ny, nz, Lx, Ly, Lz, (uint8_t*)table, nullptr);
+  if (rc) return rc;
+  MT_HIP(hipStreamSynchronize(nullptr));
+  return kOk;
+}
+
+namespace {
+// shared by the dense and the translation-invariant batched products
+int apply_bf16x2(const void* K, int n, int nx, int ny, int nz, const float* m, float* h, int nstates, float pref) {
+  int rc = kOk;
   const int64_t n3 = (int64_t)3 * n;
   int ks = gemm_ksplit_for(n, 4);
   uint8_t* packed = nullptr;
@@ -225,7 +240,11 @@ int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, in
     hipLaunchKernelGGL(pack_bf16_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr, m + s0 * n3, n,
                        S, packed);
     MT_HIP(hipGetLastError());
-    rc = launch_gemm_bf16x2((const uint8_t*)split, packed, n, part, n3, (int64_t)cols * n3, S, ks, nullptr);
+    if (nx > 0)
+      rc = launch_gemm_bf16x2_toeplitz((const uint8_t*)K, nx, ny, nz, packed, part, n3, (int64_t)cols * n3, S, ks,
+                                       nullptr);
+    else
+      rc = launch_gemm_bf16x2((const uint8_t*)K, packed, n, part, n3, (int64_t)cols * n3, S, ks, nullptr);
     if (rc) break;
     hipLaunchKernelGGL(combine_partials_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr, part, ks,
                        (int64_t)cols * n3, tot, pref, h + s0 * n3);
@@ -235,6 +254,23 @@ int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, in
   MT_HIP(hipFree(packed));
   MT_HIP(hipFree(part));
   return rc;
+}
+}  // namespace
+
+int mt_demag_apply_bf16x2_toeplitz(const void* table, int nx, int ny, int nz, const float* m, float* h, int nstates,
+                                   float pref) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(nx >= 32 && nx % 32 == 0 && ny >= 1 && nz >= 1 && nstates > 0 && table && m && h,
+           "invalid demag arguments (the table needs nx % 32 == 0)");
+  return apply_bf16x2(table, nx * ny * nz, nx, ny, nz, m, h, nstates, pref);
+}
+
+int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, int nstates, float pref) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n > 0 && nstates > 0 && split && m && h, "invalid demag arguments");
+  return apply_bf16x2(split, n, 0, 0, 0, m, h, nstates, pref);
 }
 
 int mt_ll_rhs(int nx, int ny, int nz, double Lx, double Ly, double Lz, const double* ms, const double* k0,

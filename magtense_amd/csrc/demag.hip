@@ -270,11 +270,15 @@ __device__ inline uint32_t lds_addr(const void* p) {
 // vmcnt[3:0] bits 3:0, vmcnt[5:4] bits 15:14, expcnt 7 and lgkmcnt 15 = no wait.
 constexpr int vmcnt_imm(int v) { return 0x0F70 | (v & 15) | ((v >> 4) << 14); }
 
-template <int STAGES>
+// TOEP: Kt is the translation-invariant fragment table of an nx x ny x nz
+// uniform grid (nx % 32 == 0) instead of the dense fragment-tiled tensor; the
+// fragment set of (row tile I, k-chunk J) is looked up from the cell offset.
+template <int STAGES, bool TOEP>
 __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __restrict__ Kt,
                                                               const uint4* __restrict__ Mp, int64_t np, int ksplit,
                                                               float* __restrict__ out, int n, int64_t col_stride,
-                                                              int64_t split_stride, int ncols) {
+                                                              int64_t split_stride, int ncols, int gnx, int gny,
+                                                              int gnz) {
   constexpr int FR = 36;             // fragments per stage: 2 tiles x 12 + 12 state fragments
   constexpr int LOADS = 9;           // per wave per stage: 6 tensor + 3 state fragments
   __shared__ uint4 smem[STAGES][FR][64];
@@ -303,10 +307,29 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
     ksrc_tile[q] = (It * nJ) * 12 + kf;
     kdst[q] = t * 12 + kf;
   }
+  // translation-invariant lookup: this wave's row tile (x0 tile, iy, iz)
+  const int cpr = gnx / 16, tpr = gnx / 32, U = gnx / 8 - 2;
+  int tx0 = 0, tiy = 0, tiz = 0;
+  if (TOEP) {
+    int64_t It = (int64_t)group * 2 + (wave >> 1);
+    if (It >= n_row_tiles) It = 0;
+    const int irow = (int)(It / tpr);
+    tx0 = (int)(It - (int64_t)irow * tpr);
+    tiy = irow % gny;
+    tiz = irow / gny;
+  }
   auto issue = [&](int J, int slot) {
+    int64_t tset = 0;
+    if (TOEP) {
+      const int jrow = J / cpr;
+      const int x1c = J - jrow * cpr;
+      const int jy = jrow % gny, jz = jrow / gny;
+      tset = ((int64_t)(tiz - jz + gnz - 1) * (2 * gny - 1) + (tiy - jy + gny - 1)) * U + (2 * tx0 - x1c + cpr - 1);
+    }
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      const uint4* src = Kt + (ksrc_tile[q] + (int64_t)J * 12) * 64 + lane;
+      const uint4* src = TOEP ? Kt + (tset * 12 + (wave * 6 + q) % 12) * 64 + lane
+                              : Kt + (ksrc_tile[q] + (int64_t)J * 12) * 64 + lane;
       lds_dma16(src, lds_addr(&smem[slot][kdst[q]][0]));
     }
 #pragma unroll
@@ -459,9 +482,9 @@ int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float
   if (gemm_variant() == 2) {
     int64_t np2 = round_up(n, kTileM);
     int groups2 = (int)((np2 / kTileM + 1) / 2);
-    hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4>), dim3((unsigned)(groups2 * ksplit)), dim3(256), 0, stream,
-                       reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np2, ksplit,
-                       out, n, col_stride, split_stride, ncols);
+    hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4, false>), dim3((unsigned)(groups2 * ksplit)), dim3(256), 0,
+                       stream, reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np2,
+                       ksplit, out, n, col_stride, split_stride, ncols, 0, 1, 1);
     MT_HIP(hipGetLastError());
     return kOk;
   }
@@ -477,6 +500,21 @@ int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float
     hipLaunchKernelGGL((gemm_bf16x2_kernel<WAVES, 2>), grid, dim3(WAVES * 64), 0, stream,
                        reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out,
                        n, col_stride, split_stride, ncols);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, const uint8_t* Mpack, float* out,
+                                int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+  if (nx % 32 != 0 || nx < 32 || ny < 1 || nz < 1) {
+    set_error("the translation-invariant GEMM needs nx % 32 == 0");
+    return kErrArg;
+  }
+  const int n = nx * ny * nz;
+  const int groups = (n / kTileM + 1) / 2;
+  hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4, true>), dim3((unsigned)(groups * ksplit)), dim3(256), 0, stream,
+                     reinterpret_cast<const uint4*>(Ktab), reinterpret_cast<const uint4*>(Mpack), (int64_t)n, ksplit,
+                     out, n, col_stride, split_stride, ncols, nx, ny, nz);
   MT_HIP(hipGetLastError());
   return kOk;
 }

@@ -59,6 +59,14 @@ inline int64_t split_tensor_bytes(int n) {
   return np * np * 6 * 4;
 }
 // Bytes of the packed magnetization operand (3 components, 64 columns, hi+lo).
+// Translation-invariant fragment table of a uniform grid (nx % 32 == 0): the
+// A fragment of row tile I and k-chunk J depends only on the cell offset
+// (x0 - x1, iy - jy, iz - jz); U = nx/8 - 2 x-offsets per (dy, dz) slab.
+inline int64_t toeplitz_fragments(int nx, int ny, int nz) {
+  return (int64_t)(2 * nz - 1) * (2 * ny - 1) * (nx / 8 - 2) * 12;
+}
+inline int64_t toeplitz_tensor_bytes(int nx, int ny, int nz) { return toeplitz_fragments(nx, ny, nz) * kFragBytes; }
+
 inline int64_t packed_m_bytes(int n) {
   int64_t np = round_up(n, kTileM);
   return np * 3 * kMaxCols * 4;

@@ -16,6 +16,10 @@ int launch_tiles_tensor(const double* tiles, int nt, const double* pts, int npt,
 int launch_tiles_field(const double* tiles, int nt, const double* pts, int npt, const double* Nin, double* H,
                        hipStream_t stream);
 int launch_split(const float* K, int n, uint8_t* split, hipStream_t stream);
+// Translation-invariant split-BF16 fragment table of a uniform grid (nx % 32 == 0),
+// toeplitz_tensor_bytes(nx, ny, nz) bytes.
+int launch_toeplitz_assemble(int nx, int ny, int nz, double Lx, double Ly, double Lz, uint8_t* out,
+                             hipStream_t stream);
 
 // ---- (b) demag product ------------------------------------------------------------
 // out[s*out_stride + c*n + i] = sum_j K_c.(i,j) m_s,.(j)   (positive K m)
@@ -28,6 +32,10 @@ int gemm_ksplit_for(int n, int waves);
 // out[ks*split_stride + col*col_stride + c*n + i] = k-split partial of K m
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
+// Same product with the tensor given as the translation-invariant fragment table of
+// an nx x ny x nz uniform grid (n = nx ny nz, nx % 32 == 0).
+int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, const uint8_t* Mpack, float* out,
+                                int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
 
 // Threshold-sparse demag tensor: six CSR matrices (xx xy xz yy yz zz), rows =
 // receivers, FP32 values, entries with |K| > threshold.

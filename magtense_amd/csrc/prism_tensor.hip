@@ -236,6 +236,59 @@ __global__ void __launch_bounds__(256) assemble_kernel(const double* __restrict_
   }
 }
 
+// Translation-invariant split-BF16 fragment table for a uniform grid with
+// nx % 32 == 0.  Fragment set (slab = (dz, dy), u) holds, for rows r < 32 and
+// k = e < 16, the tensor at cell offset (D + r - e, dy, dz) with D = 16 u -
+// (nx - 16): every (row tile, k-chunk) pair of the dense tensor whose
+// x0 - x1 = D and (iy - jy, iz - jz) = (dy, dz) has exactly these values.
+// One thread per (slab, u, lane): eight columns, six components, hi and lo.
+__global__ void __launch_bounds__(256) toeplitz_assemble_kernel(int nx, int ny, int nz, double Lx, double Ly,
+                                                                double Lz, uint8_t* __restrict__ out) {
+  // setupGrid (LandauLifshitzEquationSolver.f90:815-872): d = L/n, or d = L when n = 1
+  const double dx = nx > 1 ? Lx / nx : Lx, dy = ny > 1 ? Ly / ny : Ly, dz = nz > 1 ? Lz / nz : Lz;
+  const int U = nx / 8 - 2;
+  const int64_t sets = (int64_t)(2 * nz - 1) * (2 * ny - 1) * U;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= sets * 64) return;
+  const int lane = (int)(tid % 64);
+  const int64_t set = tid / 64;
+  const int u = (int)(set % U);
+  const int64_t slab = set / U;
+  const int dyi = (int)(slab % (2 * ny - 1)) - (ny - 1);
+  const int dzi = (int)(slab / (2 * ny - 1)) - (nz - 1);
+  const int D = 16 * u - (nx - 16);
+  const int r = lane % 32, e0 = 8 * (lane / 32);
+  // the offset is formed as the dense assembly forms p_i - c_j, from the cell
+  // centres of one representative pair (the first cells along each axis)
+  auto centre = [](int m, double L, double d, int i) { return m > 1 ? -L / 2 + i * d + d / 2 : 0.0; };
+  float v32[6][8];
+  for (int e = 0; e < 8; ++e) {
+    const int ddx = D + r - (e0 + e);
+    const int ix = ddx > 0 ? ddx : 0, jx = ddx > 0 ? 0 : -ddx;
+    const int iy = dyi > 0 ? dyi : 0, jy = dyi > 0 ? 0 : -dyi;
+    const int iz = dzi > 0 ? dzi : 0, jz = dzi > 0 ? 0 : -dzi;
+    double k6[6];
+    prism_k6(dx, dy, dz, centre(nx, Lx, dx, ix) - centre(nx, Lx, dx, jx), centre(ny, Ly, dy, iy) - centre(ny, Ly, dy, jy),
+             centre(nz, Lz, dz, iz) - centre(nz, Lz, dz, jz), k6);
+    for (int c = 0; c < 6; ++c) v32[c][e] = (float)k6[c];
+  }
+  for (int c = 0; c < 6; ++c) {
+    uint16_t hi[8], lo[8];
+    for (int e = 0; e < 8; ++e) {
+      hi[e] = bf16_bits(v32[c][e]);
+      lo[e] = bf16_bits(v32[c][e] - bf16_value(hi[e]));
+    }
+    uint4 h4, l4;
+    h4.x = hi[0] | ((uint32_t)hi[1] << 16); h4.y = hi[2] | ((uint32_t)hi[3] << 16);
+    h4.z = hi[4] | ((uint32_t)hi[5] << 16); h4.w = hi[6] | ((uint32_t)hi[7] << 16);
+    l4.x = lo[0] | ((uint32_t)lo[1] << 16); l4.y = lo[2] | ((uint32_t)lo[3] << 16);
+    l4.z = lo[4] | ((uint32_t)lo[5] << 16); l4.w = lo[6] | ((uint32_t)lo[7] << 16);
+    const int64_t frag = (set * 6 + c) * 2;
+    *reinterpret_cast<uint4*>(out + frag * kFragBytes + lane * 16) = h4;
+    *reinterpret_cast<uint4*>(out + (frag + 1) * kFragBytes + lane * 16) = l4;
+  }
+}
+
 // FP32 receiver-major tensor (6 x n x n) -> fragment-tiled split BF16 (padded).
 __global__ void __launch_bounds__(256) split_kernel(const float* __restrict__ K, int n, int64_t np,
                                                     uint8_t* __restrict__ split) {
@@ -394,6 +447,19 @@ int launch_assemble(const double* src, const double* abc, int abc_stride, const 
   int64_t blocks = (threads + 255) / 256;
   hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, abc, abc_stride, rot, pts,
                      n_src, n_pts, np, out32, out64, split);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_toeplitz_assemble(int nx, int ny, int nz, double Lx, double Ly, double Lz, uint8_t* out,
+                             hipStream_t stream) {
+  if (nx % 32 != 0) {
+    set_error("the translation-invariant tensor needs nx % 32 == 0");
+    return kErrArg;
+  }
+  const int64_t threads = toeplitz_fragments(nx, ny, nz) / 12 * 64;
+  hipLaunchKernelGGL(toeplitz_assemble_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, nx, ny,
+                     nz, Lx, Ly, Lz, out);
   MT_HIP(hipGetLastError());
   return kOk;
 }

@@ -283,6 +283,7 @@ struct Solver {
   hipStream_t stream = nullptr;
   float* K32 = nullptr;
   uint8_t* Ksplit = nullptr;
+  uint8_t* Ktab = nullptr;    // translation-invariant fragment table (bf16x2 on uniform grids)
   int demag_approx = 1;       // 1 dense, 2 threshold, 4 threshold fraction
   SparseDemag sparse;
   double *jfact = nullptr, *kfact = nullptr, *aniso = nullptr;
@@ -318,7 +319,7 @@ struct Solver {
     auto f = [](void* p) {
       if (p) (void)hipFree(p);
     };
-    f(K32); f(Ksplit); f(jfact); f(kfact); f(aniso); f(ms32); f(pool); f(yout); f(fout); f(packf); f(packb);
+    f(K32); f(Ksplit); f(Ktab); f(jfact); f(kfact); f(aniso); f(ms32); f(pool); f(yout); f(fout); f(packf); f(packb);
     f(hbuf); f(partials); f(results); f(arena_d);
     sparse.release();
     if (h_results) (void)hipHostFree(h_results);
@@ -326,7 +327,7 @@ struct Solver {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
-    K32 = nullptr; Ksplit = nullptr; jfact = kfact = aniso = nullptr; ms32 = nullptr; pool = yout = fout = nullptr;
+    K32 = nullptr; Ksplit = nullptr; Ktab = nullptr; jfact = kfact = aniso = nullptr; ms32 = nullptr; pool = yout = fout = nullptr;
     packf = nullptr; packb = nullptr; hbuf = nullptr; partials = results = nullptr; arena_d = nullptr;
     h_results = nullptr; arena_h = nullptr; stream = nullptr; ev0 = ev1 = nullptr;
   }
@@ -1007,7 +1008,11 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     if (rc) return rc;
     if (timing) MT_HIP(hipEventRecord(ev0, stream));
     if (precision == MT_PREC_BF16X2) {
-      rc = launch_gemm_bf16x2(Ksplit, packb, n, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit, stream);
+      if (Ktab)
+        rc = launch_gemm_bf16x2_toeplitz(Ktab, nx, ny, nz, packb, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit,
+                                         stream);
+      else
+        rc = launch_gemm_bf16x2(Ksplit, packb, n, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit, stream);
     } else if (demag_approx != MT_DEMAG_DENSE) {
       rc = sparse.apply(packf, (int64_t)N3, S, 1.0f, hbuf, stream);
     } else {
@@ -1185,7 +1190,16 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   // tensor
   const size_t nn = (size_t)n * n;
   const bool need32 = P.precision == MT_PREC_FP32 || q->keep_fp32_tensor;
-  const bool needsplit = P.precision == MT_PREC_BF16X2;
+  const bool can_toep = P.precision == MT_PREC_BF16X2 && !q->K_in && P.nx >= 32 && P.nx % 32 == 0;
+  MT_CHECK(q->toeplitz != 1 || can_toep,
+           "the translation-invariant tensor needs precision bf16x2, nx % 32 == 0 and no K_in");
+  const bool use_toep = can_toep && q->toeplitz >= 0;
+  const bool needsplit = P.precision == MT_PREC_BF16X2 && !use_toep;
+  if (use_toep) {
+    MT_HIP(hipMalloc(&P.Ktab, (size_t)toeplitz_tensor_bytes(P.nx, P.ny, P.nz)));
+    int rc = launch_toeplitz_assemble(P.nx, P.ny, P.nz, q->Lx, q->Ly, q->Lz, P.Ktab, P.stream);
+    if (rc) return rc;
+  }
   if (need32) MT_HIP(hipMalloc(&P.K32, 6 * nn * sizeof(float)));
   if (needsplit) MT_HIP(hipMalloc(&P.Ksplit, (size_t)split_tensor_bytes(n)));
   if (q->K_in) {
@@ -1200,7 +1214,7 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       MT_HIP(hipStreamSynchronize(P.stream));
       MT_HIP(hipFree(Kd));
     }
-  } else {
+  } else if (P.K32 || P.Ksplit) {
     double *dpts = nullptr, *dabc = nullptr;
     MT_HIP(hipMalloc(&dpts, pts.size() * sizeof(double)));
     MT_HIP(hipMalloc(&dabc, 3 * sizeof(double)));

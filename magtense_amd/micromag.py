@@ -275,6 +275,7 @@ class MicromagProblem:
         q.return_fields = int(bool(return_fields))
         q.max_steps = int(max_steps)
         q.demag_approx = int(self._dem_appr)
+        q.toeplitz = int(getattr(self, "_toeplitz", 0))
         q.demag_threshold = float(self.dem_thres)
         path = self._tensor_file()
         Kout = None
