@@ -287,8 +287,21 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
   const int tl = wave >> 1, sb = wave & 1;
   const int nJ = (int)(np / kTileK);
   const int n_row_tiles = (int)(np / kTileM);
-  const int ks = blockIdx.x % ksplit;
-  const int group = blockIdx.x / ksplit;
+  int ks = blockIdx.x % ksplit;
+  int group = blockIdx.x / ksplit;
+  if (TOEP) {
+    // XCD-aware order: workgroups are dispatched round-robin over the eight
+    // XCDs, so block b runs on XCD b % 8.  Give each XCD a contiguous range of
+    // (row-tile pair, k-split) work items: the ~32 workgroups resident on one
+    // XCD then cover a few neighbouring x-rows, whose table fragments (same
+    // cell offsets) and state fragments (same k-chunks) are shared in its L2.
+    const int total = ((n_row_tiles + 1) / 2) * ksplit;
+    const int per_xcd = (total + 7) / 8;
+    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (lin >= total) return;  // whole workgroup, before any barrier
+    ks = lin % ksplit;
+    group = lin / ksplit;
+  }
   const int jb = (int)((int64_t)nJ * ks / ksplit);
   const int je = (int)((int64_t)nJ * (ks + 1) / ksplit);
   const int64_t I = (int64_t)group * 2 + tl;
@@ -512,7 +525,8 @@ int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, con
   }
   const int n = nx * ny * nz;
   const int groups = (n / kTileM + 1) / 2;
-  hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4, true>), dim3((unsigned)(groups * ksplit)), dim3(256), 0, stream,
+  const int blocks = (groups * ksplit + 7) / 8 * 8;  // a multiple of the XCD count (see the kernel's mapping)
+  hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4, true>), dim3((unsigned)blocks), dim3(256), 0, stream,
                      reinterpret_cast<const uint4*>(Ktab), reinterpret_cast<const uint4*>(Mpack), (int64_t)n, ksplit,
                      out, n, col_stride, split_stride, ncols, nx, ny, nz);
   MT_HIP(hipGetLastError());
