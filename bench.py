@@ -8,10 +8,12 @@ evenly spaced from -100 to +100 mT along x (a field sweep).  RKSUITE 'M',
 tol 1e-6.  One benchmark step = one accepted RK45 step of all 64 states (7
 right-hand sides, each one pass over the demag tensor).
 
-The timed path is the split-BF16 MFMA demag product; the plain per-state FP32
-mat-vec path (the drop-in) is timed beside it on the same states, and the
-FP64 CPU oracle is timed on a bounded sample on the host cores.  Prints one
-JSON line.
+The timed path is the split-BF16 MFMA demag product over the dense 25.8 GB
+fragment-tiled tensor (HBM-bound).  Beside it, on the same states: the same
+product over the uniform grid's translation-invariant table (140 MB, L2/MALL
+resident), the plain per-state FP32 mat-vec path (the drop-in), the FP64
+field error, the 16384-prism tensor assembly, and the FP64 CPU oracle timed
+on a bounded sample on the host cores.  Prints one JSON line.
 """
 
 import argparse
@@ -28,6 +30,21 @@ sys.path.insert(0, ROOT)
 
 MU0 = 4e-7 * np.pi
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA, MI355X
+
+
+def traffic_from_profile(kind):
+    """HBM bytes per GEMM launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "gemm_bf16x2_n32768_b64_pmc.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_{kind}:"
+    with open(path) as fh:
+        for ln in fh:
+            if ln.strip().startswith(key):
+                return float(ln.split(":", 1)[1])
+    return None
 
 
 def log(msg):
@@ -251,10 +268,10 @@ def main():
     ap.add_argument("--no-field-error", action="store_true")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-assembly", action="store_true")
-    ap.add_argument("--dense-steps", type=int, default=10,
-                    help="steps of the dense fragment-tiled tensor path timed beside the table path (0: skip)")
-    ap.add_argument("--dense-tensor", action="store_true",
-                    help="time the dense fragment-tiled split tensor as the main path")
+    ap.add_argument("--table-steps", type=int, default=30,
+                    help="steps of the translation-invariant table path timed beside the dense tensor (0: skip)")
+    ap.add_argument("--table", action="store_true",
+                    help="time the translation-invariant table as the main path (default: the dense 25.8 GB tensor)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}); "
@@ -275,7 +292,7 @@ def main():
     n, B = W["n"], W["B"]
     # ---- split-BF16 MFMA path (the measured value) ---------------------------------
     t0 = time.perf_counter()
-    L, h, keep = make_solver(W, _lib.PREC_BF16X2, toeplitz=-1 if args.dense_tensor else 0)
+    L, h, keep = make_solver(W, _lib.PREC_BF16X2, toeplitz=0 if args.table else -1)
     t_setup = time.perf_counter() - t0
     time_steps(L, h, args.warmup, B)
     st0 = stats(L, h, B)
@@ -297,23 +314,29 @@ def main():
     log(f"bf16x2: setup {t_setup:.2f} s, {args.steps} steps in {wall:.3f} s = {steps_per_s:.2f} steps/s; "
         f"demag passes {passes}, rejected {rejected}, gemm {gemm_ms:.3f} ms/launch")
     alg_bytes = 6.0 * n * n * 4 + 2 * 3 * n * B * 4
-    achieved = alg_bytes / (gemm_ms * 1e-3) / 1e9
-    # ---- the dense fragment-tiled tensor (25.8 GB streamed per pass) beside it ---------
-    dense = None
-    if args.dense_steps > 0 and not args.dense_tensor:
-        Ld, hd, keepd = make_solver(W, _lib.PREC_BF16X2, toeplitz=-1)
-        time_steps(Ld, hd, 1, B)
-        Ld.mt_solver_set_timing(hd, 1)
-        sa = stats(Ld, hd, B)
-        walld, bsd = time_steps(Ld, hd, args.dense_steps, B)
-        sb = stats(Ld, hd, B)
-        Ld.mt_solver_set_timing(hd, 0)
-        Ld.mt_solver_destroy(hd)
-        del keepd
-        dms = sb[0].demag_ms / max(sb[0].demag_passes - sa[0].demag_passes, 1)
-        dense = dict(steps_per_s=bsd / walld, gemm_ms=dms, gemm_gbps=alg_bytes / (dms * 1e-3) / 1e9,
-                     note="timed with per-pass HIP events (adds a sync per pass)")
-        log(f"dense tensor: {bsd / walld:.2f} steps/s, gemm {dms:.3f} ms = {alg_bytes / (dms * 1e-3) / 1e9:.0f} GB/s")
+    alg_flops = 18.0 * n * n * B  # 9 block products x 2 flop per entry and state (split products not counted)
+    # ---- the translation-invariant table (140 MB instead of 25.8 GB) beside it -----------
+    table = None
+    if args.table_steps > 0 and not args.table:
+        Lt, ht, keept = make_solver(W, _lib.PREC_BF16X2, toeplitz=1)
+        time_steps(Lt, ht, 2, B)
+        wallt, bst = time_steps(Lt, ht, args.table_steps, B)
+        Lt.mt_solver_set_timing(ht, 1)
+        sa = stats(Lt, ht, B)
+        time_steps(Lt, ht, 2, B)
+        sb = stats(Lt, ht, B)
+        Lt.mt_solver_set_timing(ht, 0)
+        Lt.mt_solver_destroy(ht)
+        del keept
+        tms = sb[0].demag_ms / max(sb[0].demag_passes - sa[0].demag_passes, 1)
+        tf = alg_flops / (tms * 1e-3) / 1e12
+        table = dict(steps_per_s=bst / wallt, steps=args.table_steps, gemm_ms=tms, kernel="gemm_bf16x2_t4s_kernel<true>",
+                     roofline={"bound": "mfma", "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": tf / BF16_PEAK_TFLOPS, "mfma_issued_frac": 3.0 * tf / BF16_PEAK_TFLOPS,
+                               "traffic": traffic_from_profile("table")},
+                     table_bytes=float(L.mt_toeplitz_tensor_bytes(*W["res"])),
+                     note="same split-BF16 values; the uniform grid's tensor stored once per cell offset")
+        log(f"table path: {bst / wallt:.2f} steps/s, gemm {tms:.3f} ms = {tf:.0f} TFLOP/s algorithmic")
     # ---- plain FP32 per-state path ------------------------------------------------------
     plain = None
     if not args.no_plain:
@@ -333,6 +356,23 @@ def main():
     log(f"tensor assembly: {asm}")
     cpu = None if args.no_cpu else cpu_baseline(W)
     log(f"cpu baseline: {cpu}")
+    if not args.table:
+        # the dense fragment-tiled tensor streams 25.8 GB per pass: HBM-bound
+        achieved = alg_bytes / (gemm_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("dense"),
+                    "kernel": "gemm_bf16x2_lds_kernel<4,false>", "kernel_ms": gemm_ms,
+                    "alg_bytes_per_launch": alg_bytes, "measured_read_peak": rd.value,
+                    "frac_of_measured_read_peak": achieved / rd.value}
+    else:
+        # the translation-invariant table (140 MB) stays in L2/MALL: the matrix pipe bounds it
+        achieved = alg_flops / (gemm_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic_from_profile("table"),
+                    "kernel": "gemm_bf16x2_t4s_kernel<true>", "kernel_ms": gemm_ms,
+                    "alg_flops_per_launch": alg_flops,
+                    "mfma_issued_frac": 3.0 * achieved / BF16_PEAK_TFLOPS,
+                    "note": "split-BF16 issues 3 MFMA products per algorithmic flop (hi*hi + hi*lo + lo*hi)"}
     line = {
         "metric": "LL steps/s at N=32768, 64-state batch: split-BF16 MFMA vs FP32 plain, GPU",
         "value": steps_per_s,
@@ -347,7 +387,9 @@ def main():
         "dtype": "bf16x2 demag product (FP32 accumulate), FP64 state/RHS/stepper",
         "data": "synthetic: seeded random unit magnetization, 64-field sweep -100..+100 mT",
         "config": {"workload": "256x64x2 film (2x2x1.5 nm), N=32768, 64-state field sweep, RKSUITE 'M' tol 1e-6",
-                   "n_cells": n, "batch": B, "precision": "bf16x2", "tol": 1e-6},
+                   "n_cells": n, "batch": B, "precision": "bf16x2", "tol": 1e-6,
+                   "tensor": "translation-invariant table (140 MB)" if args.table
+                   else "dense split-BF16 hi+lo fragments (25.8 GB)"},
         "state_steps_per_s": steps_per_s * B,
         "rhs_per_s": passes / wall,
         "rejected_attempts": int(rejected),
@@ -356,12 +398,8 @@ def main():
         "setup_s": t_setup,
         "field_error_vs_fp64": ferr,
         "tensor_assembly": asm,
-        "dense_tensor_path": dense,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "gemm_bf16x2_lds_kernel", "kernel_ms": gemm_ms,
-                     "alg_bytes_per_launch": alg_bytes,
-                     "measured_read_peak": rd.value, "frac_of_measured_read_peak": achieved / rd.value},
+        "table_path": table,
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

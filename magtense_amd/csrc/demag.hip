@@ -290,17 +290,24 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
   int ks = blockIdx.x % ksplit;
   int group = blockIdx.x / ksplit;
   if (TOEP) {
-    // XCD-aware order: workgroups are dispatched round-robin over the eight
-    // XCDs, so block b runs on XCD b % 8.  Give each XCD a contiguous range of
-    // (row-tile pair, k-split) work items: the ~32 workgroups resident on one
-    // XCD then cover a few neighbouring x-rows, whose table fragments (same
-    // cell offsets) and state fragments (same k-chunks) are shared in its L2.
-    const int total = ((n_row_tiles + 1) / 2) * ksplit;
-    const int per_xcd = (total + 7) / 8;
-    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (lin >= total) return;  // whole workgroup, before any barrier
-    ks = lin % ksplit;
-    group = lin / ksplit;
+    // XCD-aware order (block b runs on XCD b % 8): with a k-split dividing 8,
+    // XCD x takes k-split x % ksplit for every row-tile pair, so its resident
+    // workgroups walk the same cell-offset sequence (below) and share table
+    // slabs in its L2; otherwise each XCD takes a contiguous range of work.
+    const int groups2 = (n_row_tiles + 1) / 2;
+    if (ksplit <= 8 && 8 % ksplit == 0) {
+      const int x = (int)(blockIdx.x % 8);
+      ks = x % ksplit;
+      group = (int)(blockIdx.x / 8) * (8 / ksplit) + x / ksplit;
+      if (group >= groups2) return;  // whole workgroup, before any barrier
+    } else {
+      const int total = groups2 * ksplit;
+      const int per_xcd = (total + 7) / 8;
+      const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+      if (lin >= total) return;
+      ks = lin % ksplit;
+      group = lin / ksplit;
+    }
   }
   const int jb = (int)((int64_t)nJ * ks / ksplit);
   const int je = (int)((int64_t)nJ * (ks + 1) / ksplit);
@@ -322,7 +329,7 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
   }
   // translation-invariant lookup: this wave's row tile (x0 tile, iy, iz)
   const int cpr = gnx / 16, tpr = gnx / 32, U = gnx / 8 - 2;
-  int tx0 = 0, tiy = 0, tiz = 0;
+  int tx0 = 0, tiy = 0, tiz = 0, wy = 0, wz = 0;
   if (TOEP) {
     int64_t It = (int64_t)group * 2 + (wave >> 1);
     if (It >= n_row_tiles) It = 0;
@@ -330,8 +337,23 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
     tx0 = (int)(It - (int64_t)irow * tpr);
     tiy = irow % gny;
     tiz = irow / gny;
+    const int wrow = (int)((int64_t)group * 2 / tpr);  // the pair's first tile sets the shared chunk order
+    wy = wrow % gny;
+    wz = wrow / gny;
   }
-  auto issue = [&](int J, int slot) {
+  // Chunk v of the k range -> k-chunk J.  With the table the k-chunks are
+  // visited by cell offset: v = ((s ny + r) cpr + x1c) takes source row
+  // jy = (wy - r) mod ny in plane jz = (wz - s) mod nz, so every workgroup with
+  // the same k-split reads the same (dy, dz) slab at the same step.  For each
+  // tile this is a permutation of the k-chunks.
+  auto issue = [&](int v, int slot) {
+    int J = v;
+    if (TOEP) {
+      const int x1c = v % cpr;
+      const int rs = v / cpr;
+      const int r = rs % gny, sp = rs / gny;
+      J = (((wz - sp + gnz) % gnz) * gny + (wy - r + gny) % gny) * cpr + x1c;
+    }
     int64_t tset = 0;
     if (TOEP) {
       const int jrow = J / cpr;
@@ -482,17 +504,20 @@ int gemm_ksplit_for(int n, int waves) {
   return ks;
 }
 
+// Kernel choice for measurements (MT_GEMM_VARIANT): 0 (default) picks the
+// fastest measured kernel for each storage; dense 1 = register-staged kernel;
+// table 2 = two-tile four-stage kernel, 3 = four-tile two-stage kernel.
 int gemm_variant() {
   static int v = [] {
     const char* e = getenv("MT_GEMM_VARIANT");
-    return e ? atoi(e) : 2;
+    return e ? atoi(e) : 0;
   }();
   return v;
 }
 
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
-  if (gemm_variant() == 2) {
+  if (gemm_variant() != 1) {
     int64_t np2 = round_up(n, kTileM);
     int groups2 = (int)((np2 / kTileM + 1) / 2);
     hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4, false>), dim3((unsigned)(groups2 * ksplit)), dim3(256), 0,
@@ -517,6 +542,319 @@ int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float
   return kOk;
 }
 
+// Four-tile variant for the translation-invariant table: a workgroup of four
+// waves owns four 32-row tiles, each wave one tile and all 64 states (six
+// accumulators).  Tensor and state fragments of a k-chunk (48 + 12 KiB) are
+// DMA'd into one of two LDS slots one chunk ahead; each wave reads the next
+// chunk's 24 fragments from LDS into a second register set while its 54
+// MFMAs of the current chunk execute, so LDS traffic overlaps the matrix
+// pipe.  (The table is L2-resident, so the kernel is bound by the MFMA and
+// LDS pipes rather than by HBM.)
+template <bool TOEP>
+__global__ void __launch_bounds__(256, 1) gemm_bf16x2_t4_kernel(const uint4* __restrict__ Kt,
+                                                                const uint4* __restrict__ Mp, int64_t np,
+                                                                int ksplit, float* __restrict__ out, int n,
+                                                                int64_t col_stride, int64_t split_stride, int ncols,
+                                                                int gnx, int gny, int gnz) {
+  constexpr int FR = 60;     // fragments per slot: 4 tiles x 12 + 12 state fragments
+  constexpr int LOADS = 15;  // per wave per chunk: 12 tensor + 3 state fragments
+  __shared__ uint4 smem[2][FR][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nJ = (int)(np / kTileK);
+  const int n_row_tiles = (int)(np / kTileM);
+  const int groups = (n_row_tiles + 3) / 4;
+  // XCD-aware order (block b runs on XCD b % 8).  With the table and a k-split
+  // dividing 8, XCD x takes k-split x % ksplit for every group: its resident
+  // workgroups then walk the same offset sequence (below) and share the table
+  // slabs in its L2.  Otherwise each XCD takes a contiguous range of work.
+  int ks, group;
+  const bool by_ks = TOEP && ksplit <= 8 && 8 % ksplit == 0;
+  if (by_ks) {
+    const int xpk = 8 / ksplit;  // XCDs per k-split
+    const int x = (int)(blockIdx.x % 8);
+    ks = x % ksplit;
+    group = (int)(blockIdx.x / 8) * xpk + x / ksplit;
+    if (group >= groups) return;  // whole workgroup, before any barrier
+  } else {
+    const int total = groups * ksplit;
+    const int per_xcd = (total + 7) / 8;
+    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (lin >= total) return;
+    ks = lin % ksplit;
+    group = lin / ksplit;
+  }
+  const int jb = (int)((int64_t)nJ * ks / ksplit);
+  const int je = (int)((int64_t)nJ * (ks + 1) / ksplit);
+  const int64_t I = (int64_t)group * 4 + wave;
+  const bool tile_ok = I < n_row_tiles;
+  const int64_t Il = tile_ok ? I : 0;  // loads stay uniform; never consumed
+  const int cpr = gnx / 16, tpr = gnx / 32, U = gnx / 8 - 2;
+  int tx0 = 0, tiy = 0, tiz = 0, wy = 0, wz = 0;
+  if (TOEP) {
+    const int irow = (int)(Il / tpr);
+    tx0 = (int)(Il - (int64_t)irow * tpr);
+    tiy = irow % gny;
+    tiz = irow / gny;
+    // the workgroup's first tile sets the chunk order shared by its four waves
+    const int64_t I0 = (int64_t)group * 4 < n_row_tiles ? (int64_t)group * 4 : 0;
+    const int wrow = (int)(I0 / tpr);
+    wy = wrow % gny;
+    wz = wrow / gny;
+  }
+  // Chunk v of the k range -> k-chunk J.  With the table the k-chunks are
+  // visited by cell offset: v = ((s ny + r) cpr + x1c) takes source row
+  // jy = (wy - r) mod ny in plane jz = (wz - s) mod nz, so every workgroup
+  // with the same k-split reads the same (dy, dz) slab at the same step.
+  // For each tile this is a permutation of the k-chunks.
+  auto chunk_of = [&](int v) {
+    if (!TOEP) return v;
+    const int x1c = v % cpr;
+    const int rs = v / cpr;
+    const int r = rs % gny, sp = rs / gny;
+    const int jy = (wy - r + gny) % gny, jz = (wz - sp + gnz) % gnz;
+    return (jz * gny + jy) * cpr + x1c;
+  };
+  auto issue = [&](int v, int slot) {
+    const int J = chunk_of(v);
+    int64_t base;
+    if (TOEP) {
+      const int jrow = J / cpr;
+      const int x1c = J - jrow * cpr;
+      const int jy = jrow % gny, jz = jrow / gny;
+      base = (((int64_t)(tiz - jz + gnz - 1) * (2 * gny - 1) + (tiy - jy + gny - 1)) * U + (2 * tx0 - x1c + cpr - 1)) * 12;
+    } else {
+      base = (Il * nJ + J) * 12;
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) lds_dma16(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int f = wave * 3 + q;  // 0..11
+      lds_dma16(Mp + ((int64_t)J * 12 + f) * 64 + lane, lds_addr(&smem[slot][48 + f][0]));
+    }
+  };
+  floatx16 acc[2][3];
+#pragma unroll
+  for (int sb = 0; sb < 2; ++sb)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[sb][a][r] = 0.0f;
+
+  // register sets: 12 tensor fragments of this wave's tile, 12 state fragments
+  uint4 ra[2][12], rb[2][12];
+  auto read_slot = [&](int slot, int set) {
+#pragma unroll
+    for (int q = 0; q < 12; ++q) ra[set][q] = smem[slot][wave * 12 + q][lane];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) rb[set][q] = smem[slot][48 + q][lane];
+  };
+  auto mfmas = [&](int set) {
+    const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+    for (int beta = 0; beta < 3; ++beta)
+#pragma unroll
+      for (int t = 0; t < 3; ++t)  // hi*hi, hi*lo, lo*hi
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int sb = 0; sb < 2; ++sb) {
+            const int c = comp[a][beta];
+            const bf16x8 kf = as_bf16x8(ra[set][2 * c + (t == 2 ? 1 : 0)]);
+            const bf16x8 mf = as_bf16x8(rb[set][(beta * 2 + sb) * 2 + (t == 1 ? 1 : 0)]);
+            acc[sb][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kf, mf, acc[sb][a], 0, 0, 0);
+          }
+  };
+
+  const int nchunks = je - jb;
+  if (nchunks > 0) {
+    issue(jb, 0);
+    issue(jb + (nchunks > 1 ? 1 : 0), 1);
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(LOADS));  // my chunk-0 loads
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    read_slot(0, 0);
+    for (int j = 0; j < nchunks; j += 2) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int jj = j + h;
+        if (jj < nchunks) {
+          // my loads of chunk jj+1 have landed and my reads of slot jj%2 are done
+          __builtin_amdgcn_s_waitcnt(0xC07F & 0xFFFF);  // lgkmcnt(0)
+          __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+          __builtin_amdgcn_s_barrier();
+          asm volatile("" ::: "memory");
+          const int jn = jj + 2 < nchunks ? jj + 2 : nchunks - 1;
+          issue(jb + jn, h);                            // slot jj % 2 == h
+          read_slot(h ^ 1, h ^ 1);                      // chunk jj+1 (or a repeat at the end)
+          if (tile_ok) mfmas(h);
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // no LDS write may outlive the workgroup
+  }
+  if (!tile_ok) return;
+  float* o = out + (int64_t)ks * split_stride;
+#pragma unroll
+  for (int sb = 0; sb < 2; ++sb) {
+    const int col = 32 * sb + (lane & 31);
+    if (col >= ncols) continue;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+        if (row < n) o[col * col_stride + (int64_t)a * n + row] = acc[sb][a][r];
+      }
+  }
+}
+
+template <bool TOEP>
+__global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __restrict__ Kt,
+                                                                const uint4* __restrict__ Mp, int64_t np,
+                                                                int ksplit, float* __restrict__ out, int n,
+                                                                int64_t col_stride, int64_t split_stride, int ncols,
+                                                                int gnx, int gny, int gnz) {
+  constexpr int FR = 60;     // fragments per slot: 4 tiles x 12 + 12 state fragments
+  constexpr int LOADS = 15;  // per wave per chunk: 12 tensor + 3 state fragments
+  __shared__ uint4 smem[1][FR][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nJ = (int)(np / kTileK);
+  const int n_row_tiles = (int)(np / kTileM);
+  const int groups = (n_row_tiles + 3) / 4;
+  // XCD-aware order (block b runs on XCD b % 8).  With the table and a k-split
+  // dividing 8, XCD x takes k-split x % ksplit for every group: its resident
+  // workgroups then walk the same offset sequence (below) and share the table
+  // slabs in its L2.  Otherwise each XCD takes a contiguous range of work.
+  int ks, group;
+  const bool by_ks = TOEP && ksplit <= 8 && 8 % ksplit == 0;
+  if (by_ks) {
+    const int xpk = 8 / ksplit;  // XCDs per k-split
+    const int x = (int)(blockIdx.x % 8);
+    ks = x % ksplit;
+    group = (int)(blockIdx.x / 8) * xpk + x / ksplit;
+    if (group >= groups) return;  // whole workgroup, before any barrier
+  } else {
+    const int total = groups * ksplit;
+    const int per_xcd = (total + 7) / 8;
+    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (lin >= total) return;
+    ks = lin % ksplit;
+    group = lin / ksplit;
+  }
+  const int jb = (int)((int64_t)nJ * ks / ksplit);
+  const int je = (int)((int64_t)nJ * (ks + 1) / ksplit);
+  const int64_t I = (int64_t)group * 4 + wave;
+  const bool tile_ok = I < n_row_tiles;
+  const int64_t Il = tile_ok ? I : 0;  // loads stay uniform; never consumed
+  const int cpr = gnx / 16, tpr = gnx / 32, U = gnx / 8 - 2;
+  int tx0 = 0, tiy = 0, tiz = 0, wy = 0, wz = 0;
+  if (TOEP) {
+    const int irow = (int)(Il / tpr);
+    tx0 = (int)(Il - (int64_t)irow * tpr);
+    tiy = irow % gny;
+    tiz = irow / gny;
+    // the workgroup's first tile sets the chunk order shared by its four waves
+    const int64_t I0 = (int64_t)group * 4 < n_row_tiles ? (int64_t)group * 4 : 0;
+    const int wrow = (int)(I0 / tpr);
+    wy = wrow % gny;
+    wz = wrow / gny;
+  }
+  // Chunk v of the k range -> k-chunk J.  With the table the k-chunks are
+  // visited by cell offset: v = ((s ny + r) cpr + x1c) takes source row
+  // jy = (wy - r) mod ny in plane jz = (wz - s) mod nz, so every workgroup
+  // with the same k-split reads the same (dy, dz) slab at the same step.
+  // For each tile this is a permutation of the k-chunks.
+  auto chunk_of = [&](int v) {
+    if (!TOEP) return v;
+    const int x1c = v % cpr;
+    const int rs = v / cpr;
+    const int r = rs % gny, sp = rs / gny;
+    const int jy = (wy - r + gny) % gny, jz = (wz - sp + gnz) % gnz;
+    return (jz * gny + jy) * cpr + x1c;
+  };
+  auto issue = [&](int v, int slot) {
+    const int J = chunk_of(v);
+    int64_t base;
+    if (TOEP) {
+      const int jrow = J / cpr;
+      const int x1c = J - jrow * cpr;
+      const int jy = jrow % gny, jz = jrow / gny;
+      base = (((int64_t)(tiz - jz + gnz - 1) * (2 * gny - 1) + (tiy - jy + gny - 1)) * U + (2 * tx0 - x1c + cpr - 1)) * 12;
+    } else {
+      base = (Il * nJ + J) * 12;
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) lds_dma16(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int f = wave * 3 + q;  // 0..11
+      lds_dma16(Mp + ((int64_t)J * 12 + f) * 64 + lane, lds_addr(&smem[slot][48 + f][0]));
+    }
+  };
+  floatx16 acc[2][3];
+#pragma unroll
+  for (int sb = 0; sb < 2; ++sb)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[sb][a][r] = 0.0f;
+
+  uint4 ra[12], rb[12];
+  auto mfmas = [&]() {
+    const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+    for (int beta = 0; beta < 3; ++beta)
+#pragma unroll
+      for (int t = 0; t < 3; ++t)  // hi*hi, hi*lo, lo*hi
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int sb = 0; sb < 2; ++sb) {
+            const int c = comp[a][beta];
+            const bf16x8 kf = as_bf16x8(ra[2 * c + (t == 2 ? 1 : 0)]);
+            const bf16x8 mf = as_bf16x8(rb[(beta * 2 + sb) * 2 + (t == 1 ? 1 : 0)]);
+            acc[sb][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kf, mf, acc[sb][a], 0, 0, 0);
+          }
+  };
+
+  const int nchunks = je - jb;
+  if (nchunks > 0) {
+    issue(jb, 0);
+    for (int j = 0; j < nchunks; ++j) {
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // my loads of chunk j
+      __builtin_amdgcn_s_barrier();              // everyone's
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int q = 0; q < 12; ++q) ra[q] = smem[0][wave * 12 + q][lane];
+#pragma unroll
+      for (int q = 0; q < 12; ++q) rb[q] = smem[0][48 + q][lane];
+      __builtin_amdgcn_s_waitcnt(0xC07F);        // lgkmcnt(0): my reads done
+      __builtin_amdgcn_s_barrier();              // everyone's: the slot is free
+      asm volatile("" ::: "memory");
+      issue(jb + (j + 1 < nchunks ? j + 1 : j), 0);
+      if (tile_ok) mfmas();
+    }
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // no LDS write may outlive the workgroup
+  }
+  if (!tile_ok) return;
+  float* o = out + (int64_t)ks * split_stride;
+#pragma unroll
+  for (int sb = 0; sb < 2; ++sb) {
+    const int col = 32 * sb + (lane & 31);
+    if (col >= ncols) continue;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+        if (row < n) o[col * col_stride + (int64_t)a * n + row] = acc[sb][a][r];
+      }
+  }
+}
+
 int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, const uint8_t* Mpack, float* out,
                                 int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
   if (nx % 32 != 0 || nx < 32 || ny < 1 || nz < 1) {
@@ -524,6 +862,28 @@ int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, con
     return kErrArg;
   }
   const int n = nx * ny * nz;
+  if (gemm_variant() != 2 && gemm_variant() != 3) {
+    // default: single-stage four-tile kernel, two workgroups per CU (measured at
+    // N = 32768, 64 states: 3.19 ms, against 3.31 ms for the two-stage four-tile
+    // kernel and 3.50 ms for the two-tile four-stage kernel)
+    const int groups4 = (n / kTileM + 3) / 4;
+    const int blocks4 = (groups4 * ksplit + 7) / 8 * 8;
+    hipLaunchKernelGGL((gemm_bf16x2_t4s_kernel<true>), dim3((unsigned)blocks4), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4*>(Ktab), reinterpret_cast<const uint4*>(Mpack), (int64_t)n,
+                       ksplit, out, n, col_stride, split_stride, ncols, nx, ny, nz);
+    MT_HIP(hipGetLastError());
+    return kOk;
+  }
+  if (gemm_variant() == 3) {
+    // four-tile kernel; the k-split only changes how many partial sums the LL kernel adds
+    const int groups4 = (n / kTileM + 3) / 4;
+    const int blocks4 = (groups4 * ksplit + 7) / 8 * 8;
+    hipLaunchKernelGGL((gemm_bf16x2_t4_kernel<true>), dim3((unsigned)blocks4), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4*>(Ktab), reinterpret_cast<const uint4*>(Mpack), (int64_t)n,
+                       ksplit, out, n, col_stride, split_stride, ncols, nx, ny, nz);
+    MT_HIP(hipGetLastError());
+    return kOk;
+  }
   const int groups = (n / kTileM + 1) / 2;
   const int blocks = (groups * ksplit + 7) / 8 * 8;  // a multiple of the XCD count (see the kernel's mapping)
   hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4, true>), dim3((unsigned)blocks), dim3(256), 0, stream,
