@@ -124,6 +124,8 @@ def time_steps(L, h, steps, B):
     """Advance until B*steps accepted state-steps were taken (a full batch the whole time)."""
     from magtense_amd import _lib
 
+    if steps <= 0:
+        return 0.0, 0.0
     s0 = L.mt_solver_steps_total(h)
     t0 = time.perf_counter()
     _lib.check(L.mt_solver_advance_total(h, steps * B))
