@@ -227,7 +227,7 @@ namespace {
 int apply_bf16x2(const void* K, int n, int nx, int ny, int nz, const float* m, float* h, int nstates, float pref) {
   int rc = kOk;
   const int64_t n3 = (int64_t)3 * n;
-  int ks = gemm_ksplit_for(n, 4);
+  int ks = gemm_ksplit_for(n, 2);
   uint8_t* packed = nullptr;
   float* part = nullptr;
   MT_HIP(hipMalloc(&packed, (size_t)packed_m_bytes(n)));

@@ -6,14 +6,18 @@
 // FP32, each field component is -(K_a1 m_1 + K_a2 m_2 + K_a3 m_3) in FP32, and
 // the off-diagonal blocks serve both of their symmetric positions.
 //
-// Two device paths, both reading each tensor block exactly once per pass:
+// Device paths:
 //   * matvec_f32  - the plain per-state FP32 drop-in: one pass over the FP32
 //                   receiver-major blocks per state, six row streams per cell.
 //   * gemm_bf16x2 - the batched path: the tensor stored as BF16 hi+lo MFMA
 //                   A-fragments, up to 64 states as the N dimension, three
 //                   split products (hi*hi + hi*lo + lo*hi) per block on
-//                   v_mfma_f32_32x32x16_bf16 with FP32 accumulation.
-// Both write partial sums S = K m (positive); the LL kernel applies the sign,
+//                   v_mfma_f32_32x32x16_bf16 with FP32 accumulation; the
+//                   fragments come either from the dense fragment-tiled tensor
+//                   (streamed once per pass) or, on a uniform grid, from the
+//                   translation-invariant table (looked up by cell offset).
+//   * matvec_f64  - the FP64 product of the adjoint's gradient-check mode.
+// All write partial sums S = K m (positive); the LL kernel applies the sign,
 // sums k-split partials and scales by the receiver cell's Ms.
 
 #include <hip/hip_bf16.h>
@@ -492,12 +496,13 @@ int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, f
   return kOk;
 }
 
-int gemm_ksplit_for(int n, int waves) {
+int gemm_ksplit_for(int n, int tiles_per_group) {
   int64_t np = round_up(n, kTileM);
   int nJ = (int)(np / kTileK);
-  int groups = (int)((np / kTileM + waves - 1) / waves);
-  // enough workgroups to fill 256 CUs several times (measured on N = 32768:
-  // k-split 4 streams 6.0 TB/s, 8 5.9 TB/s, 16 5.5 TB/s)
+  int groups = (int)((np / kTileM + tiles_per_group - 1) / tiles_per_group);
+  // enough workgroups to fill 256 CUs several times (measured on N = 32768 with
+  // two-tile groups, 2048 workgroups at k-split 4: k-split 4 streams 6.0 TB/s,
+  // 8 5.9 TB/s, 16 5.5 TB/s); fewer k-splits also mean fewer partial sums
   int ks = 4;
   while (groups * ks < 2048 && ks * 2 <= nJ) ks *= 2;
   if (ks > nJ) ks = nJ > 0 ? nJ : 1;

@@ -28,7 +28,7 @@ int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, f
 // FP64 form of the same product (FP64 tensor planes and states)
 int launch_matvec_f64(const double* K, int n, const double* m, int64_t m_stride, double* out, int64_t out_stride,
                       int nstates, hipStream_t stream);
-int gemm_ksplit_for(int n, int waves);
+int gemm_ksplit_for(int n, int tiles_per_group);  // k-split of the batched GEMM (two-tile groups)
 // out[ks*split_stride + col*col_stride + c*n + i] = k-split partial of K m
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
