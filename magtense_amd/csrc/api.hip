@@ -78,10 +78,24 @@ __global__ void scale_kernel(float* __restrict__ h, int64_t total, float pref) {
 }
 
 // streaming kernels used to measure the achievable HBM bandwidth on the device
+// Streaming read: four independent 16-byte non-temporal loads per thread per
+// iteration (the fastest form measured on the MI355X: ~6.5-6.9 TB/s on 16 GiB
+// with 16384 blocks, against ~6.1-6.3 TB/s for one plain load per iteration).
 __global__ void read_bw_kernel(const uint4* __restrict__ p, int64_t n16, uint32_t* __restrict__ sink) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4* q = reinterpret_cast<const u32x4*>(p);
   uint32_t acc = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x) {
-    uint4 v = p[i];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    u32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(q + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+  }
+  for (; i < n16; i += stride) {
+    u32x4 v = __builtin_nontemporal_load(q + i);
     acc ^= v.x ^ v.y ^ v.z ^ v.w;
   }
   if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the loads; practically never taken
@@ -147,12 +161,12 @@ int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* 
   hipEvent_t e0, e1;
   MT_HIP(hipEventCreate(&e0));
   MT_HIP(hipEventCreate(&e1));
-  const dim3 grid(2048), block(256);
+  const dim3 grid(2048), rgrid(16384), block(256);
   float ms = 0;
   if (read_gbps) {
-    hipLaunchKernelGGL(read_bw_kernel, grid, block, 0, nullptr, a, n16, sink);
+    hipLaunchKernelGGL(read_bw_kernel, rgrid, block, 0, nullptr, a, n16, sink);
     MT_HIP(hipEventRecord(e0, nullptr));
-    for (int k = 0; k < iters; ++k) hipLaunchKernelGGL(read_bw_kernel, grid, block, 0, nullptr, a, n16, sink);
+    for (int k = 0; k < iters; ++k) hipLaunchKernelGGL(read_bw_kernel, rgrid, block, 0, nullptr, a, n16, sink);
     MT_HIP(hipEventRecord(e1, nullptr));
     MT_HIP(hipEventSynchronize(e1));
     MT_HIP(hipEventElapsedTime(&ms, e0, e1));

@@ -266,6 +266,22 @@ __device__ inline void lds_dma16(const void* src, uint32_t lds) {
       : "v"(src), "s"(__builtin_amdgcn_readfirstlane(lds))
       : "memory");
 }
+// Same transfer with the non-temporal hint: for data read once per pass (the
+// dense tensor), so that it does not displace reused lines in L2/MALL.
+__device__ inline void lds_dma16_nt(const void* src, uint32_t lds) {
+  uint32_t saved;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %2\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dwordx4 %1, off nt\n\t"
+      "s_mov_b32 m0, %0\n\t"
+      "s_nop 0"
+      : "=&s"(saved)
+      : "v"(src), "s"(__builtin_amdgcn_readfirstlane(lds))
+      : "memory");
+}
+
 __device__ inline uint32_t lds_addr(const void* p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
@@ -367,9 +383,10 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      const uint4* src = TOEP ? Kt + (tset * 12 + (wave * 6 + q) % 12) * 64 + lane
-                              : Kt + (ksrc_tile[q] + (int64_t)J * 12) * 64 + lane;
-      lds_dma16(src, lds_addr(&smem[slot][kdst[q]][0]));
+      if (TOEP)
+        lds_dma16(Kt + (tset * 12 + (wave * 6 + q) % 12) * 64 + lane, lds_addr(&smem[slot][kdst[q]][0]));
+      else  // the dense tensor is read once per pass
+        lds_dma16_nt(Kt + (ksrc_tile[q] + (int64_t)J * 12) * 64 + lane, lds_addr(&smem[slot][kdst[q]][0]));
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -520,8 +537,25 @@ int gemm_variant() {
   return v;
 }
 
+template <bool TOEP>
+__global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __restrict__ Kt,
+                                                                 const uint4* __restrict__ Mp, int64_t np,
+                                       int ksplit, float* __restrict__ out, int n, int64_t col_stride,
+                                       int64_t split_stride, int ncols, int gnx, int gny, int gnz);
+
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+  if (gemm_variant() == 5) {
+    // measurement option: the single-stage four-tile kernel on the dense tensor
+    int64_t np4 = round_up(n, kTileM);
+    const int groups4 = (int)((np4 / kTileM + 3) / 4);
+    const int blocks4 = (groups4 * ksplit + 7) / 8 * 8;
+    hipLaunchKernelGGL((gemm_bf16x2_t4s_kernel<false>), dim3((unsigned)blocks4), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np4, ksplit,
+                       out, n, col_stride, split_stride, ncols, 0, 1, 1);
+    MT_HIP(hipGetLastError());
+    return kOk;
+  }
   if (gemm_variant() != 1) {
     int64_t np2 = round_up(n, kTileM);
     int groups2 = (int)((np2 / kTileM + 1) / 2);
@@ -632,7 +666,12 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_t4_kernel(const uint4* __r
       base = (Il * nJ + J) * 12;
     }
 #pragma unroll
-    for (int q = 0; q < 12; ++q) lds_dma16(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
+    for (int q = 0; q < 12; ++q) {
+      if (TOEP)
+        lds_dma16(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
+      else  // the dense tensor is read once per pass
+        lds_dma16_nt(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
+    }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const int f = wave * 3 + q;  // 0..11
@@ -792,7 +831,12 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __
       base = (Il * nJ + J) * 12;
     }
 #pragma unroll
-    for (int q = 0; q < 12; ++q) lds_dma16(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
+    for (int q = 0; q < 12; ++q) {
+      if (TOEP)
+        lds_dma16(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
+      else  // the dense tensor is read once per pass
+        lds_dma16_nt(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
+    }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const int f = wave * 3 + q;  // 0..11
