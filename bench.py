@@ -11,9 +11,11 @@ right-hand sides, each one pass over the demag tensor).
 The timed path is the split-BF16 MFMA demag product over the dense 25.8 GB
 fragment-tiled tensor (HBM-bound).  Beside it, on the same states: the same
 product over the uniform grid's translation-invariant table (140 MB, L2/MALL
-resident), the plain per-state FP32 mat-vec path (the drop-in), the FP64
-field error, the 16384-prism tensor assembly, and the FP64 CPU oracle timed
-on a bounded sample on the host cores.  Prints one JSON line.
+resident), the plain per-state FP32 mat-vec path (the drop-in), the product
+at one row shard's shape (N/8 receivers x N sources, the per-GPU work of the
+row-sharded design), the FP64 field error, the 16384-prism tensor assembly,
+and the FP64 CPU oracle timed on a bounded sample on the host cores.  Prints
+one JSON line.
 """
 
 import argparse
@@ -133,7 +135,7 @@ def time_steps(L, h, steps, B):
     return wall, (L.mt_solver_steps_total(h) - s0) / B
 
 
-def field_error_vs_fp64(W, n_rows=1024, n_states=8, seed=7):
+def field_error_vs_fp64(W, n_rows=1024, n_states=64, seed=7):
     """Relative demag-field error of the split-BF16 and FP32 products vs an FP64
     product on sampled receiver rows (FP64 tensor rows from the GPU kernel)."""
     import torch
@@ -210,6 +212,49 @@ def cpu_baseline(W, rows=2048):
                         f"not counted)"))
 
 
+def shard_shape(W, shards=8, shard=3, reps=10):
+    """The batched split-BF16 product at one row shard's shape: N/shards receiver
+    rows x all N sources x B states, from the shard's own block of fragment rows
+    (assembled for its receivers only, as a GPU holding one shard would)."""
+    import torch
+    from magtense_amd import _lib
+
+    L = _lib.lib()
+    n, B = W["n"], W["B"]
+    rows = n // shards
+    row0 = shard * rows
+    res, Lg = W["res"], W["L"]
+    d = [Lg[k] / res[k] for k in range(3)]
+    ax = [np.array([-Lg[k] / 2 + i * d[k] + d[k] / 2 for i in range(res[k])]) for k in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    np32 = (n + 31) // 32 * 32
+    blk = torch.empty((rows + 31) // 32 * 32 * np32 * 24, dtype=torch.uint8, device="cuda")
+    dp = torch.as_tensor(pts).cuda()
+    dsub = torch.as_tensor(np.ascontiguousarray(pts[row0:row0 + rows])).cuda()
+    dabc = torch.as_tensor(np.array(d)).cuda()
+    t0 = time.perf_counter()
+    _lib.check(L.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, None, dsub.data_ptr(), n, rows, None, None,
+                                 blk.data_ptr()))
+    t_asm = time.perf_counter() - t0
+    dm = torch.as_tensor(W["m0"].astype(np.float32)).cuda()
+    hd = torch.empty(B * 3 * rows, dtype=torch.float32, device="cuda")
+    args = (blk.data_ptr(), n, row0, rows, dm.data_ptr(), hd.data_ptr(), B, -1.0)
+    _lib.check(L.mt_demag_apply_bf16x2_rows(*args))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _lib.check(L.mt_demag_apply_bf16x2_rows(*args))  # synchronous
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    alg = 6.0 * rows * n * 4 + 3.0 * n * B * 4 + 3.0 * rows * B * 4
+    del blk
+    return dict(shape=f"{rows} receiver rows (shard {shard} of {shards}) x {n} sources x {B} states",
+                block_bytes=float((rows + 31) // 32 * 32 * np32 * 24), assembly_ms=1e3 * t_asm, call_ms=ms,
+                achieved_gbs=alg / (ms * 1e-3) / 1e9, alg_bytes=alg,
+                note="wall time per synchronous mt_demag_apply_bf16x2_rows call: state packing, the GEMM, the "
+                     "partial-sum combine and the call's workspace allocation; the RCCL all-gather of the states "
+                     "is not built")
+
+
 def lattice_prisms(seed=4, side=128):
     """16384 prisms: side x side lattice of pitch 5 nm, centres jittered by
     U[-0.5, 0.5] nm in x and y, edges U[0.8, 1.2] * (5, 5, 2) nm."""
@@ -262,7 +307,7 @@ def tensor_assembly(reps=3, cpu_rows=64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
@@ -270,6 +315,8 @@ def main():
     ap.add_argument("--no-field-error", action="store_true")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-assembly", action="store_true")
+    ap.add_argument("--no-shard", action="store_true",
+                    help="skip the product at one row shard's shape (N/8 rows x N sources)")
     ap.add_argument("--table-steps", type=int, default=30,
                     help="steps of the translation-invariant table path timed beside the dense tensor (0: skip)")
     ap.add_argument("--table", action="store_true",
@@ -349,6 +396,8 @@ def main():
         del keep2
         plain = bs2 / wall2
         log(f"fp32 plain: {args.plain_steps} steps in {wall2:.3f} s = {plain:.4f} steps/s")
+    shard = None if args.no_shard else shard_shape(W)
+    log(f"row shard shape: {shard}")
     rd, cp = C.c_double(0), C.c_double(0)
     _lib.check(L.mt_measure_bandwidth(8 << 30, 10, C.byref(rd), C.byref(cp)))
     log(f"measured streaming bandwidth: read {rd.value:.0f} GB/s, copy {cp.value:.0f} GB/s")
@@ -363,7 +412,7 @@ def main():
         achieved = alg_bytes / (gemm_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("dense"),
-                    "kernel": "gemm_bf16x2_lds_kernel<4,false>", "kernel_ms": gemm_ms,
+                    "kernel": "gemm_bf16x2_reg_kernel<4>", "kernel_ms": gemm_ms,
                     "alg_bytes_per_launch": alg_bytes, "measured_read_peak": rd.value,
                     "frac_of_measured_read_peak": achieved / rd.value}
     else:
@@ -401,6 +450,7 @@ def main():
         "field_error_vs_fp64": ferr,
         "tensor_assembly": asm,
         "table_path": table,
+        "row_shard": shard,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -38,6 +38,14 @@ extern "C" {
 #define MT_PREC_BF16X2 1 /* split BF16 hi+lo tensor, batched MFMA GEMM, FP32 accumulation */
 #define MT_PREC_FP64 2   /* FP64 tensor, product and Ms scaling (mt_rk4 only: gradient checks) */
 
+/* status codes (every int-returning entry point; MT_ERR_UNSUPPORTED below) */
+#define MT_OK 0
+#define MT_ERR_ARG (-1)    /* invalid argument, rejected before any launch */
+#define MT_ERR_HIP (-2)    /* HIP runtime failure */
+#define MT_ERR_SOLVER (-3) /* RKSUITE fatal condition */
+#define MT_ERR_ALLOC (-4)  /* device allocation failure */
+#define MT_ERR_STATE (-5)  /* entry points called in the wrong order */
+
 /* demag approximations (dem_appr of MicromagProblem, micromag.py:61-68) */
 #define MT_DEMAG_DENSE 1
 #define MT_DEMAG_THRESHOLD 2
@@ -64,8 +72,11 @@ int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* 
  * are device pointers, row-major.  Outputs (any may be null):
  *   out32[c*n_pts*n_src + i*n_src + j]  FP32, c = xx,xy,xz,yy,yz,zz
  *   out64                               same layout, FP64
- *   split                               fragment-tiled BF16 hi/lo (square only,
- *                                       mt_split_tensor_bytes(n) bytes)
+ *   split                               fragment-tiled BF16 hi/lo,
+ *                                       round_up(n_pts,32) * round_up(n_src,32) * 24
+ *                                       bytes (mt_split_tensor_bytes(n) when square;
+ *                                       with pts = receivers row0.. of the sources,
+ *                                       the row block of mt_demag_apply_bf16x2_rows)
  * Replaces getN_prism_3D over all pairs (TileNComponents.f90:629-724 and
  * LandauLifshitzEquationSolver.f90:935-984 / 1044-1133). */
 int mt_prism_tensor(const double* src, const double* abc, int abc_stride, const double* rot, const double* pts,
@@ -124,6 +135,18 @@ int mt_demag_apply_f32(const float* K32, int n, const float* m, float* h, int ns
 /* Same product from the split tensor: packs m (FP32, device) into BF16 hi/lo
  * fragments internally; nstates <= 64 per call (larger batches are chunked). */
 int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, int nstates, float pref);
+/* Row block of the same product, the per-GPU product of a tensor sharded by
+ * receiver rows: h[s*3*nrows + c*nrows + r] = pref * sum_j K_c(row0 + r, j) m[...j]
+ * for r < nrows, all n sources.  split_rows points at the fragment rows of
+ * receivers row0 ... row0 + nrows - 1 (row0 % 32 == 0, row0 + nrows <= n), which
+ * are contiguous in the split tensor from byte mt_split_tensor_row_offset(n, row0)
+ * on (-1 for an invalid row0) and span round_up(nrows, 32) * round_up(n, 32) * 24
+ * bytes; a GPU holding only its shard passes its own copy of those bytes.
+ * No reference counterpart: the row split of updateDemagfield's products
+ * (LandauLifshitzEquationSolver.f90:706-754). */
+long long mt_split_tensor_row_offset(int n, int row0);
+int mt_demag_apply_bf16x2_rows(const void* split_rows, int n, int row0, int nrows, const float* m, float* h,
+                               int nstates, float pref);
 /* Same product from the translation-invariant table of mt_toeplitz_tensor. */
 int mt_demag_apply_bf16x2_toeplitz(const void* table, int nx, int ny, int nz, const float* m, float* h, int nstates,
                                    float pref);

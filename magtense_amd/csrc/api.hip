@@ -237,16 +237,19 @@ int mt_toeplitz_tensor(int nx, int ny, int nz, double Lx, double Ly, double Lz, 
 }
 
 namespace {
-// shared by the dense and the translation-invariant batched products
-int apply_bf16x2(const void* K, int n, int nx, int ny, int nz, const float* m, float* h, int nstates, float pref) {
+// shared by the dense, row-block and translation-invariant batched products;
+// nrows < n only for a row block of the dense tensor (h holds 3 nrows per state)
+int apply_bf16x2(const void* K, int n, int nrows, int nx, int ny, int nz, const float* m, float* h, int nstates,
+                 float pref) {
   int rc = kOk;
   const int64_t n3 = (int64_t)3 * n;
-  int ks = gemm_ksplit_for(n, 2);
+  const int64_t r3 = (int64_t)3 * nrows;
+  int ks = nx > 0 ? gemm_ksplit_for(n, 2) : gemm_ksplit_dense(n, nrows);
   uint8_t* packed = nullptr;
   float* part = nullptr;
   MT_HIP(hipMalloc(&packed, (size_t)packed_m_bytes(n)));
   int cols = nstates < kMaxCols ? nstates : kMaxCols;
-  MT_HIP(hipMalloc(&part, (size_t)ks * cols * n3 * sizeof(float)));
+  MT_HIP(hipMalloc(&part, (size_t)ks * cols * r3 * sizeof(float)));
   for (int s0 = 0; s0 < nstates && rc == kOk; s0 += kMaxCols) {
     int S = nstates - s0 < kMaxCols ? nstates - s0 : kMaxCols;
     MT_HIP(hipMemsetAsync(packed, 0, (size_t)packed_m_bytes(n), nullptr));
@@ -257,11 +260,15 @@ int apply_bf16x2(const void* K, int n, int nx, int ny, int nz, const float* m, f
     if (nx > 0)
       rc = launch_gemm_bf16x2_toeplitz((const uint8_t*)K, nx, ny, nz, packed, part, n3, (int64_t)cols * n3, S, ks,
                                        nullptr);
+    else if (nrows < n)
+      rc = launch_gemm_bf16x2_rows((const uint8_t*)K, packed, n, nrows, part, r3, (int64_t)cols * r3, S, ks,
+                                   nullptr);
     else
       rc = launch_gemm_bf16x2((const uint8_t*)K, packed, n, part, n3, (int64_t)cols * n3, S, ks, nullptr);
     if (rc) break;
-    hipLaunchKernelGGL(combine_partials_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr, part, ks,
-                       (int64_t)cols * n3, tot, pref, h + s0 * n3);
+    const int64_t otot = (int64_t)S * r3;
+    hipLaunchKernelGGL(combine_partials_kernel, dim3((unsigned)((otot + 255) / 256)), dim3(256), 0, nullptr, part,
+                       ks, (int64_t)cols * r3, otot, pref, h + s0 * r3);
     MT_HIP(hipGetLastError());
   }
   MT_HIP(hipStreamSynchronize(nullptr));
@@ -277,14 +284,29 @@ int mt_demag_apply_bf16x2_toeplitz(const void* table, int nx, int ny, int nz, co
   if (rc) return rc;
   MT_CHECK(nx >= 32 && nx % 32 == 0 && ny >= 1 && nz >= 1 && nstates > 0 && table && m && h,
            "invalid demag arguments (the table needs nx % 32 == 0)");
-  return apply_bf16x2(table, nx * ny * nz, nx, ny, nz, m, h, nstates, pref);
+  return apply_bf16x2(table, nx * ny * nz, nx * ny * nz, nx, ny, nz, m, h, nstates, pref);
 }
 
 int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, int nstates, float pref) {
   int rc = device_ok();
   if (rc) return rc;
   MT_CHECK(n > 0 && nstates > 0 && split && m && h, "invalid demag arguments");
-  return apply_bf16x2(split, n, 0, 0, 0, m, h, nstates, pref);
+  return apply_bf16x2(split, n, n, 0, 0, 0, m, h, nstates, pref);
+}
+
+long long mt_split_tensor_row_offset(int n, int row0) {
+  if (n <= 0 || row0 < 0 || row0 >= n || row0 % kTileM != 0) return -1;
+  return (long long)row0 * round_up(n, kTileM) * 24;
+}
+
+int mt_demag_apply_bf16x2_rows(const void* split_rows, int n, int row0, int nrows, const float* m, float* h,
+                               int nstates, float pref) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n > 0 && nrows > 0 && row0 >= 0 && row0 % kTileM == 0 && nrows <= n - row0,
+           "the row block must start on a multiple of 32 and end at or before n");
+  MT_CHECK(nstates > 0 && split_rows && m && h, "invalid demag arguments");
+  return apply_bf16x2(split_rows, n, nrows, 0, 0, 0, m, h, nstates, pref);
 }
 
 int mt_ll_rhs(int nx, int ny, int nz, double Lx, double Ly, double Lz, const double* ms, const double* k0,

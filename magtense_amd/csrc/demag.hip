@@ -305,8 +305,8 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tl = wave >> 1, sb = wave & 1;
-  const int nJ = (int)(np / kTileK);
-  const int n_row_tiles = (int)(np / kTileM);
+  const int nJ = (int)(np / kTileK);                      // np: padded k extent (source cells)
+  const int n_row_tiles = (int)((n + kTileM - 1) / kTileM);  // n: receiver rows (np for a square tensor)
   int ks = blockIdx.x % ksplit;
   int group = blockIdx.x / ksplit;
   if (TOEP) {
@@ -452,6 +452,155 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
     }
 }
 
+// Register-ring kernel for the dense tensor.  Each wave owns one 32-row tile
+// and all 64 states (six accumulators); its A fragments (12 per k-chunk,
+// 12 KiB) stream from HBM straight into a D-deep register ring with
+// non-temporal loads, so D-1 chunks per wave, four waves per CU, are in
+// flight without the LDS capacity bounding the lookahead.  The state
+// fragments of a chunk (12 KiB, L2/MALL resident) ride in the same ring
+// (3 x 16 B per thread) and are shared through a two-slot LDS buffer.  All
+// loads are ordinary vector loads issued in chunk order, so the compiler's
+// in-order vmcnt wait for chunk j lets chunks j+1 .. j+D-1 stay in flight.
+// Row tiles come from n (receiver rows), k-chunks from np (padded sources).
+template <int D>
+__global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __restrict__ Kt,
+                                                                 const uint4* __restrict__ Mp, int64_t np,
+                                                                 int ksplit, float* __restrict__ out, int n,
+                                                                 int64_t col_stride, int64_t split_stride,
+                                                                 int ncols) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  __shared__ u32x4 bs[2][12][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nJ = (int)(np / kTileK);
+  const int n_row_tiles = (int)((n + kTileM - 1) / kTileM);
+  const int groups = (n_row_tiles + 3) / 4;
+  // XCD-aware order (block b runs on XCD b % 8): the workgroups resident on
+  // one XCD share k-splits, hence the state slab they read through its L2.
+  int ks, group;
+  {
+    const int x = (int)(blockIdx.x % 8), q = (int)(blockIdx.x / 8);
+    if (ksplit % 8 == 0) {
+      const int per = ksplit / 8;
+      ks = x + 8 * (q % per);
+      group = q / per;
+    } else if (8 % ksplit == 0) {
+      ks = x % ksplit;
+      group = q * (8 / ksplit) + x / ksplit;
+    } else {
+      ks = (int)(blockIdx.x % ksplit);
+      group = (int)(blockIdx.x / ksplit);
+    }
+  }
+  if (group >= groups) return;  // whole workgroup, before any barrier
+  const int jb = (int)((int64_t)nJ * ks / ksplit);
+  const int nchunks = (int)((int64_t)nJ * (ks + 1) / ksplit) - jb;
+  const int64_t I = (int64_t)group * 4 + wave;
+  const bool tile_ok = I < n_row_tiles;
+  // a wave past the last tile loads the group's first tile (kept uniform, never consumed)
+  const u32x4* A = reinterpret_cast<const u32x4*>(Kt) + (tile_ok ? I : (int64_t)group * 4) * nJ * 12 * 64 + lane;
+  const u32x4* B = reinterpret_cast<const u32x4*>(Mp) + threadIdx.x;
+  const bool two_halves = ncols > 32;
+
+  // native vector element types keep both rings in registers
+  u32x4 ar[D][12];
+  u32x4 br[D][3];
+  floatx16 acc[2][3];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[h][a][r] = 0.0f;
+
+#define MT_REG_ISSUE(v, slot)                                                      \
+  do {                                                                              \
+    const int J_ = jb + ((v) < nchunks ? (v) : nchunks - 1);                        \
+    const u32x4* a_ = A + (int64_t)J_ * 12 * 64;                                    \
+    _Pragma("unroll") for (int f = 0; f < 12; ++f) ar[slot][f] = __builtin_nontemporal_load(a_ + f * 64); \
+    const u32x4* b_ = B + (int64_t)J_ * 12 * 64;                                    \
+    _Pragma("unroll") for (int q = 0; q < 3; ++q) br[slot][q] = b_[q * 256];        \
+  } while (0)
+
+  // chunk j -> LDS slot j & 1, barrier, MFMAs from register slot `slot`
+#define MT_REG_CONSUME(j, slot)                                                                         \
+  do {                                                                                                   \
+    u32x4* dst_ = &bs[(j) & 1][0][0] + threadIdx.x;                                                      \
+    _Pragma("unroll") for (int q = 0; q < 3; ++q) dst_[q * 256] = br[slot][q];                          \
+    __syncthreads();                                                                                     \
+    if (tile_ok) {                                                                                       \
+      bf16x8 bh[3][2], bl[3][2];                                                                         \
+      _Pragma("unroll") for (int beta = 0; beta < 3; ++beta)                                             \
+      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                                    \
+        bh[beta][h] = __builtin_bit_cast(bf16x8, bs[(j) & 1][(beta * 2 + h) * 2 + 0][lane]);             \
+        bl[beta][h] = __builtin_bit_cast(bf16x8, bs[(j) & 1][(beta * 2 + h) * 2 + 1][lane]);             \
+      }                                                                                                  \
+      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                                    \
+        if (h == 1 && !two_halves) break;                                                                \
+        _Pragma("unroll") for (int a = 0; a < 3; ++a)                                                    \
+        _Pragma("unroll") for (int beta = 0; beta < 3; ++beta) {                                         \
+          const int c = (a == 0 ? beta : (a == 1 ? (beta == 0 ? 1 : 2 + beta) : (beta == 0 ? 2 : 3 + beta))); \
+          const bf16x8 khi = __builtin_bit_cast(bf16x8, ar[slot][2 * c]);                                \
+          const bf16x8 klo = __builtin_bit_cast(bf16x8, ar[slot][2 * c + 1]);                            \
+          acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(khi, bh[beta][h], acc[h][a], 0, 0, 0);     \
+          acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(khi, bl[beta][h], acc[h][a], 0, 0, 0);     \
+          acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(klo, bh[beta][h], acc[h][a], 0, 0, 0);     \
+        }                                                                                                \
+      }                                                                                                  \
+    }                                                                                                    \
+  } while (0)
+
+  if (nchunks > 0) {
+    // chunk j0 + s always sits in register slot s: no branch inside the main
+    // loop, so every slot keeps one register assignment around the loop
+#pragma unroll
+    for (int s = 0; s < D - 1; ++s) MT_REG_ISSUE(s, s);
+    int j0 = 0;
+    for (; j0 + D <= nchunks; j0 += D) {
+#pragma unroll
+      for (int s = 0; s < D; ++s) {
+        MT_REG_ISSUE(j0 + s + D - 1, (s + D - 1) % D);
+        MT_REG_CONSUME(j0 + s, s);
+      }
+    }
+    // fewer than D chunks left, all already issued into slots 0 .. rem - 1
+    const int rem = nchunks - j0;
+#pragma unroll
+    for (int s = 0; s < D - 1; ++s)
+      if (s < rem) MT_REG_CONSUME(j0 + s, s);
+  }
+#undef MT_REG_CONSUME
+#undef MT_REG_ISSUE
+  if (!tile_ok) return;
+  float* o = out + (int64_t)ks * split_stride;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int col = 32 * h + (lane & 31);
+    if (col >= ncols) break;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+        if (row < n) o[col * col_stride + (int64_t)a * n + row] = acc[h][a][r];
+      }
+  }
+}
+
+template <int D>
+int launch_reg(const uint8_t* K, const uint8_t* Mpack, int n, int nrows, float* out, int64_t col_stride,
+               int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+  const int64_t np = round_up(n, kTileM);
+  const int groups = (int)((round_up(nrows, kTileM) / kTileM + 3) / 4);
+  int64_t blocks = (int64_t)groups * ksplit;
+  if (ksplit % 8 != 0 && 8 % ksplit == 0) blocks = round_up(blocks, 8);
+  hipLaunchKernelGGL((gemm_bf16x2_reg_kernel<D>), dim3((unsigned)blocks), dim3(256), 0, stream,
+                     reinterpret_cast<const uint4*>(K), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out, nrows,
+                     col_stride, split_stride, ncols);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
 }  // namespace
 
 // FP64 product for the gradient-check mode of the RK4 adjoint: one wavefront
@@ -513,22 +662,12 @@ int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, f
   return kOk;
 }
 
-int gemm_ksplit_for(int n, int tiles_per_group) {
-  int64_t np = round_up(n, kTileM);
-  int nJ = (int)(np / kTileK);
-  int groups = (int)((np / kTileM + tiles_per_group - 1) / tiles_per_group);
-  // enough workgroups to fill 256 CUs several times (measured on N = 32768 with
-  // two-tile groups, 2048 workgroups at k-split 4: k-split 4 streams 6.0 TB/s,
-  // 8 5.9 TB/s, 16 5.5 TB/s); fewer k-splits also mean fewer partial sums
-  int ks = 4;
-  while (groups * ks < 2048 && ks * 2 <= nJ) ks *= 2;
-  if (ks > nJ) ks = nJ > 0 ? nJ : 1;
-  return ks;
-}
-
 // Kernel choice for measurements (MT_GEMM_VARIANT): 0 (default) picks the
-// fastest measured kernel for each storage; dense 1 = register-staged kernel;
-// table 2 = two-tile four-stage kernel, 3 = four-tile two-stage kernel.
+// fastest measured kernel for each storage.  Dense tensor: 0 = register-ring
+// kernel with 4 chunks in flight, 7 / 8 = the same with 5 / 3 chunks, 6 =
+// two-tile kernel with a four-stage LDS DMA ring, 1 = register-staged
+// kernel, 5 = four-tile single-stage kernel.  Table: 2 = two-tile four-stage
+// kernel, 3 = four-tile two-stage kernel.
 int gemm_variant() {
   static int v = [] {
     const char* e = getenv("MT_GEMM_VARIANT");
@@ -537,11 +676,59 @@ int gemm_variant() {
   return v;
 }
 
+namespace {
+int forced_ksplit() {
+  static const int forced = [] {
+    const char* e = getenv("MT_GEMM_KSPLIT");  // measurement override
+    return e ? atoi(e) : 0;
+  }();
+  return forced;
+}
+int clamp_ksplit(int ks, int nJ) {
+  if (forced_ksplit() > 0) ks = forced_ksplit();
+  if (ks > nJ) ks = nJ > 0 ? nJ : 1;
+  return ks;
+}
+}  // namespace
+
+int gemm_ksplit_rows(int n, int nrows, int tiles_per_group);
+
+int gemm_ksplit_for(int n, int tiles_per_group) { return gemm_ksplit_rows(n, n, tiles_per_group); }
+
+int gemm_ksplit_rows(int n, int nrows, int tiles_per_group) {
+  int64_t np = round_up(n, kTileM);
+  int nJ = (int)(np / kTileK);
+  int groups = (int)((round_up(nrows, kTileM) / kTileM + tiles_per_group - 1) / tiles_per_group);
+  // enough workgroups to fill 256 CUs several times (measured on N = 32768 with
+  // two-tile groups, 2048 workgroups at k-split 4: k-split 4 streams 6.0 TB/s,
+  // 8 5.9 TB/s, 16 5.5 TB/s); fewer k-splits also mean fewer partial sums
+  int ks = 4;
+  while (groups * ks < 2048 && ks * 2 <= nJ) ks *= 2;
+  return clamp_ksplit(ks, nJ);
+}
+
+int gemm_ksplit_dense(int n, int nrows) {
+  const int v = gemm_variant();
+  if (v == 1 || v == 5 || v == 6) return gemm_ksplit_rows(n, nrows, 2);
+  // register-ring kernel: one resident workgroup (four row tiles) per CU; the
+  // smallest k-split that gives every CU a workgroup (measured: N = 32768
+  // k-split 4 / 8 / 16 -> 4.19 / 4.33 / 4.52 ms; a 4096-row block 8 -> 0.69 ms
+  // per call, 4 (half the CUs idle) -> 0.99 ms)
+  const int nJ = (int)(round_up(n, kTileM) / kTileK);
+  const int groups = (int)((round_up(nrows, kTileM) / kTileM + 3) / 4);
+  int ks = 4;
+  while (groups * ks < 256 && ks * 2 <= nJ) ks *= 2;
+  return clamp_ksplit(ks, nJ);
+}
+
 template <bool TOEP>
 __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __restrict__ Kt,
                                                                  const uint4* __restrict__ Mp, int64_t np,
                                        int ksplit, float* __restrict__ out, int n, int64_t col_stride,
                                        int64_t split_stride, int ncols, int gnx, int gny, int gnz);
+
+int launch_gemm_bf16x2_rows(const uint8_t* Krows, const uint8_t* Mpack, int n, int nrows, float* out,
+                            int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
 
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
@@ -556,15 +743,8 @@ int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float
     MT_HIP(hipGetLastError());
     return kOk;
   }
-  if (gemm_variant() != 1) {
-    int64_t np2 = round_up(n, kTileM);
-    int groups2 = (int)((np2 / kTileM + 1) / 2);
-    hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4, false>), dim3((unsigned)(groups2 * ksplit)), dim3(256), 0,
-                       stream, reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np2,
-                       ksplit, out, n, col_stride, split_stride, ncols, 0, 1, 1);
-    MT_HIP(hipGetLastError());
-    return kOk;
-  }
+  if (gemm_variant() != 1)
+    return launch_gemm_bf16x2_rows(Ksplit, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream);
   constexpr int WAVES = 4;
   int64_t np = round_up(n, kTileM);
   int groups = (int)((np / kTileM + WAVES - 1) / WAVES);
@@ -577,6 +757,24 @@ int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float
     hipLaunchKernelGGL((gemm_bf16x2_kernel<WAVES, 2>), grid, dim3(WAVES * 64), 0, stream,
                        reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out,
                        n, col_stride, split_stride, ncols);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_gemm_bf16x2_rows(const uint8_t* Krows, const uint8_t* Mpack, int n, int nrows, float* out,
+                            int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+  switch (gemm_variant()) {
+    case 6: break;
+    case 7: return launch_reg<5>(Krows, Mpack, n, nrows, out, col_stride, split_stride, ncols, ksplit, stream);
+    case 8: return launch_reg<3>(Krows, Mpack, n, nrows, out, col_stride, split_stride, ncols, ksplit, stream);
+    default: return launch_reg<4>(Krows, Mpack, n, nrows, out, col_stride, split_stride, ncols, ksplit, stream);
+  }
+  // variant 6: the two-tile kernel with the four-stage LDS DMA ring
+  const int64_t np = round_up(n, kTileM);
+  const int groups2 = (int)((round_up(nrows, kTileM) / kTileM + 1) / 2);
+  hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4, false>), dim3((unsigned)(groups2 * ksplit)), dim3(256), 0, stream,
+                     reinterpret_cast<const uint4*>(Krows), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out,
+                     nrows, col_stride, split_stride, ncols, 0, 1, 1);
   MT_HIP(hipGetLastError());
   return kOk;
 }

@@ -29,9 +29,16 @@ int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, f
 int launch_matvec_f64(const double* K, int n, const double* m, int64_t m_stride, double* out, int64_t out_stride,
                       int nstates, hipStream_t stream);
 int gemm_ksplit_for(int n, int tiles_per_group);  // k-split of the batched GEMM (two-tile groups)
+int gemm_ksplit_rows(int n, int nrows, int tiles_per_group);  // same for nrows receiver rows of n sources
+int gemm_ksplit_dense(int n, int nrows);  // k-split of the dense-tensor kernel launch_gemm_bf16x2{,_rows} picks
 // out[ks*split_stride + col*col_stride + c*n + i] = k-split partial of K m
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
+// Row block: Krows holds the fragment rows of nrows receivers (row tiles
+// row0/32 ... of an n-source split tensor, row0 % 32 == 0); output rows 0..nrows-1
+// (out[ks*split_stride + col*col_stride + c*nrows + r]).
+int launch_gemm_bf16x2_rows(const uint8_t* Krows, const uint8_t* Mpack, int n, int nrows, float* out,
+                            int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
 // Same product with the tensor given as the translation-invariant fragment table of
 // an nx x ny x nz uniform grid (n = nx ny nz, nx % 32 == 0).
 int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, const uint8_t* Mpack, float* out,

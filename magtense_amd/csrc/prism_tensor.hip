@@ -174,12 +174,13 @@ __global__ void __launch_bounds__(256) assemble_kernel(const double* __restrict_
                                                        const double* __restrict__ pts, int n_src,
                                                        int n_pts, int64_t np, float* __restrict__ out32,
                                                        double* __restrict__ out64, uint8_t* __restrict__ split) {
-  // Thread -> (row i, 8 columns starting at j0); grid covers the padded square for the split layout.
+  // Thread -> (row i, 8 columns starting at j0); for the split layout the grid
+  // covers the padded rows x padded sources (np = round_up(n_src, 32)).
   int64_t groups_per_row = (split ? np : round_up(n_src, 8)) / 8;
   int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i = tid / groups_per_row;
   int64_t j0 = (tid % groups_per_row) * 8;
-  int64_t rows = split ? np : n_pts;
+  int64_t rows = split ? round_up(n_pts, kTileM) : n_pts;
   if (i >= rows) return;
   float v32[6][8];
   bool row_ok = i < n_pts;
@@ -436,13 +437,9 @@ int launch_tiles_field(const double* tiles, int nt, const double* pts, int npt, 
 
 int launch_assemble(const double* src, const double* abc, int abc_stride, const double* rot, const double* pts,
                     int n_src, int n_pts, float* out32, double* out64, uint8_t* split, hipStream_t stream) {
-  if (split && n_src != n_pts) {
-    set_error("split tensor layout needs a square tensor (n_src == n_pts)");
-    return kErrArg;
-  }
   int64_t np = round_up(n_src, kTileM);
   int64_t groups_per_row = (split ? np : round_up(n_src, 8)) / 8;
-  int64_t rows = split ? np : n_pts;
+  int64_t rows = split ? round_up(n_pts, kTileM) : n_pts;
   int64_t threads = rows * groups_per_row;
   int64_t blocks = (threads + 255) / 256;
   hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, abc, abc_stride, rot, pts,

@@ -43,3 +43,34 @@ def test_no_gpu_fails_loudly():
     rc = lib.mt_demag_apply_f32(None, 4, None, None, 1, ctypes.c_float(-1.0))
     assert rc != 0
     assert b"GPU" in lib.mt_last_error()
+
+
+def test_split_row_offsets():
+    """Byte offset of a row block of the fragment-tiled split tensor: 24 bytes
+    (6 components x hi/lo x 2 B) per padded source per receiver row; row blocks
+    start on 32-row tiles (host arithmetic only)."""
+    from magtense_amd import _lib
+
+    lib = _lib.load()
+    lib.mt_split_tensor_row_offset.restype = ctypes.c_longlong
+    lib.mt_split_tensor_bytes.restype = ctypes.c_longlong
+    n = 32768
+    assert lib.mt_split_tensor_row_offset(n, 0) == 0
+    assert lib.mt_split_tensor_row_offset(n, 4096) * 8 == lib.mt_split_tensor_bytes(n)
+    assert lib.mt_split_tensor_row_offset(540, 512) == 512 * 544 * 24
+    assert lib.mt_split_tensor_bytes(540) == 544 * 544 * 24
+    for bad in ((540, 33), (540, 544), (540, -32), (0, 0)):
+        assert lib.mt_split_tensor_row_offset(*bad) == -1
+
+
+def test_status_codes_match_library():
+    """The header's MT_ERR_* values equal the library's internal status enum."""
+    hdr = open(os.path.join(ROOT, "include", "magtense_hip.h")).read()
+    src = open(os.path.join(ROOT, "magtense_amd", "csrc", "mt_common.h")).read()
+    declared = {k: int(v) for k, v in re.findall(r"#define MT_(OK|ERR_\w+) \(?(-?\d+)\)?", hdr)}
+    enum = {k: int(v) for k, v in re.findall(r"\bk(Ok|Err\w+) = (-?\d+)", src)}
+    names = {"Ok": "OK", "ErrArg": "ERR_ARG", "ErrHip": "ERR_HIP", "ErrSolver": "ERR_SOLVER",
+             "ErrAlloc": "ERR_ALLOC", "ErrState": "ERR_STATE", "ErrUnsupported": "ERR_UNSUPPORTED"}
+    assert set(enum) == set(names)
+    for k, v in enum.items():
+        assert declared[names[k]] == v, k
