@@ -214,6 +214,20 @@ typedef struct {
   double demag_threshold; /* dem_thres: |K| threshold (mode 2) or fraction of the positive entries (mode 4) */
   int toeplitz;          /* MT_PREC_BF16X2 tensor storage: 0 automatic (the translation-invariant fragment
                             table when nx % 32 == 0 and no K_in), 1 require it, -1 dense fragments */
+  /* grid of unstructured prisms (grid_type 3, gridTypeUnstructuredPrisms; loadMicroMagProblem,
+     MagTenseMicroMagPyIO.f90:76-85,163-180): n = nx*ny*nz cells at grid_pts with full edges
+     grid_abc; the tensor is averaged over n_ave sub-points of each receiver cell
+     (LandauLifshitzEquationSolver.f90:1044-1133) and the exchange operator is the loaded CSR
+     (ConvertExchangeTerm3D_NonUniform, :1530-1544).  grid_type 0 or 1: the uniform grid. */
+  int grid_type;
+  const double* grid_pts;      /* n*3 row-major */
+  const double* grid_abc;      /* n*3 row-major */
+  int n_ave[3];
+  int exch_nnz;
+  const int* exch_rows_start;  /* n, 1-based (mkl_sparse_d_create_csr rows_start) */
+  const int* exch_rows_end;    /* n, 1-based */
+  const int* exch_cols;        /* exch_nnz, 1-based */
+  const double* exch_vals;     /* exch_nnz (double; the reference's Python path passes int32, Q7) */
 } mt_micromag_problem;
 
 typedef struct {

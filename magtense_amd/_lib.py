@@ -47,6 +47,12 @@ class MtProblem(C.Structure):
         ("demag_approx", C.c_int),
         ("demag_threshold", C.c_double),
         ("toeplitz", C.c_int),
+        ("grid_type", C.c_int),
+        ("grid_pts", _dp), ("grid_abc", _dp),
+        ("n_ave", C.c_int * 3),
+        ("exch_nnz", C.c_int),
+        ("exch_rows_start", C.POINTER(C.c_int)), ("exch_rows_end", C.POINTER(C.c_int)),
+        ("exch_cols", C.POINTER(C.c_int)), ("exch_vals", _dp),
     ]
 
 

@@ -550,6 +550,7 @@ int mt_rk4_create(const mt_micromag_problem* q, double dt, long long nsteps, int
     set_error("no HIP device visible: the MI355X backend needs a GPU");
     return kErrHip;
   }
+  MT_CHECK(q->grid_type == 0 || q->grid_type == 1, "the RK4 adjoint runs on uniform grids (grid_type 1)");
   MT_CHECK(q->nx >= 2 && q->ny >= 2 && q->nz >= 1, "the uniform-grid exchange operator needs nx >= 2 and ny >= 2");
   MT_CHECK(q->n_states >= 1 && q->n_states <= 1024, "n_states must be in [1, 1024]");
   MT_CHECK(dt > 0 && nsteps >= 1 && checkpoint_every >= 1, "need dt > 0, nsteps >= 1, checkpoint_every >= 1");

@@ -41,7 +41,18 @@ __global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc
   const int iz = i / (g.nx * g.ny);
   const int sy = g.nx, sz = g.nx * g.ny;
   double mc[3], hj[3], hm[3], hk[3];
-  for (int c = 0; c < 3; ++c) {
+  if (g.xs) {
+    // loaded operator: alpha = -2 times the row's stored entries in order
+    // (updateExchangeTerms, LandauLifshitzEquationSolver.f90:440-474)
+    const int kb = g.xs[i], ke = g.xe[i];
+    for (int c = 0; c < 3; ++c) {
+      const double* v = m + (size_t)c * n;
+      mc[c] = v[i];
+      double sum = 0.0;
+      for (int k = kb; k < ke; ++k) sum += g.xv[k] * v[g.xc[k]];
+      hj[c] = (-2.0 * sum) * g.jfact[i];
+    }
+  } else for (int c = 0; c < 3; ++c) {
     const double* v = m + (size_t)c * n;
     double vi = v[i];
     mc[c] = vi;

@@ -18,6 +18,10 @@ int launch_tiles_field(const double* tiles, int nt, const double* pts, int npt, 
 int launch_split(const float* K, int n, uint8_t* split, hipStream_t stream);
 // Translation-invariant split-BF16 fragment table of a uniform grid (nx % 32 == 0),
 // toeplitz_tensor_bytes(nx, ny, nz) bytes.
+// Unstructured prisms: pts / abc (n x 3) per cell, receiver averaging over
+// nave[0] x nave[1] x nave[2] sub-points; FP32 and/or FP64 receiver-major out.
+int launch_assemble_avg(const double* pts, const double* abc, int n, const int nave[3], float* out32, double* out64,
+                        hipStream_t stream);
 int launch_toeplitz_assemble(int nx, int ny, int nz, double Lx, double Ly, double Lz, uint8_t* out,
                              hipStream_t stream);
 
@@ -74,6 +78,12 @@ int threshold_for_fraction(const float* K32, int n, double fraction, float* thr,
 struct LlgGrid {
   int n, nx, ny, nz;
   double idx2, idy2, idz2;   // 1/dx^2 etc.
+  // loaded exchange operator (unstructured grids, 0-based CSR: row i holds
+  // entries xs[i] .. xe[i]-1); null: the uniform-grid stencil
+  const int* xs;
+  const int* xe;
+  const int* xc;
+  const double* xv;
   const double* jfact;       // A0 / (mu0 Ms_i)
   const double* kfact;       // K0_i / (mu0 Ms_i)
   const float* ms32;         // Ms_i in FP32 (demag scaling)

@@ -448,6 +448,52 @@ int launch_assemble(const double* src, const double* abc, int abc_stride, const 
   return kOk;
 }
 
+namespace {
+// Unstructured prisms (LandauLifshitzEquationSolver.f90:1044-1133): prism j
+// (edges abc_j, centre p_j) at the nave sub-points (p_i - abc_i/2) +
+// (abc_i/(nave+1)) * (ia, ja, ka) of receiver i, summed in FP64 (ka outer, ia
+// inner) and divided by their number; out32[c*n*n + i*n + j].  One thread per
+// pair, consecutive threads along j (coalesced stores).
+__global__ void __launch_bounds__(256) assemble_avg_kernel(const double* __restrict__ pts,
+                                                           const double* __restrict__ abc, int n, int nax, int nay,
+                                                           int naz, float* __restrict__ out32,
+                                                           double* __restrict__ out64) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  if (tid >= nn) return;
+  const int i = (int)(tid / n), j = (int)(tid % n);
+  const double ax = abc[3 * i], ay = abc[3 * i + 1], az = abc[3 * i + 2];
+  const double dx = ax / (nax + 1), dy = ay / (nay + 1), dz = az / (naz + 1);
+  const double bx = pts[3 * i] - ax / 2, by = pts[3 * i + 1] - ay / 2, bz = pts[3 * i + 2] - az / 2;
+  const double cx = pts[3 * j], cy = pts[3 * j + 1], cz = pts[3 * j + 2];
+  const double* a = abc + 3 * j;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int ka = 1; ka <= naz; ++ka)
+    for (int ja = 1; ja <= nay; ++ja)
+      for (int ia = 1; ia <= nax; ++ia) {
+        const double px = bx + dx * ia, py = by + dy * ja, pz = bz + dz * ka;
+        double k6[6];
+        prism_k6(a[0], a[1], a[2], px - cx, py - cy, pz - cz, k6);
+        for (int c = 0; c < 6; ++c) acc[c] += k6[c];
+      }
+  const double cnt = (double)(nax * nay * naz);
+  for (int c = 0; c < 6; ++c) {
+    const double v = acc[c] / cnt;
+    if (out32) out32[c * nn + tid] = (float)v;
+    if (out64) out64[c * nn + tid] = v;
+  }
+}
+}  // namespace
+
+int launch_assemble_avg(const double* pts, const double* abc, int n, const int nave[3], float* out32, double* out64,
+                        hipStream_t stream) {
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(assemble_avg_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, stream, pts, abc, n,
+                     nave[0], nave[1], nave[2], out32, out64);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
 int launch_toeplitz_assemble(int nx, int ny, int nz, double Lx, double Ly, double Lz, uint8_t* out,
                              hipStream_t stream) {
   if (nx % 32 != 0) {

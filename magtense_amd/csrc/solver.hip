@@ -283,6 +283,8 @@ struct Solver {
   hipStream_t stream = nullptr;
   float* K32 = nullptr;
   uint8_t* Ksplit = nullptr;
+  int *xs = nullptr, *xe = nullptr, *xc = nullptr;  // loaded exchange operator (unstructured grids)
+  double* xv = nullptr;
   uint8_t* Ktab = nullptr;    // translation-invariant fragment table (bf16x2 on uniform grids)
   int demag_approx = 1;       // 1 dense, 2 threshold, 4 threshold fraction
   SparseDemag sparse;
@@ -320,13 +322,14 @@ struct Solver {
       if (p) (void)hipFree(p);
     };
     f(K32); f(Ksplit); f(Ktab); f(jfact); f(kfact); f(aniso); f(ms32); f(pool); f(yout); f(fout); f(packf); f(packb);
-    f(hbuf); f(partials); f(results); f(arena_d);
+    f(hbuf); f(partials); f(results); f(arena_d); f(xs); f(xe); f(xc); f(xv);
     sparse.release();
     if (h_results) (void)hipHostFree(h_results);
     if (arena_h) (void)hipHostFree(arena_h);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
+    xs = xe = xc = nullptr; xv = nullptr;
     K32 = nullptr; Ksplit = nullptr; Ktab = nullptr; jfact = kfact = aniso = nullptr; ms32 = nullptr; pool = yout = fout = nullptr;
     packf = nullptr; packb = nullptr; hbuf = nullptr; partials = results = nullptr; arena_d = nullptr;
     h_results = nullptr; arena_h = nullptr; stream = nullptr; ev0 = ev1 = nullptr;
@@ -1129,8 +1132,19 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
     return kErrArg;
   }
   *out = nullptr;
-  MT_CHECK(q->nx >= 2 && q->ny >= 2 && q->nz >= 1,
-           "the uniform-grid exchange operator needs nx >= 2 and ny >= 2 (LandauLifshitzEquationSolver.f90:1215)");
+  const bool unstructured = q->grid_type == 3;
+  MT_CHECK(q->grid_type == 0 || q->grid_type == 1 || unstructured,
+           "grid_type must be 1 (uniform) or 3 (unstructured prisms); tetrahedra are not implemented");
+  if (unstructured) {
+    MT_CHECK(q->nx >= 1 && q->ny >= 1 && q->nz >= 1 && q->grid_pts && q->grid_abc,
+             "unstructured prisms need grid_pts and grid_abc");
+    MT_CHECK(q->n_ave[0] >= 1 && q->n_ave[1] >= 1 && q->n_ave[2] >= 1, "n_ave entries must be >= 1");
+    MT_CHECK(q->exch_nnz >= 0 && q->exch_rows_start && q->exch_rows_end && (q->exch_nnz == 0 || (q->exch_cols && q->exch_vals)),
+             "unstructured prisms need the exchange operator (exch_rows_start / rows_end / cols / vals)");
+  } else {
+    MT_CHECK(q->nx >= 2 && q->ny >= 2 && q->nz >= 1,
+             "the uniform-grid exchange operator needs nx >= 2 and ny >= 2 (LandauLifshitzEquationSolver.f90:1215)");
+  }
   MT_CHECK(q->n_states >= 1, "n_states must be >= 1");
   MT_CHECK(q->nt >= 2 && q->t, "need at least two output times");
   MT_CHECK(q->nt_hext >= 1 && q->hext, "need an applied-field table");
@@ -1178,21 +1192,41 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   axis(P.ny, q->Ly, P.dy, ys);
   axis(P.nz, q->Lz, P.dz, zs);
   std::vector<double> pts((size_t)n * 3);
-  for (int k = 0; k < P.nz; ++k)
-    for (int j = 0; j < P.ny; ++j)
-      for (int i = 0; i < P.nx; ++i) {
-        size_t ind = (size_t)i + (size_t)j * P.nx + (size_t)k * P.nx * P.ny;
-        pts[ind * 3] = xs[i]; pts[ind * 3 + 1] = ys[j]; pts[ind * 3 + 2] = zs[k];
-      }
+  if (unstructured) {
+    pts.assign(q->grid_pts, q->grid_pts + (size_t)n * 3);
+  } else {
+    for (int k = 0; k < P.nz; ++k)
+      for (int j = 0; j < P.ny; ++j)
+        for (int i = 0; i < P.nx; ++i) {
+          size_t ind = (size_t)i + (size_t)j * P.nx + (size_t)k * P.nx * P.ny;
+          pts[ind * 3] = xs[i]; pts[ind * 3 + 1] = ys[j]; pts[ind * 3 + 2] = zs[k];
+        }
+  }
+  // loaded exchange operator: 1-based rows_start / rows_end / cols -> 0-based, checked
+  std::vector<int> hxs, hxe, hxc;
+  if (unstructured) {
+    hxs.resize(n); hxe.resize(n);
+    hxc.resize(q->exch_nnz);
+    for (int r = 0; r < n; ++r) {
+      hxs[r] = q->exch_rows_start[r] - 1;
+      hxe[r] = q->exch_rows_end[r] - 1;
+      MT_CHECK(hxs[r] >= 0 && hxs[r] <= hxe[r] && hxe[r] <= q->exch_nnz,
+               "exchange operator: rows_start / rows_end out of range (1-based, within exch_nnz)");
+    }
+    for (int k = 0; k < q->exch_nnz; ++k) {
+      hxc[k] = q->exch_cols[k] - 1;
+      MT_CHECK(hxc[k] >= 0 && hxc[k] < n, "exchange operator: column index out of range (1-based)");
+    }
+  }
   MT_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   MT_HIP(hipEventCreate(&P.ev0));
   MT_HIP(hipEventCreate(&P.ev1));
   // tensor
   const size_t nn = (size_t)n * n;
   const bool need32 = P.precision == MT_PREC_FP32 || q->keep_fp32_tensor;
-  const bool can_toep = P.precision == MT_PREC_BF16X2 && !q->K_in && P.nx >= 32 && P.nx % 32 == 0;
+  const bool can_toep = !unstructured && P.precision == MT_PREC_BF16X2 && !q->K_in && P.nx >= 32 && P.nx % 32 == 0;
   MT_CHECK(q->toeplitz != 1 || can_toep,
-           "the translation-invariant tensor needs precision bf16x2, nx % 32 == 0 and no K_in");
+           "the translation-invariant tensor needs a uniform grid, precision bf16x2, nx % 32 == 0 and no K_in");
   const bool use_toep = can_toep && q->toeplitz >= 0;
   const bool needsplit = P.precision == MT_PREC_BF16X2 && !use_toep;
   if (use_toep) {
@@ -1214,6 +1248,22 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       MT_HIP(hipStreamSynchronize(P.stream));
       MT_HIP(hipFree(Kd));
     }
+  } else if (unstructured) {
+    // averaged prism tensor in FP32, split afterwards for the MFMA product
+    float* Kd = P.K32;
+    if (!Kd) MT_HIP(hipMalloc(&Kd, 6 * nn * sizeof(float)));
+    double *dpts = nullptr, *dabc = nullptr;
+    MT_HIP(hipMalloc(&dpts, pts.size() * sizeof(double)));
+    MT_HIP(hipMalloc(&dabc, pts.size() * sizeof(double)));
+    MT_HIP(hipMemcpyAsync(dpts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
+    MT_HIP(hipMemcpyAsync(dabc, q->grid_abc, pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
+    int rc = launch_assemble_avg(dpts, dabc, n, q->n_ave, Kd, nullptr, P.stream);
+    if (rc == kOk && P.Ksplit) rc = launch_split(Kd, n, P.Ksplit, P.stream);
+    MT_HIP(hipStreamSynchronize(P.stream));
+    MT_HIP(hipFree(dpts));
+    MT_HIP(hipFree(dabc));
+    if (!P.K32) MT_HIP(hipFree(Kd));
+    if (rc) return rc;
   } else if (P.K32 || P.Ksplit) {
     double *dpts = nullptr, *dabc = nullptr;
     MT_HIP(hipMalloc(&dpts, pts.size() * sizeof(double)));
@@ -1277,6 +1327,19 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   P.grid.n = n; P.grid.nx = P.nx; P.grid.ny = P.ny; P.grid.nz = P.nz;
   P.grid.idx2 = 1.0 / (P.dx * P.dx); P.grid.idy2 = 1.0 / (P.dy * P.dy); P.grid.idz2 = 1.0 / (P.dz * P.dz);
   P.grid.jfact = P.jfact; P.grid.kfact = P.kfact; P.grid.ms32 = P.ms32; P.grid.aniso = P.aniso;
+  if (unstructured) {
+    MT_HIP(hipMalloc(&P.xs, n * sizeof(int)));
+    MT_HIP(hipMalloc(&P.xe, n * sizeof(int)));
+    MT_HIP(hipMalloc(&P.xc, (q->exch_nnz > 0 ? q->exch_nnz : 1) * sizeof(int)));
+    MT_HIP(hipMalloc(&P.xv, (q->exch_nnz > 0 ? q->exch_nnz : 1) * sizeof(double)));
+    MT_HIP(hipMemcpyAsync(P.xs, hxs.data(), n * sizeof(int), hipMemcpyHostToDevice, P.stream));
+    MT_HIP(hipMemcpyAsync(P.xe, hxe.data(), n * sizeof(int), hipMemcpyHostToDevice, P.stream));
+    if (q->exch_nnz > 0) {
+      MT_HIP(hipMemcpyAsync(P.xc, hxc.data(), q->exch_nnz * sizeof(int), hipMemcpyHostToDevice, P.stream));
+      MT_HIP(hipMemcpyAsync(P.xv, q->exch_vals, q->exch_nnz * sizeof(double), hipMemcpyHostToDevice, P.stream));
+    }
+    P.grid.xs = P.xs; P.grid.xe = P.xe; P.grid.xc = P.xc; P.grid.xv = P.xv;
+  }
   // state vectors: 23 per state
   constexpr int kSlots = 23;
   const size_t slot = (size_t)round_up((int64_t)n3, 64);

@@ -18,13 +18,23 @@ Additive extensions (no reference counterpart):
     pass over the demag tensor.
   * ``solver='explicit'`` returns all field rows, shape (nt, N, nt_Hext, 3)
     (the reference's f2py wrapper declares only one row).
+  * ``set_exchange_matrix(A)`` fills the exch_* arrays of a non-uniform grid
+    from a sparse or dense operator (the Matlab front end's
+    ``setExchangeMatrixSparse``).
 
 Behaviour that differs on purpose: ``cuda`` is accepted but ignored (this
 backend always runs on the GPU and raises when none is visible); the FFT
-``demag_approx`` modes ('fft_thres', 'fft_threshold_fraction') and the
-non-uniform grid types raise ``NotImplementedError`` ('threshold' and
-'threshold_fraction' build the CSR tensor on the device); the demag tensor file (``filename`` longer than two
-characters) is loaded when it exists and written otherwise.
+``demag_approx`` modes ('fft_thres', 'fft_threshold_fraction') and
+``grid_type='tetrahedron'`` raise ``NotImplementedError`` ('threshold' and
+'threshold_fraction' build the CSR tensor on the device); the demag tensor
+file (``filename`` longer than two characters) is loaded when it exists and
+written otherwise.  ``grid_type='unstructuredPrisms'`` (res = [N, 1, 1],
+``grid_pts`` / ``grid_abc`` per cell, ``N_ave`` receiver averaging, the
+exchange operator in ``exch_rows`` / ``exch_rowe`` / ``exch_col`` /
+``exch_val`` as a 1-based CSR) runs as the reference's Fortran does
+(MagTenseMicroMagPyIO.f90:76-85,163-180); ``exch_val`` is float64 here
+because the reference's int32 array (micromag.py:127) truncates every
+physical exchange coefficient (1/d^2 ~ 1e16 m^-2).
 """
 
 import ctypes as C
@@ -47,7 +57,8 @@ def _ptr(a):
 
 
 class MicromagProblem:
-    """Micromagnetic problem on a uniform grid (reference: micromag.py:8-146)."""
+    """Micromagnetic problem on a uniform grid or a grid of unstructured prisms
+    (reference: micromag.py:8-146)."""
 
     def __init__(
         self,
@@ -128,7 +139,7 @@ class MicromagProblem:
         self.alphat[:, 0] = t_alpha
         self.alphat[:, 1] = np.asarray(alpha_fct(t_alpha)).reshape(-1)
 
-        self.exch_val = np.zeros(shape=(exch_nval), dtype=np.int32, order="F")
+        self.exch_val = np.zeros(shape=(exch_nval), dtype=np.float64, order="F")
         self.exch_rows = np.zeros(shape=(exch_nrow), dtype=np.int32, order="F")
         self.exch_rowe = np.zeros(shape=(exch_nrow), dtype=np.int32, order="F")
         self.exch_col = np.zeros(shape=(exch_nval), dtype=np.int32, order="F")
@@ -208,9 +219,27 @@ class MicromagProblem:
                 raise ValueError("precision must be 'fp32' or 'bf16x2'") from None
         return _lib.PREC_FP32
 
+    def set_exchange_matrix(self, A):
+        """Exchange operator of a non-uniform grid (N x N, the second-derivative
+        operator that multiplies -2 A0/(mu0 Ms)) into exch_* as a 1-based CSR."""
+        import scipy.sparse as sp
+
+        A = sp.csr_matrix(A)
+        if A.shape != (self.ntot, self.ntot):
+            raise ValueError(f"the exchange operator must be {self.ntot} x {self.ntot}")
+        A.sort_indices()
+        indptr, indices, data = A.indptr, A.indices, A.data
+        self.exch_nrow = self.ntot
+        self.exch_nval = int(len(data))
+        self.exch_rows = np.asarray(indptr[:-1] + 1, dtype=np.int32, order="F")
+        self.exch_rowe = np.asarray(indptr[1:] + 1, dtype=np.int32, order="F")
+        self.exch_col = np.asarray(indices + 1, dtype=np.int32, order="F")
+        self.exch_val = np.asarray(data, dtype=np.float64, order="F")
+
     def _check_supported(self):
-        if self._grid_type != 1:
-            raise NotImplementedError("only grid_type='uniform' is implemented on the MI355X backend")
+        if self._grid_type not in (1, 3):
+            raise NotImplementedError("grid_type 'uniform' and 'unstructuredPrisms' are implemented on the MI355X "
+                                      "backend ('tetrahedron' is not)")
         if self._dem_appr in (3, 5):
             raise NotImplementedError("the FFT-threshold demag approximations are not implemented "
                                       "(demag_approx None, 'threshold' and 'threshold_fraction' are)")
@@ -220,9 +249,15 @@ class MicromagProblem:
             raise NotImplementedError("solver must be 'explicit' or 'dynamic'")
         if self.CV > 0:
             raise NotImplementedError("demag-field noise (CV > 0) is not implemented")
-        if int(self.grid_n[0]) < 2 or int(self.grid_n[1]) < 2:
+        if self._grid_type == 1 and (int(self.grid_n[0]) < 2 or int(self.grid_n[1]) < 2):
             raise ValueError("the uniform-grid exchange operator needs nx >= 2 and ny >= 2 "
                              "(LandauLifshitzEquationSolver.f90:1215)")
+        if self._grid_type == 3:
+            if len(self.exch_rows) != self.ntot or len(self.exch_rowe) != self.ntot:
+                raise ValueError("unstructured prisms need the exchange operator: exch_rows / exch_rowe of length N "
+                                 "(set_exchange_matrix fills them)")
+            if np.any(np.asarray(self.N_ave) < 1):
+                raise ValueError("N_ave entries must be >= 1")
 
     def _tensor_file(self):
         """Path of the demag tensor file when filename is longer than two characters."""
@@ -277,6 +312,22 @@ class MicromagProblem:
         q.demag_approx = int(self._dem_appr)
         q.toeplitz = int(getattr(self, "_toeplitz", 0))
         q.demag_threshold = float(self.dem_thres)
+        q.grid_type = self._grid_type
+        if self._grid_type == 3:
+            keep["gp"] = _c(np.asarray(self.grid_pts).reshape(n, 3))
+            keep["ga"] = _c(np.asarray(self.grid_abc).reshape(n, 3))
+            keep["xr"] = _c(self.exch_rows, np.int32)
+            keep["xe"] = _c(self.exch_rowe, np.int32)
+            keep["xc"] = _c(self.exch_col, np.int32)
+            keep["xv"] = _c(self.exch_val)
+            q.grid_pts = _ptr(keep["gp"]); q.grid_abc = _ptr(keep["ga"])
+            q.n_ave[0], q.n_ave[1], q.n_ave[2] = (int(v) for v in np.asarray(self.N_ave).reshape(3))
+            q.exch_nnz = int(keep["xc"].size)
+            ip = C.POINTER(C.c_int)
+            q.exch_rows_start = keep["xr"].ctypes.data_as(ip)
+            q.exch_rows_end = keep["xe"].ctypes.data_as(ip)
+            q.exch_cols = keep["xc"].ctypes.data_as(ip)
+            q.exch_vals = _ptr(keep["xv"])
         path = self._tensor_file()
         Kout = None
         if path is not None and os.path.exists(path):
@@ -312,6 +363,8 @@ class MicromagProblem:
         return t_out, y_out, fields
 
     def _pts(self):
+        if self._grid_type == 3:
+            return np.asfortranarray(np.asarray(self.grid_pts, dtype=np.float64).reshape(self.ntot, 3))
         nx, ny, nz = (int(v) for v in self.grid_n)
         Lx, Ly, Lz = (float(v) for v in self.grid_L)
 

@@ -1129,6 +1129,17 @@ typedef struct {
   int return_fields;     // 1: fill H_exc/H_ext/H_dem/H_ani
   int demag_approx;      // 0/1 dense, 2 threshold, 4 threshold fraction
   double demag_threshold;
+  // grid of unstructured prisms (gridTypeUnstructuredPrisms = 3,
+  // MagTenseMicroMagPyIO.f90:76-85,163-180); grid_type 0 or 1: uniform
+  int grid_type;
+  const double* grid_pts;       // n*3 row-major cell centres
+  const double* grid_abc;       // n*3 row-major full edges
+  int n_ave[3];                 // receiver sub-points per axis (N_ave)
+  int exch_nnz;
+  const int* exch_rows_start;   // n, 1-based (mkl_sparse_d_create_csr rows_start)
+  const int* exch_rows_end;     // n, 1-based
+  const int* exch_cols;         // exch_nnz, 1-based
+  const double* exch_vals;      // exch_nnz
 } or_problem;
 
 typedef struct {
@@ -1178,16 +1189,82 @@ void apply_threshold(std::vector<float>& K, float thr) {
     if (!(std::fabs(v) > thr)) v = 0.0f;
 }
 
+// ComputeDemagfieldTensor, unstructured-prism branch
+// (LandauLifshitzEquationSolver.f90:1044-1133): tile j (edges abc_j, centred at
+// p_j) evaluated at the nx_ave x ny_ave x nz_ave sub-points
+// (p_i - abc_i/2) + (abc_i/(n_ave+1)) * (i_a, j_a, k_a) of every receiver cell
+// i, summed in FP64 (k_a outer, i_a inner) and divided by their number.
+void assemble_unstructured(int n, const double* pts, const double* abc, const int nave[3], std::vector<double>& Kd) {
+  size_t nn = (size_t)n * n;
+  Kd.assign(6 * nn, 0.0);
+  const double cnt = (double)(nave[0] * nave[1] * nave[2]);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int j = 0; j < n; ++j) {
+    const double* aj = abc + (size_t)j * 3;
+    const double* cj = pts + (size_t)j * 3;
+    for (int i = 0; i < n; ++i) {
+      const double* pi = pts + (size_t)i * 3;
+      const double* ai = abc + (size_t)i * 3;
+      const double d[3] = {ai[0] / (nave[0] + 1), ai[1] / (nave[1] + 1), ai[2] / (nave[2] + 1)};
+      double acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int ka = 1; ka <= nave[2]; ++ka)
+        for (int ja = 1; ja <= nave[1]; ++ja)
+          for (int ia = 1; ia <= nave[0]; ++ia) {
+            const double px = (pi[0] - ai[0] / 2) + d[0] * ia;
+            const double py = (pi[1] - ai[1] / 2) + d[1] * ja;
+            const double pz = (pi[2] - ai[2] / 2) + d[2] * ka;
+            double k6[6];
+            prism_k6(aj[0], aj[1], aj[2], px - cj[0], py - cj[1], pz - cj[2], k6);
+            for (int c = 0; c < 6; ++c) acc[c] += k6[c];
+          }
+      for (int c = 0; c < 6; ++c) Kd[c * nn + (size_t)i * n + j] = acc[c] / cnt;
+    }
+  }
+}
+
+// ConvertExchangeTerm3D_NonUniform (LandauLifshitzEquationSolver.f90:1530-1544):
+// the loaded 1-based CSR (rows_start / rows_end) as it is; the product sums each
+// row's entries in stored order.
+int loaded_exchange(const or_problem* q, int n, Csr& A) {
+  if (!q->exch_rows_start || !q->exch_rows_end || !q->exch_cols || !q->exch_vals) return -1;
+  A.n = n;
+  A.rowptr.assign(n + 1, 0);
+  A.col.clear(); A.val.clear();
+  for (int r = 0; r < n; ++r) {
+    for (int k = q->exch_rows_start[r] - 1; k < q->exch_rows_end[r] - 1; ++k) {
+      if (k < 0 || k >= q->exch_nnz) return -1;
+      const int c = q->exch_cols[k] - 1;
+      if (c < 0 || c >= n) return -1;
+      A.col.push_back(c);
+      A.val.push_back(q->exch_vals[k]);
+    }
+    A.rowptr[r + 1] = (int)A.col.size();
+  }
+  return 0;
+}
+
 int build_problem(const or_problem* q, Problem& P) {
-  if (q->nx < 2 || q->ny < 2) return -1;
-  setup_grid(q->nx, q->ny, q->nz, q->Lx, q->Ly, q->Lz, P.g);
-  int n = P.g.n;
+  const bool unstructured = q->grid_type == 3;
+  if (q->grid_type != 0 && q->grid_type != 1 && !unstructured) return -3;
+  if (!unstructured && (q->nx < 2 || q->ny < 2)) return -1;
+  int n;
+  if (unstructured) {
+    n = q->nx * q->ny * q->nz;
+    if (!q->grid_pts || !q->grid_abc || q->n_ave[0] < 1 || q->n_ave[1] < 1 || q->n_ave[2] < 1) return -1;
+    P.g.nx = q->nx; P.g.ny = q->ny; P.g.nz = q->nz; P.g.n = n;
+    P.g.pts.assign(q->grid_pts, q->grid_pts + (size_t)n * 3);
+    if (loaded_exchange(q, n, P.Aex)) return -1;
+  } else {
+    setup_grid(q->nx, q->ny, q->nz, q->Lx, q->Ly, q->Lz, P.g);
+    n = P.g.n;
+    build_exchange(P.g, P.Aex);
+  }
   P.n = n;
-  build_exchange(P.g, P.Aex);
   size_t nn = (size_t)n * n;
   P.demag_fp64 = q->demag_fp64 != 0;
   if (P.demag_fp64 || !q->K_in) {
-    assemble_uniform(P.g, P.Kd);
+    if (unstructured) assemble_unstructured(n, q->grid_pts, q->grid_abc, q->n_ave, P.Kd);
+    else assemble_uniform(P.g, P.Kd);
     if (!P.demag_fp64) {
       P.K.resize(6 * nn);
       for (size_t k = 0; k < 6 * nn; ++k) P.K[k] = (float)P.Kd[k];
@@ -1755,4 +1832,16 @@ float or_threshold_fraction(const float* K, long long total, double fraction) {
   return threshold_fraction(K, (size_t)total, fraction);
 }
 
+}  // extern "C"
+
+extern "C" {
+// Receiver-averaged tensor of unstructured prisms (LandauLifshitzEquationSolver.f90:1044-1133),
+// FP64, K6[c*n*n + i*n + j].
+int or_tensor_unstructured(int n, const double* pts, const double* abc, const int* nave, double* K6) {
+  if (n <= 0 || nave[0] < 1 || nave[1] < 1 || nave[2] < 1) return -1;
+  std::vector<double> Kd;
+  assemble_unstructured(n, pts, abc, nave, Kd);
+  std::memcpy(K6, Kd.data(), sizeof(double) * Kd.size());
+  return 0;
+}
 }  // extern "C"

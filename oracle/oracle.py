@@ -39,6 +39,11 @@ class OrProblem(C.Structure):
         ("return_fields", C.c_int),
         ("demag_approx", C.c_int),
         ("demag_threshold", C.c_double),
+        ("grid_type", C.c_int),
+        ("grid_pts", _dp), ("grid_abc", _dp),
+        ("n_ave", C.c_int * 3),
+        ("exch_nnz", C.c_int),
+        ("exch_rows_start", _ip), ("exch_rows_end", _ip), ("exch_cols", _ip), ("exch_vals", _dp),
     ]
 
 
@@ -123,6 +128,18 @@ def tensor_points(src, abc, pts, j0=0, j1=None):
     return out
 
 
+def tensor_unstructured(pts, abc, n_ave=(1, 1, 1)):
+    """Receiver-averaged FP64 tensor of unstructured prisms, shape (6, N, N)."""
+    pts = _c64(pts); abc = _c64(abc)
+    n = pts.shape[0]
+    nave = np.ascontiguousarray(n_ave, dtype=np.int32)
+    K = np.zeros((6, n, n))
+    rc = lib().or_tensor_unstructured(n, _d(pts), _d(abc), nave.ctypes.data_as(_ip), _d(K))
+    if rc:
+        raise ValueError("or_tensor_unstructured: invalid arguments")
+    return K
+
+
 def exchange_csr(res, L):
     nnz = lib().or_exchange_csr(int(res[0]), int(res[1]), int(res[2]), float(L[0]), float(L[1]), float(L[2]),
                                 None, None, None)
@@ -142,7 +159,12 @@ class Problem:
 
     def __init__(self, res, L, Ms=8e5, A0=1.3e-11, K0=0.0, gamma=0.0, alpha=0.02, u_ea=None, solver=2,
                  hext=None, alphat=None, t=None, m0=None, tol=1e-4, thres=1e-6, t_conv=None, conv_tol=1e-4,
-                 demag_fp64=False, K_in=None, return_fields=True, demag_approx=1, dem_thres=0.0):
+                 demag_fp64=False, K_in=None, return_fields=True, demag_approx=1, dem_thres=0.0,
+                 grid_pts=None, grid_abc=None, n_ave=(1, 1, 1), exch=None):
+        """grid_pts / grid_abc ((n, 3) centres and full edges) select the
+        unstructured-prism grid; exch = (rows_start, rows_end, cols, vals) is
+        its exchange operator as a 1-based CSR (rows_start / rows_end of
+        mkl_sparse_d_create_csr)."""
         n = int(np.prod(res))
         self.n = n
         self.res = list(res)
@@ -173,6 +195,21 @@ class Problem:
         q.return_fields = int(bool(return_fields))
         q.demag_approx = int(demag_approx)
         q.demag_threshold = float(dem_thres)
+        q.grid_type = 1
+        if grid_pts is not None:
+            self._gp = _c64(np.asarray(grid_pts, dtype=np.float64).reshape(n, 3))
+            self._ga = _c64(np.asarray(grid_abc, dtype=np.float64).reshape(n, 3))
+            rs, re_, cols, vals = exch
+            self._ex = [np.ascontiguousarray(rs, dtype=np.int32), np.ascontiguousarray(re_, dtype=np.int32),
+                        np.ascontiguousarray(cols, dtype=np.int32), _c64(vals)]
+            q.grid_type = 3
+            q.grid_pts = _d(self._gp); q.grid_abc = _d(self._ga)
+            q.n_ave[0], q.n_ave[1], q.n_ave[2] = (int(v) for v in n_ave)
+            q.exch_nnz = self._ex[2].size
+            q.exch_rows_start = self._ex[0].ctypes.data_as(_ip)
+            q.exch_rows_end = self._ex[1].ctypes.data_as(_ip)
+            q.exch_cols = self._ex[2].ctypes.data_as(_ip)
+            q.exch_vals = _d(self._ex[3])
         self.q = q
 
     def dmdt(self, t, m):

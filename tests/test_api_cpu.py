@@ -99,3 +99,36 @@ def test_tensor_file_round_trip(tmp_path):
     assert np.array_equal(micromag.load_demag_tensor(str(f), 5), K)
     with pytest.raises(ValueError):
         micromag.load_demag_tensor(str(f), 4)
+
+
+def test_unstructured_prisms_marshalling(oracle):
+    """grid_type='unstructuredPrisms': set_exchange_matrix writes the 1-based CSR
+    the Fortran loader passes to mkl_sparse_d_create_csr (rows_start / rows_end,
+    MagTenseMicroMagPyIO.f90:163-180); exch_val keeps float values; the points
+    returned are grid_pts; tetrahedra raise before any GPU call."""
+    import scipy.sparse as sp
+
+    res, L = [6, 3, 1], [30e-9, 9e-9, 3e-9]
+    rowptr, col, val = oracle.exchange_csr(res, L)
+    n = 18
+    A = sp.csr_matrix((val, col, rowptr), shape=(n, n))
+    p = MicromagProblem([n, 1, 1], grid_type="unstructuredPrisms")
+    with pytest.raises(ValueError):  # no exchange operator yet
+        p._check_supported()
+    p.set_exchange_matrix(A)
+    assert p.exch_nval == len(val) and p.exch_nrow == n
+    assert np.array_equal(p.exch_rows, rowptr[:-1] + 1) and np.array_equal(p.exch_rowe, rowptr[1:] + 1)
+    assert np.array_equal(p.exch_col, col + 1)
+    assert p.exch_val.dtype == np.float64 and np.array_equal(p.exch_val, val)
+    assert np.abs(val).max() > 1e15  # physical values that an int32 array cannot hold
+    pts, _ = oracle.setup_grid(res, L)
+    p.grid_pts[:] = pts
+    p._check_supported()
+    assert np.array_equal(p._pts(), pts)
+    with pytest.raises(ValueError):
+        p.set_exchange_matrix(sp.identity(n + 1))
+    p.N_ave = np.array([0, 1, 1], dtype=np.int32)
+    with pytest.raises(ValueError):
+        p._check_supported()
+    with pytest.raises(NotImplementedError):
+        MicromagProblem([4, 1, 1], grid_type="tetrahedron")._check_supported()
