@@ -98,6 +98,8 @@ typedef struct {
   const int* tile_type;  /* tileType */
   const int* use_sym;    /* exploitSymmetry (1 = eight mirrored evaluations, :744-822) */
   const double* sym_ops; /* symmetryOps */
+  const double* vertices; /* (n_tiles,3,4) Fortran order, the vertices of tetrahedron tiles (tile_type 5;
+                             getN_tensor_tetrahedron, TileNComponents.f90:944-961); may be null without them */
 } mt_tiles;
 
 #define MT_ERR_UNSUPPORTED (-6)
@@ -228,6 +230,12 @@ typedef struct {
   const int* exch_rows_end;    /* n, 1-based */
   const int* exch_cols;        /* exch_nnz, 1-based */
   const double* exch_vals;     /* exch_nnz (double; the reference's Python path passes int32, Q7) */
+  /* tetrahedral grid (grid_type 2, gridTypeTetrahedron; LandauLifshitzEquationSolver.f90:988-1041):
+     element i (evaluation point grid_pts[i]) has the 1-based nodes grid_elements[4i .. 4i+3];
+     n_ave is not used; the exchange operator is the loaded CSR as for grid_type 3 */
+  int grid_nnod;
+  const double* grid_nodes;    /* grid_nnod*3 row-major */
+  const int* grid_elements;    /* 4n, 1-based */
 } mt_micromag_problem;
 
 typedef struct {

@@ -53,6 +53,7 @@ class MtProblem(C.Structure):
         ("exch_nnz", C.c_int),
         ("exch_rows_start", C.POINTER(C.c_int)), ("exch_rows_end", C.POINTER(C.c_int)),
         ("exch_cols", C.POINTER(C.c_int)), ("exch_vals", _dp),
+        ("grid_nnod", C.c_int), ("grid_nodes", _dp), ("grid_elements", C.POINTER(C.c_int)),
     ]
 
 
@@ -71,6 +72,7 @@ class MtTiles(C.Structure):
         ("n_tiles", C.c_int),
         ("size", _dp), ("offset", _dp), ("rot", _dp), ("M", _dp),
         ("tile_type", C.POINTER(C.c_int)), ("use_sym", C.POINTER(C.c_int)), ("sym_ops", _dp),
+        ("vertices", _dp),
     ]
 
 

@@ -486,19 +486,33 @@ void rotation_matrices(const double ang[3], double* R, double* Ri) {
   mat3_mul(XtYt, Zt, Ri);
 }
 
-// Fortran-ordered (n,3) host arrays -> device tile records of 32 doubles.
+// Fortran-ordered (n,3) host arrays -> device tile records of kTileRec doubles.
 int tile_records(const mt_tiles* t, bool need_m, std::vector<double>& rec) {
   MT_CHECK(t && t->n_tiles > 0 && t->size && t->offset, "mt_tiles needs n_tiles > 0, size and offset");
   MT_CHECK(!need_m || t->M, "mt_tiles_h_field needs the tile magnetisation M");
   const int nt = t->n_tiles;
-  rec.assign((size_t)nt * 32, 0.0);
+  rec.assign((size_t)nt * kTileRec, 0.0);
   for (int i = 0; i < nt; ++i) {
-    if (t->tile_type && t->tile_type[i] != 2) {
-      set_error("tile " + std::to_string(i) + " has tile_type " + std::to_string(t->tile_type[i]) +
-                "; only rectangular prisms (tile_type 2) are supported");
+    const int type = t->tile_type ? t->tile_type[i] : 2;
+    if (type != 2 && type != 5) {
+      set_error("tile " + std::to_string(i) + " has tile_type " + std::to_string(type) +
+                "; rectangular prisms (2) and tetrahedra (5) are supported");
       return kErrUnsupported;
     }
-    double* r = rec.data() + (size_t)i * 32;
+    MT_CHECK(type != 5 || t->vertices, "tetrahedron tiles (tile_type 5) need the vertices");
+    double* r = rec.data() + (size_t)i * kTileRec;
+    r[31] = type;
+    if (type == 5) {  // vertices (n,3,4) Fortran order: component c of vertex k at i + n*(c + 3k)
+      for (int k = 0; k < 4; ++k)
+        for (int c = 0; c < 3; ++c) r[32 + 3 * k + c] = t->vertices[i + (size_t)nt * (c + 3 * (size_t)k)];
+      double e[3][3];
+      for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 3; ++c) e[k][c] = r[35 + 3 * k + c] - r[32 + c];
+      const double vol = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+                         e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                         e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+      MT_CHECK(vol != 0.0, "tetrahedron tile " + std::to_string(i) + " is degenerate (zero volume)");
+    }
     double ang[3], M[3];
     for (int k = 0; k < 3; ++k) {
       r[k] = t->size[i + (size_t)nt * k];

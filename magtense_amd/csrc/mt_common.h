@@ -50,6 +50,7 @@ constexpr int kTileM = 32;   // rows per A fragment
 constexpr int kTileK = 16;   // k per MFMA
 constexpr int kFragBytes = 1024;
 constexpr int kMaxCols = 64;  // states per GEMM pass
+constexpr int kTileRec = 48;  // doubles per magnetostatic tile record
 
 __host__ __device__ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 

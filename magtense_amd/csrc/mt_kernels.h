@@ -10,8 +10,9 @@ namespace mt {
 // ---- (a) tensor assembly ------------------------------------------------------
 int launch_assemble(const double* src, const double* abc, int abc_stride, const double* rot, const double* pts,
                     int n_src, int n_pts, float* out32, double* out64, uint8_t* split, hipStream_t stream);
-// Prism tiles: tile records of 32 doubles (abc, offset, R, Rinv, R*M, symmetry
-// ops, symmetry flag, pad); points Fortran-ordered (npt,3).
+// Magnetostatic tiles: records of kTileRec doubles (abc, offset, R, Rinv, R*M,
+// symmetry ops, symmetry flag, tile type 2 / 5, tetrahedron vertices); points
+// Fortran-ordered (npt,3); record size kTileRec (mt_common.h).
 int launch_tiles_tensor(const double* tiles, int nt, const double* pts, int npt, double* out, hipStream_t stream);
 int launch_tiles_field(const double* tiles, int nt, const double* pts, int npt, const double* Nin, double* H,
                        hipStream_t stream);
@@ -22,6 +23,10 @@ int launch_split(const float* K, int n, uint8_t* split, hipStream_t stream);
 // nave[0] x nave[1] x nave[2] sub-points; FP32 and/or FP64 receiver-major out.
 int launch_assemble_avg(const double* pts, const double* abc, int n, const int nave[3], float* out32, double* out64,
                         hipStream_t stream);
+// Tetrahedral grid: evaluation points pts (n x 3), element j's nodes
+// elem[4j..4j+3] (0-based) into nodes (x,y,z rows); receiver-major out.
+int launch_assemble_tetra(const double* pts, const double* nodes, const int* elem, int n, float* out32,
+                          double* out64, hipStream_t stream);
 int launch_toeplitz_assemble(int nx, int ny, int nz, double Lx, double Ly, double Lz, uint8_t* out,
                              hipStream_t stream);
 

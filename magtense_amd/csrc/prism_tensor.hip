@@ -317,10 +317,156 @@ __global__ void __launch_bounds__(256) split_kernel(const float* __restrict__ K,
   }
 }
 
-// ---- magnetostatic tiles (getNFromTiles / getHFromTiles, prism tiles) -------
-// Tile record (TileRec, 32 doubles): abc[3] off[3] R[9] Rinv[9] RM[3] ops[3]
-// sym pad, with R/Rinv of getRotationMatrices (DemagFieldGetSolution.f90:925-948)
-// and RM = R M (the rotated magnetisation of getFieldFromTile, :718).
+// ---- tetrahedron (tileType 5) ---------------------------------------------------
+// Sum over the four faces of the right-triangle face tensor
+// (getN_tensor_tetrahedron, TileNComponents.f90:944-961; getN_Triangle and the
+// Nxz / Nyz / Nzz integrals, TileTriangle.f90:26-286), FP64 as the reference
+// builds it (default real of 8 bytes).  Returns false for a degenerate face
+// (the reference stops); the caller writes NaN then.
+namespace tet {
+constexpr double kPi = 3.14159265359;  // TileTriangle.f90:6, as written
+
+__device__ double F_Nxz(const double r[3], double yp, double l, double h) {
+  return h / sqrt(h * h + l * l) *
+         atanh((l * l - l * r[0] + h * r[1] - h * yp * (1 + l * l / (h * h))) /
+               (sqrt(h * h + l * l) * sqrt(l * l - 2 * l * r[0] + r[0] * r[0] + r[1] * r[1] -
+                                           2 * yp * (l * l - l * r[0] + h * r[1]) / h +
+                                           yp * yp * (1. + l * l / (h * h)) + r[2] * r[2])));
+}
+__device__ double G_Nxz(const double r[3], double yp) {
+  const double g = atanh((r[1] - yp) / (sqrt(r[0] * r[0] + (r[1] - yp) * (r[1] - yp) + r[2] * r[2])));
+  return isfinite(g) ? g : 0.0;
+}
+__device__ double K_Nyz(const double r[3], double xp, double l, double h) {
+  return l / sqrt(h * h + l * l) *
+         atanh((h * h - h * r[1] + l * r[0] - l * xp * (1 + h * h / (l * l))) /
+               (sqrt(h * h + l * l) * sqrt(h * h - 2 * h * r[1] + r[0] * r[0] + r[1] * r[1] -
+                                           2 * xp * (h * h + l * r[0] - h * r[1]) / l +
+                                           xp * xp * (1 + h * h / (l * l)) + r[2] * r[2])));
+}
+__device__ double L_Nyz(const double r[3], double xp) {
+  const double g = atanh((r[0] - xp) / (sqrt((r[0] - xp) * (r[0] - xp) + r[1] * r[1] + r[2] * r[2])));
+  return isfinite(g) ? g : 0.0;
+}
+__device__ double P_Nzz(const double r[3], double xp, double l, double h) {
+  return atan((r[0] * (h - r[1]) - xp * (h * (1 - r[0] / l) - r[1]) - h * (r[0] * r[0] + r[2] * r[2]) / l) /
+              (r[2] * sqrt(h * h + r[0] * r[0] + xp * xp * (1 + h * h / (l * l)) - 2 * h * r[1] + r[1] * r[1] -
+                           2 * xp * (h * h + l * r[0] - h * r[1]) / l + r[2] * r[2])));
+}
+__device__ double Q_Nzz(const double r[3], double xp) {
+  return -atan((r[0] - xp) * r[1] / (r[2] * sqrt((r[0] - xp) * (r[0] - xp) + r[1] * r[1] + r[2] * r[2])));
+}
+
+__device__ inline double nrm(const double a[3]) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+__device__ inline double dt(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ inline void sb(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[0] - b[0]; c[1] = a[1] - b[1]; c[2] = a[2] - b[2];
+}
+__device__ inline void cr(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ inline void cp(double d[3], const double s[3]) { d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; }
+
+// getN_Triangle: face v[0..2], v[3] behind it; N (row-major 3x3) += face tensor
+__device__ bool triangle_add(const double vin[4][3], const double r[3], double N[9]) {
+  const double numErr = 1e-25, threshold = 1e-20;
+  double v[4][3];
+  for (int k = 0; k < 4; ++k) cp(v[k], vin[k]);
+  double a[3], b[3], c[3];
+  sb(v[0], v[1], a); sb(v[1], v[2], b); sb(v[0], v[2], c);
+  if (fabs(nrm(c) - (nrm(a) + nrm(b))) < numErr) return false;
+  double ang[3];
+  sb(v[0], v[1], a); sb(v[0], v[2], b); ang[0] = acos(dt(a, b) / (nrm(a) * nrm(b)));
+  sb(v[1], v[0], a); sb(v[1], v[2], b); ang[1] = acos(dt(a, b) / (nrm(a) * nrm(b)));
+  sb(v[2], v[1], a); sb(v[2], v[0], b); ang[2] = acos(dt(a, b) / (nrm(a) * nrm(b)));
+  // simple_sort (SpecialFunctions.f90:13-34): repeated MINLOC, first of equals
+  int ind[3];
+  bool mk[3] = {true, true, true};
+  double srt[3];
+  for (int i = 0; i < 3; ++i) {
+    int best = -1;
+    for (int k = 0; k < 3; ++k)
+      if (mk[k] && (best < 0 || ang[k] < ang[best])) best = k;
+    srt[i] = ang[best]; ind[i] = best; mk[best] = false;
+  }
+  double w[3][3];
+  for (int i = 0; i < 3; ++i) cp(w[i], v[ind[i]]);
+  // sorted order with the largest angle in the middle: v = (w0, w2, w1)
+  cp(v[0], w[0]); cp(v[1], w[2]); cp(v[2], w[1]);
+  ang[0] = srt[0]; ang[1] = srt[2]; ang[2] = srt[1];
+  double e1[3], e2[3], e3[3], d32[3], d41[3];
+  sb(v[0], v[2], e1);
+  double n1 = nrm(e1);
+  for (int k = 0; k < 3; ++k) e1[k] = e1[k] / n1;
+  sb(v[1], v[2], d32);
+  cr(e1, d32, e3);
+  double n3 = nrm(e3);
+  for (int k = 0; k < 3; ++k) e3[k] = e3[k] / n3;
+  sb(v[3], v[0], d41);
+  if (fabs(dt(d41, e3)) <= numErr) return false;
+  if (dt(e3, d41) > 0.) {
+    double t[3];
+    cp(t, v[0]); cp(v[0], v[2]); cp(v[2], t);
+    const double ta = ang[0]; ang[0] = ang[2]; ang[2] = ta;
+    sb(v[0], v[2], e1);
+    n1 = nrm(e1);
+    for (int k = 0; k < 3; ++k) e1[k] = e1[k] / n1;
+    sb(v[1], v[2], d32);
+    cr(e1, d32, e3);
+    n3 = nrm(e3);
+    for (int k = 0; k < 3; ++k) e3[k] = e3[k] / n3;
+  }
+  cr(e3, e1, e2);
+  // P = [e1 e2 e3] (columns); local coordinates q' = P^T q
+  sb(v[1], v[2], d32);
+  const double cd = cos(ang[2]) * nrm(d32);
+  double D[3], q[3];
+  for (int k = 0; k < 3; ++k) D[k] = cd * e1[k] + v[2][k];
+  const double* E[3] = {e1, e2, e3};
+  double vp00, vp11, vp20, rt[3];
+  sb(v[0], D, q); vp00 = dt(E[0], q);   // vp(1,1)
+  sb(v[1], D, q); vp11 = dt(E[1], q);   // vp(2,2)
+  sb(v[2], D, q); vp20 = dt(E[0], q);   // vp(1,3)
+  sb(r, D, q);
+  for (int k = 0; k < 3; ++k) rt[k] = dt(E[k], q);  // Pinv (r - D), Pinv = P^T
+  for (int k = 0; k < 3; ++k)
+    if (fabs(rt[k]) < threshold) rt[k] = rt[k] < 0 ? -threshold : threshold;
+  const double pre = -1. / (4. * kPi);
+  const double n13 = pre * (F_Nxz(rt, vp11, vp00, vp11) - F_Nxz(rt, 0., vp00, vp11) - (G_Nxz(rt, vp11) - G_Nxz(rt, 0.))) -
+                     pre * (F_Nxz(rt, vp11, vp20, vp11) - F_Nxz(rt, 0., vp20, vp11) - (G_Nxz(rt, vp11) - G_Nxz(rt, 0.)));
+  const double n23 = pre * (K_Nyz(rt, vp00, vp00, vp11) - K_Nyz(rt, 0., vp00, vp11) - (L_Nyz(rt, vp00) - L_Nyz(rt, 0.))) -
+                     pre * (K_Nyz(rt, vp20, vp20, vp11) - K_Nyz(rt, 0., vp20, vp11) - (L_Nyz(rt, vp20) - L_Nyz(rt, 0.)));
+  const double n33 = pre * (P_Nzz(rt, vp00, vp00, vp11) - P_Nzz(rt, 0., vp00, vp11) - (Q_Nzz(rt, vp00) - Q_Nzz(rt, 0.))) -
+                     pre * (P_Nzz(rt, vp20, vp20, vp11) - P_Nzz(rt, 0., vp20, vp11) - (Q_Nzz(rt, vp20) - Q_Nzz(rt, 0.)));
+  // N += (P N_loc) P^T with N_loc(:,3) = (n13, n23, n33)
+  double pn[3];
+  for (int a2 = 0; a2 < 3; ++a2) pn[a2] = e1[a2] * n13 + e2[a2] * n23 + e3[a2] * n33;
+  for (int a2 = 0; a2 < 3; ++a2)
+    for (int b2 = 0; b2 < 3; ++b2) N[a2 * 3 + b2] = N[a2 * 3 + b2] + pn[a2] * e3[b2];
+  return true;
+}
+}  // namespace tet
+
+// getN_tensor_tetrahedron: faces cshift(vert, i-1, 2), i = 1..4
+__device__ bool tetra_n(const double* __restrict__ vert, const double r[3], double N[9]) {
+  for (int q = 0; q < 9; ++q) N[q] = 0.0;
+  for (int i = 0; i < 4; ++i) {
+    double v[4][3];
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) v[k][c] = vert[((k + i) & 3) * 3 + c];
+    if (!tet::triangle_add(v, r, N)) return false;
+  }
+  return true;
+}
+
+// ---- magnetostatic tiles (getNFromTiles / getHFromTiles) ---------------------
+// Tile record (kTileRec = 48 doubles): abc[3] off[3] R[9] Rinv[9] RM[3] ops[3]
+// sym type vert[12], with R/Rinv of getRotationMatrices
+// (DemagFieldGetSolution.f90:925-948), RM = R M (the rotated magnetisation of
+// getFieldFromTile, :718), type 2 (prism) or 5 (tetrahedron, vertices v_k at
+// vert[3k..3k+2]).
 
 // Tensor of one tile at point (px,py,pz): getFieldFromTile (:685-730), or the
 // eight mirrored evaluations of getFieldFromTile_symm (:744-822).  The symmetry
@@ -328,6 +474,7 @@ __global__ void __launch_bounds__(256) split_kernel(const float* __restrict__ K,
 // diagonal mirror (for H and the point), applied here as sign products.
 __device__ void tile_tensor(const double* __restrict__ T, double px, double py, double pz, double N[9]) {
   const double* R = T + 6;
+  const bool tetra = T[31] == 5.0;
   int sym = T[30] != 0.0;
   int cases = sym ? 8 : 1;
   double o0 = T[27], o1 = T[28], o2 = T[29];
@@ -344,9 +491,18 @@ __device__ void tile_tensor(const double* __restrict__ T, double px, double py, 
     double rx = R[0] * dx + R[1] * dy + R[2] * dz;
     double ry = R[3] * dx + R[4] * dy + R[5] * dz;
     double rz = R[6] * dx + R[7] * dy + R[8] * dz;
-    double k[6];
-    prism_k6(T[0], T[1], T[2], rx, ry, rz, k);
-    double full[9] = {k[0], k[1], k[2], k[1], k[3], k[4], k[2], k[4], k[5]};
+    double full[9];
+    if (tetra) {
+      const double r3[3] = {rx, ry, rz};
+      if (!tetra_n(T + 32, r3, full))
+        for (int q = 0; q < 9; ++q) full[q] = __builtin_nan("");
+    } else {
+      double k[6];
+      prism_k6(T[0], T[1], T[2], rx, ry, rz, k);
+      full[0] = k[0]; full[1] = k[1]; full[2] = k[2];
+      full[3] = k[1]; full[4] = k[3]; full[5] = k[4];
+      full[6] = k[2]; full[7] = k[4]; full[8] = k[5];
+    }
     if (!sym) {
       for (int q = 0; q < 9; ++q) N[q] = full[q];
     } else {
@@ -363,7 +519,7 @@ __global__ void __launch_bounds__(256) tiles_tensor_kernel(const double* __restr
                                                            double* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt) return;
-  const double* T = tiles + (size_t)i * 32;
+  const double* T = tiles + (size_t)i * kTileRec;
   for (int p = blockIdx.y; p < npt; p += gridDim.y) {
     double N[9];
     tile_tensor(T, pts[p], pts[p + (size_t)npt], pts[p + 2 * (size_t)npt], N);
@@ -384,7 +540,7 @@ __global__ void __launch_bounds__(256) tiles_field_kernel(const double* __restri
     double px = pts[p], py = pts[p + (size_t)npt], pz = pts[p + 2 * (size_t)npt];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < nt; i += blockDim.x) {
-      const double* T = tiles + (size_t)i * 32;
+      const double* T = tiles + (size_t)i * kTileRec;
       double N[9];
       if (Nin) {
         for (int a = 0; a < 3; ++a)
@@ -484,6 +640,42 @@ __global__ void __launch_bounds__(256) assemble_avg_kernel(const double* __restr
   }
 }
 }  // namespace
+
+namespace {
+// Tetrahedral grid (LandauLifshitzEquationSolver.f90:988-1041): element j
+// (nodes elem[4j..4j+3], 0-based here) at evaluation point i, no offset or
+// rotation; stored components N(1,1) N(1,2) N(1,3) N(2,2) N(2,3) N(3,3).
+__global__ void __launch_bounds__(256) assemble_tetra_kernel(const double* __restrict__ pts,
+                                                             const double* __restrict__ nodes,
+                                                             const int* __restrict__ elem, int n,
+                                                             float* __restrict__ out32, double* __restrict__ out64) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  if (tid >= nn) return;
+  const int i = (int)(tid / n), j = (int)(tid % n);
+  double v[12];
+  for (int k = 0; k < 4; ++k)
+    for (int c = 0; c < 3; ++c) v[3 * k + c] = nodes[(int64_t)elem[4 * j + k] * 3 + c];
+  const double r[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+  double N[9];
+  if (!tetra_n(v, r, N))
+    for (int q = 0; q < 9; ++q) N[q] = __builtin_nan("");
+  const double k6[6] = {N[0], N[1], N[2], N[4], N[5], N[8]};
+  for (int c = 0; c < 6; ++c) {
+    if (out32) out32[c * nn + tid] = (float)k6[c];
+    if (out64) out64[c * nn + tid] = k6[c];
+  }
+}
+}  // namespace
+
+int launch_assemble_tetra(const double* pts, const double* nodes, const int* elem, int n, float* out32,
+                          double* out64, hipStream_t stream) {
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(assemble_tetra_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, stream, pts, nodes,
+                     elem, n, out32, out64);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
 
 int launch_assemble_avg(const double* pts, const double* abc, int n, const int nave[3], float* out32, double* out64,
                         hipStream_t stream) {

@@ -1132,15 +1132,22 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
     return kErrArg;
   }
   *out = nullptr;
-  const bool unstructured = q->grid_type == 3;
+  const bool tetra = q->grid_type == 2;
+  const bool unstructured = q->grid_type == 3 || tetra;  // loaded points and exchange operator
   MT_CHECK(q->grid_type == 0 || q->grid_type == 1 || unstructured,
-           "grid_type must be 1 (uniform) or 3 (unstructured prisms); tetrahedra are not implemented");
-  if (unstructured) {
+           "grid_type must be 1 (uniform), 2 (tetrahedra) or 3 (unstructured prisms)");
+  if (tetra) {
+    MT_CHECK(q->nx >= 1 && q->ny >= 1 && q->nz >= 1 && q->grid_pts && q->grid_nodes && q->grid_elements &&
+                 q->grid_nnod >= 4,
+             "tetrahedra need grid_pts, grid_nodes (grid_nnod >= 4) and grid_elements");
+  } else if (unstructured) {
     MT_CHECK(q->nx >= 1 && q->ny >= 1 && q->nz >= 1 && q->grid_pts && q->grid_abc,
              "unstructured prisms need grid_pts and grid_abc");
     MT_CHECK(q->n_ave[0] >= 1 && q->n_ave[1] >= 1 && q->n_ave[2] >= 1, "n_ave entries must be >= 1");
+  }
+  if (unstructured) {
     MT_CHECK(q->exch_nnz >= 0 && q->exch_rows_start && q->exch_rows_end && (q->exch_nnz == 0 || (q->exch_cols && q->exch_vals)),
-             "unstructured prisms need the exchange operator (exch_rows_start / rows_end / cols / vals)");
+             "non-uniform grids need the exchange operator (exch_rows_start / rows_end / cols / vals)");
   } else {
     MT_CHECK(q->nx >= 2 && q->ny >= 2 && q->nz >= 1,
              "the uniform-grid exchange operator needs nx >= 2 and ny >= 2 (LandauLifshitzEquationSolver.f90:1215)");
@@ -1249,19 +1256,48 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       MT_HIP(hipFree(Kd));
     }
   } else if (unstructured) {
-    // averaged prism tensor in FP32, split afterwards for the MFMA product
+    // averaged prism tensor / tetrahedron tensor in FP32, split afterwards for the MFMA product
+    double *dpts = nullptr, *dabc = nullptr;
+    int* delem = nullptr;
+    std::vector<int> elem0;
+    const size_t geo = tetra ? (size_t)q->grid_nnod * 3 : pts.size();
+    if (tetra) {  // 1-based node indices -> 0-based, checked; degenerate elements refused
+      elem0.resize((size_t)n * 4);
+      for (size_t k = 0; k < elem0.size(); ++k) {
+        elem0[k] = q->grid_elements[k] - 1;
+        MT_CHECK(elem0[k] >= 0 && elem0[k] < q->grid_nnod, "grid_elements: node index out of range (1-based)");
+      }
+      for (int e = 0; e < n; ++e) {
+        const double* v0 = q->grid_nodes + (size_t)elem0[4 * e] * 3;
+        double d[3][3];
+        for (int k = 0; k < 3; ++k)
+          for (int c = 0; c < 3; ++c) d[k][c] = q->grid_nodes[(size_t)elem0[4 * e + 1 + k] * 3 + c] - v0[c];
+        const double vol = d[0][0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1]) -
+                           d[0][1] * (d[1][0] * d[2][2] - d[1][2] * d[2][0]) +
+                           d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
+        MT_CHECK(vol != 0.0, "grid_elements: degenerate tetrahedron (zero volume)");
+      }
+    }
     float* Kd = P.K32;
     if (!Kd) MT_HIP(hipMalloc(&Kd, 6 * nn * sizeof(float)));
-    double *dpts = nullptr, *dabc = nullptr;
     MT_HIP(hipMalloc(&dpts, pts.size() * sizeof(double)));
-    MT_HIP(hipMalloc(&dabc, pts.size() * sizeof(double)));
+    MT_HIP(hipMalloc(&dabc, geo * sizeof(double)));
     MT_HIP(hipMemcpyAsync(dpts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
-    MT_HIP(hipMemcpyAsync(dabc, q->grid_abc, pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
-    int rc = launch_assemble_avg(dpts, dabc, n, q->n_ave, Kd, nullptr, P.stream);
+    MT_HIP(hipMemcpyAsync(dabc, tetra ? q->grid_nodes : q->grid_abc, geo * sizeof(double), hipMemcpyHostToDevice,
+                          P.stream));
+    int rc = kOk;
+    if (tetra) {
+      MT_HIP(hipMalloc(&delem, elem0.size() * sizeof(int)));
+      MT_HIP(hipMemcpyAsync(delem, elem0.data(), elem0.size() * sizeof(int), hipMemcpyHostToDevice, P.stream));
+      rc = launch_assemble_tetra(dpts, dabc, delem, n, Kd, nullptr, P.stream);
+    } else {
+      rc = launch_assemble_avg(dpts, dabc, n, q->n_ave, Kd, nullptr, P.stream);
+    }
     if (rc == kOk && P.Ksplit) rc = launch_split(Kd, n, P.Ksplit, P.stream);
     MT_HIP(hipStreamSynchronize(P.stream));
     MT_HIP(hipFree(dpts));
     MT_HIP(hipFree(dabc));
+    if (delem) MT_HIP(hipFree(delem));
     if (!P.K32) MT_HIP(hipFree(Kd));
     if (rc) return rc;
   } else if (P.K32 || P.Ksplit) {

@@ -4,9 +4,11 @@
 
 The tensor and field evaluations run in ``libmagtense_hip.so``
 (``mt_tiles_demag_tensor`` / ``mt_tiles_h_field``, the prism part of
-getNFromTiles / getHFromTiles, FortranToPythonIO.f90:25-223).  Only
-rectangular prisms (``tile_type == 2``) are supported; any other tile type
-raises ``NotImplementedError``.  The iterative soft-magnet solver
+getNFromTiles / getHFromTiles, FortranToPythonIO.f90:25-223).  Rectangular
+prisms (``tile_type == 2``) and tetrahedra (``tile_type == 5``, ``vertices``
+(3 x 4 per tile) in the tile's frame, the offset subtracted from the point as
+for every tile) are supported; any other tile type raises
+``NotImplementedError``.  The iterative soft-magnet solver
 (``run_simulation`` / ``iterate_magnetization`` of the reference) is not part
 of this package.
 
@@ -25,6 +27,7 @@ from . import _lib
 
 MU0 = 4 * np.pi * 1e-7
 PRISM = 2
+TETRAHEDRON = 5
 
 
 class Tile:
@@ -381,11 +384,12 @@ def grid_config(spots: Union[List, np.ndarray], area: Union[List, np.ndarray], f
 def _tiles_struct(tiles: Tiles, with_m: bool):
     """Fortran-ordered copies of the tile arrays and the mt_tiles struct over them."""
     n = tiles.n
-    bad = np.nonzero(np.asarray(tiles.tile_type).reshape(n) != PRISM)[0]
+    types = np.asarray(tiles.tile_type).reshape(n)
+    bad = np.nonzero((types != PRISM) & (types != TETRAHEDRON))[0]
     if bad.size:
         raise NotImplementedError(
-            f"tile {int(bad[0])} has tile_type {int(tiles.tile_type[bad[0]])}: only rectangular prisms "
-            "(tile_type 2) are supported by the MI355X backend")
+            f"tile {int(bad[0])} has tile_type {int(tiles.tile_type[bad[0]])}: rectangular prisms (2) and "
+            "tetrahedra (5) are supported by the MI355X backend")
     keep = {
         "size": np.asfortranarray(np.asarray(tiles.size, dtype=np.float64).reshape(n, 3)),
         "offset": np.asfortranarray(np.asarray(tiles.offset, dtype=np.float64).reshape(n, 3)),
@@ -394,6 +398,7 @@ def _tiles_struct(tiles: Tiles, with_m: bool):
         "tile_type": np.ascontiguousarray(np.asarray(tiles.tile_type, dtype=np.int32).reshape(n)),
         "use_sym": np.ascontiguousarray(np.asarray(tiles.use_sym, dtype=np.int32).reshape(n)),
         "sym_ops": np.asfortranarray(np.asarray(tiles.sym_op, dtype=np.float64).reshape(n, 3)),
+        "vertices": np.asfortranarray(np.asarray(tiles.vertices, dtype=np.float64).reshape(n, 3, 4)),
     }
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int)
@@ -404,6 +409,7 @@ def _tiles_struct(tiles: Tiles, with_m: bool):
     st.M = keep["M"].ctypes.data_as(dp) if with_m else None
     st.tile_type = keep["tile_type"].ctypes.data_as(ip)
     st.use_sym = keep["use_sym"].ctypes.data_as(ip)
+    st.vertices = keep["vertices"].ctypes.data_as(dp)
     return st, keep
 
 

@@ -24,8 +24,8 @@ Additive extensions (no reference counterpart):
 
 Behaviour that differs on purpose: ``cuda`` is accepted but ignored (this
 backend always runs on the GPU and raises when none is visible); the FFT
-``demag_approx`` modes ('fft_thres', 'fft_threshold_fraction') and
-``grid_type='tetrahedron'`` raise ``NotImplementedError`` ('threshold' and
+``demag_approx`` modes ('fft_thres', 'fft_threshold_fraction') raise
+``NotImplementedError`` ('threshold' and
 'threshold_fraction' build the CSR tensor on the device); the demag tensor
 file (``filename`` longer than two characters) is loaded when it exists and
 written otherwise.  ``grid_type='unstructuredPrisms'`` (res = [N, 1, 1],
@@ -34,7 +34,15 @@ exchange operator in ``exch_rows`` / ``exch_rowe`` / ``exch_col`` /
 ``exch_val`` as a 1-based CSR) runs as the reference's Fortran does
 (MagTenseMicroMagPyIO.f90:76-85,163-180); ``exch_val`` is float64 here
 because the reference's int32 array (micromag.py:127) truncates every
-physical exchange coefficient (1/d^2 ~ 1e16 m^-2).
+physical exchange coefficient (1/d^2 ~ 1e16 m^-2).  ``grid_type='tetrahedron'``
+(res = [N, 1, 1], ``grid_nnod`` nodes in ``grid_nod`` (grid_nnod x 3),
+``grid_ele`` (4 x N) 1-based node indices per element, ``grid_pts`` the
+evaluation points, the same exchange arrays) assembles the tetrahedron tensor
+(LandauLifshitzEquationSolver.f90:988-1041) with the element's vertices in
+global coordinates; ``grid_nod`` is read as N x 3 coordinates (the
+reference's loader assigns the (grid_nnod, 3) array to a (3, grid_nnod) one,
+MagTenseMicroMagPyIO.f90:67-70, which reorders its memory instead of
+transposing it).
 """
 
 import ctypes as C
@@ -237,9 +245,8 @@ class MicromagProblem:
         self.exch_val = np.asarray(data, dtype=np.float64, order="F")
 
     def _check_supported(self):
-        if self._grid_type not in (1, 3):
-            raise NotImplementedError("grid_type 'uniform' and 'unstructuredPrisms' are implemented on the MI355X "
-                                      "backend ('tetrahedron' is not)")
+        if self._grid_type not in (1, 2, 3):
+            raise NotImplementedError("grid_type must be 'uniform', 'tetrahedron' or 'unstructuredPrisms'")
         if self._dem_appr in (3, 5):
             raise NotImplementedError("the FFT-threshold demag approximations are not implemented "
                                       "(demag_approx None, 'threshold' and 'threshold_fraction' are)")
@@ -252,11 +259,17 @@ class MicromagProblem:
         if self._grid_type == 1 and (int(self.grid_n[0]) < 2 or int(self.grid_n[1]) < 2):
             raise ValueError("the uniform-grid exchange operator needs nx >= 2 and ny >= 2 "
                              "(LandauLifshitzEquationSolver.f90:1215)")
-        if self._grid_type == 3:
+        if self._grid_type == 2:
+            if int(self.grid_nnod) < 4 or np.asarray(self.grid_nod).shape != (int(self.grid_nnod), 3):
+                raise ValueError("tetrahedra need grid_nnod >= 4 nodes in grid_nod (grid_nnod x 3)")
+            ele = np.asarray(self.grid_ele)
+            if ele.shape != (4, self.ntot) or ele.min() < 1 or ele.max() > int(self.grid_nnod):
+                raise ValueError("tetrahedra need grid_ele (4 x N, 1-based node indices within grid_nnod)")
+        if self._grid_type in (2, 3):
             if len(self.exch_rows) != self.ntot or len(self.exch_rowe) != self.ntot:
                 raise ValueError("unstructured prisms need the exchange operator: exch_rows / exch_rowe of length N "
                                  "(set_exchange_matrix fills them)")
-            if np.any(np.asarray(self.N_ave) < 1):
+            if self._grid_type == 3 and np.any(np.asarray(self.N_ave) < 1):
                 raise ValueError("N_ave entries must be >= 1")
 
     def _tensor_file(self):
@@ -313,9 +326,15 @@ class MicromagProblem:
         q.toeplitz = int(getattr(self, "_toeplitz", 0))
         q.demag_threshold = float(self.dem_thres)
         q.grid_type = self._grid_type
-        if self._grid_type == 3:
+        if self._grid_type in (2, 3):
             keep["gp"] = _c(np.asarray(self.grid_pts).reshape(n, 3))
             keep["ga"] = _c(np.asarray(self.grid_abc).reshape(n, 3))
+            if self._grid_type == 2:
+                keep["gn"] = _c(np.asarray(self.grid_nod).reshape(-1, 3))
+                keep["ge"] = _c(np.rint(np.asarray(self.grid_ele, dtype=np.float64)).T.reshape(n, 4), np.int32)
+                q.grid_nnod = int(keep["gn"].shape[0])
+                q.grid_nodes = _ptr(keep["gn"])
+                q.grid_elements = keep["ge"].ctypes.data_as(C.POINTER(C.c_int))
             keep["xr"] = _c(self.exch_rows, np.int32)
             keep["xe"] = _c(self.exch_rowe, np.int32)
             keep["xc"] = _c(self.exch_col, np.int32)
@@ -363,7 +382,7 @@ class MicromagProblem:
         return t_out, y_out, fields
 
     def _pts(self):
-        if self._grid_type == 3:
+        if self._grid_type in (2, 3):
             return np.asfortranarray(np.asarray(self.grid_pts, dtype=np.float64).reshape(self.ntot, 3))
         nx, ny, nz = (int(v) for v in self.grid_n)
         Lx, Ly, Lz = (float(v) for v in self.grid_L)

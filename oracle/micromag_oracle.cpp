@@ -1140,6 +1140,11 @@ typedef struct {
   const int* exch_rows_end;     // n, 1-based
   const int* exch_cols;         // exch_nnz, 1-based
   const double* exch_vals;      // exch_nnz
+  // tetrahedral grid (gridTypeTetrahedron = 2): grid_pts are the evaluation
+  // points, element i has the 1-based nodes grid_elements[4i .. 4i+3]
+  int grid_nnod;
+  const double* grid_nodes;     // grid_nnod*3 row-major
+  const int* grid_elements;     // 4n
 } or_problem;
 
 typedef struct {
@@ -1222,6 +1227,33 @@ void assemble_unstructured(int n, const double* pts, const double* abc, const in
   }
 }
 
+// ComputeDemagfieldTensor, tetrahedron branch (LandauLifshitzEquationSolver.f90:988-1041):
+// element j (vertices = its four nodes, offset 0, no rotation) at every
+// evaluation point i; the upper triangle N(1,1) N(1,2) N(1,3) N(2,2) N(2,3)
+// N(3,3) is stored (:1026-1037).  N_ave is not used for tetrahedra (:990-992).
+int tetra_tensor_at(const double* vert12, const double r[3], double N9[9]);
+int assemble_tetra(int n, const double* pts, int nnod, const double* nodes, const int* elem, std::vector<double>& Kd) {
+  size_t nn = (size_t)n * n;
+  Kd.assign(6 * nn, 0.0);
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < 4; ++k)
+      if (elem[4 * j + k] < 1 || elem[4 * j + k] > nnod) return -1;
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : bad)
+  for (int j = 0; j < n; ++j) {
+    double v[12];
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) v[3 * k + c] = nodes[(size_t)(elem[4 * j + k] - 1) * 3 + c];
+    for (int i = 0; i < n; ++i) {
+      double N[9];
+      if (tetra_tensor_at(v, pts + (size_t)i * 3, N)) { ++bad; continue; }
+      const double k6[6] = {N[0], N[1], N[2], N[4], N[5], N[8]};
+      for (int c = 0; c < 6; ++c) Kd[c * nn + (size_t)i * n + j] = k6[c];
+    }
+  }
+  return bad ? -1 : 0;
+}
+
 // ConvertExchangeTerm3D_NonUniform (LandauLifshitzEquationSolver.f90:1530-1544):
 // the loaded 1-based CSR (rows_start / rows_end) as it is; the product sums each
 // row's entries in stored order.
@@ -1244,13 +1276,16 @@ int loaded_exchange(const or_problem* q, int n, Csr& A) {
 }
 
 int build_problem(const or_problem* q, Problem& P) {
-  const bool unstructured = q->grid_type == 3;
+  const bool tetra = q->grid_type == 2;
+  const bool unstructured = q->grid_type == 3 || tetra;
   if (q->grid_type != 0 && q->grid_type != 1 && !unstructured) return -3;
   if (!unstructured && (q->nx < 2 || q->ny < 2)) return -1;
   int n;
   if (unstructured) {
     n = q->nx * q->ny * q->nz;
-    if (!q->grid_pts || !q->grid_abc || q->n_ave[0] < 1 || q->n_ave[1] < 1 || q->n_ave[2] < 1) return -1;
+    if (!q->grid_pts) return -1;
+    if (tetra && (!q->grid_nodes || !q->grid_elements || q->grid_nnod < 4)) return -1;
+    if (!tetra && (!q->grid_abc || q->n_ave[0] < 1 || q->n_ave[1] < 1 || q->n_ave[2] < 1)) return -1;
     P.g.nx = q->nx; P.g.ny = q->ny; P.g.nz = q->nz; P.g.n = n;
     P.g.pts.assign(q->grid_pts, q->grid_pts + (size_t)n * 3);
     if (loaded_exchange(q, n, P.Aex)) return -1;
@@ -1263,8 +1298,13 @@ int build_problem(const or_problem* q, Problem& P) {
   size_t nn = (size_t)n * n;
   P.demag_fp64 = q->demag_fp64 != 0;
   if (P.demag_fp64 || !q->K_in) {
-    if (unstructured) assemble_unstructured(n, q->grid_pts, q->grid_abc, q->n_ave, P.Kd);
-    else assemble_uniform(P.g, P.Kd);
+    if (tetra) {
+      if (assemble_tetra(n, q->grid_pts, q->grid_nnod, q->grid_nodes, q->grid_elements, P.Kd)) return -1;
+    } else if (unstructured) {
+      assemble_unstructured(n, q->grid_pts, q->grid_abc, q->n_ave, P.Kd);
+    } else {
+      assemble_uniform(P.g, P.Kd);
+    }
     if (!P.demag_fp64) {
       P.K.resize(6 * nn);
       for (size_t k = 0; k < 6 * nn; ++k) P.K[k] = (float)P.Kd[k];
@@ -1707,15 +1747,31 @@ struct PrismTile {
   double abc[3], off[3], M[3], ops[3];
   int sym;
   Mat3 R, Rinv;
+  int type = 2;          // 2 prism, 5 tetrahedron
+  double vert[4][3];     // tetrahedron vertices in the tile's frame
 };
+
+int tetra_n(const double vert[4][3], const double r[3], Mat3 N);
+
+// tensor of the tile's own shape at the rotated relative position r
+void shape_tensor(const PrismTile& T, const double r[3], Mat3 N) {
+  if (T.type == 5) {
+    if (tetra_n(T.vert, r, N))
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) N[a][b] = std::nan("");
+    return;
+  }
+  double k[6];
+  prism_k6(T.abc[0], T.abc[1], T.abc[2], r[0], r[1], r[2], k);
+  k6_to_mat(k, N);
+}
 
 // getFieldFromTile (:685-730) and getFieldFromTile_symm (:744-822): tensor at p
 void tile_tensor(const PrismTile& T, const double p[3], Mat3 N) {
   if (!T.sym) {
-    double d[3] = {p[0] - T.off[0], p[1] - T.off[1], p[2] - T.off[2]}, r[3], k[6];
+    double d[3] = {p[0] - T.off[0], p[1] - T.off[1], p[2] - T.off[2]}, r[3];
     for (int a = 0; a < 3; ++a) r[a] = T.R[a][0] * d[0] + T.R[a][1] * d[1] + T.R[a][2] * d[2];
-    prism_k6(T.abc[0], T.abc[1], T.abc[2], r[0], r[1], r[2], k);
-    k6_to_mat(k, N);
+    shape_tensor(T, r, N);
     return;
   }
   double c[8], s[8][3];
@@ -1723,16 +1779,175 @@ void tile_tensor(const PrismTile& T, const double p[3], Mat3 N) {
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b) N[a][b] = 0.0;
   for (int j = 0; j < 8; ++j) {
-    double d[3], r[3], k[6];
+    double d[3], r[3];
     Mat3 Nj;
     for (int a = 0; a < 3; ++a) d[a] = s[j][a] * p[a] - T.off[a];
     for (int a = 0; a < 3; ++a) r[a] = T.R[a][0] * d[0] + T.R[a][1] * d[1] + T.R[a][2] * d[2];
-    prism_k6(T.abc[0], T.abc[1], T.abc[2], r[0], r[1], r[2], k);
-    k6_to_mat(k, Nj);
+    shape_tensor(T, r, Nj);
     // N = symm_H * (N * symm_M): row a scaled by s[j][a], all scaled by c[j]
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b) N[a][b] = s[j][a] * (Nj[a][b] * c[j]) + N[a][b];
   }
+}
+
+// ---- tetrahedron (tileType 5): sum over the four faces of the right-triangle
+// face tensor (getN_tensor_tetrahedron, TileNComponents.f90:944-961;
+// getN_Triangle and the Nxz / Nyz / Nzz integrals, TileTriangle.f90:26-286).
+// FP64 throughout: the reference builds with default real = 8 bytes
+// (-real-size 64 / -fdefault-real-8, python/src/magtense/lib/Makefile).
+const double kTriPi = 3.14159265359;  // TileTriangle.f90:6, as written
+
+double F_Nxz(const double r[3], double yp, double l, double h) {
+  return h / std::sqrt(h * h + l * l) *
+         std::atanh((l * l - l * r[0] + h * r[1] - h * yp * (1 + l * l / (h * h))) /
+                    (std::sqrt(h * h + l * l) *
+                     std::sqrt(l * l - 2 * l * r[0] + r[0] * r[0] + r[1] * r[1] - 2 * yp * (l * l - l * r[0] + h * r[1]) / h +
+                               yp * yp * (1. + l * l / (h * h)) + r[2] * r[2])));
+}
+double G_Nxz(const double r[3], double yp) {
+  double g = std::atanh((r[1] - yp) / (std::sqrt(r[0] * r[0] + (r[1] - yp) * (r[1] - yp) + r[2] * r[2])));
+  return std::isfinite(g) ? g : 0.0;
+}
+double Nxz(const double r[3], double l, double h) {
+  return -1. / (4. * kTriPi) * (F_Nxz(r, h, l, h) - F_Nxz(r, 0., l, h) - (G_Nxz(r, h) - G_Nxz(r, 0.)));
+}
+double K_Nyz(const double r[3], double xp, double l, double h) {
+  return l / std::sqrt(h * h + l * l) *
+         std::atanh((h * h - h * r[1] + l * r[0] - l * xp * (1 + h * h / (l * l))) /
+                    (std::sqrt(h * h + l * l) *
+                     std::sqrt(h * h - 2 * h * r[1] + r[0] * r[0] + r[1] * r[1] - 2 * xp * (h * h + l * r[0] - h * r[1]) / l +
+                               xp * xp * (1 + h * h / (l * l)) + r[2] * r[2])));
+}
+double L_Nyz(const double r[3], double xp) {
+  double g = std::atanh((r[0] - xp) / (std::sqrt((r[0] - xp) * (r[0] - xp) + r[1] * r[1] + r[2] * r[2])));
+  return std::isfinite(g) ? g : 0.0;
+}
+double Nyz(const double r[3], double l, double h) {
+  return -1. / (4. * kTriPi) * (K_Nyz(r, l, l, h) - K_Nyz(r, 0., l, h) - (L_Nyz(r, l) - L_Nyz(r, 0.)));
+}
+double P_Nzz(const double r[3], double xp, double l, double h) {
+  return std::atan((r[0] * (h - r[1]) - xp * (h * (1 - r[0] / l) - r[1]) - h * (r[0] * r[0] + r[2] * r[2]) / l) /
+                   (r[2] * std::sqrt(h * h + r[0] * r[0] + xp * xp * (1 + h * h / (l * l)) - 2 * h * r[1] + r[1] * r[1] -
+                                     2 * xp * (h * h + l * r[0] - h * r[1]) / l + r[2] * r[2])));
+}
+double Q_Nzz(const double r[3], double xp) {
+  return -std::atan((r[0] - xp) * r[1] / (r[2] * std::sqrt((r[0] - xp) * (r[0] - xp) + r[1] * r[1] + r[2] * r[2])));
+}
+double Nzz(const double r[3], double l, double h) {
+  return -1. / (4. * kTriPi) * (P_Nzz(r, l, l, h) - P_Nzz(r, 0., l, h) - (Q_Nzz(r, l) - Q_Nzz(r, 0.)));
+}
+
+inline double norm3(const double a[3]) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+inline double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline void sub3(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[0] - b[0]; c[1] = a[1] - b[1]; c[2] = a[2] - b[2];
+}
+inline void cross3(const double a[3], const double b[3], double c[3]) {  // SpecialFunctions.f90:72-79
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// getN_Triangle (TileTriangle.f90:26-155): face v[0..2], v[3] behind the face;
+// returns -1 for the reference's collinear / coplanar stops.
+int triangle_n(const double vin[4][3], const double r[3], Mat3 N) {
+  const double numErr = 1e-25, threshold = 1e-20;
+  double v[4][3];
+  std::memcpy(v, vin, sizeof(v));
+  double t12[3], t23[3], t13[3];
+  sub3(v[0], v[1], t12); sub3(v[1], v[2], t23); sub3(v[0], v[2], t13);
+  if (std::fabs(norm3(t13) - (norm3(t12) + norm3(t23))) < numErr) return -1;
+  double ang[3], a[3], b[3];
+  sub3(v[0], v[1], a); sub3(v[0], v[2], b); ang[0] = std::acos(dot3(a, b) / (norm3(a) * norm3(b)));
+  sub3(v[1], v[0], a); sub3(v[1], v[2], b); ang[1] = std::acos(dot3(a, b) / (norm3(a) * norm3(b)));
+  sub3(v[2], v[1], a); sub3(v[2], v[0], b); ang[2] = std::acos(dot3(a, b) / (norm3(a) * norm3(b)));
+  // simple_sort (SpecialFunctions.f90:13-34): repeated MINLOC, first of equals
+  int ind[3];
+  bool mk[3] = {true, true, true};
+  double srt[3];
+  for (int i = 0; i < 3; ++i) {
+    int best = -1;
+    for (int k = 0; k < 3; ++k)
+      if (mk[k] && (best < 0 || ang[k] < ang[best])) best = k;
+    srt[i] = ang[best]; ind[i] = best; mk[best] = false;
+  }
+  double w[3][3];
+  for (int i = 0; i < 3; ++i) std::memcpy(w[i], v[ind[i]], sizeof(w[i]));
+  for (int i = 0; i < 3; ++i) std::memcpy(v[i], w[i], sizeof(w[i]));
+  double tmp[3];
+  std::memcpy(tmp, v[2], sizeof(tmp)); std::memcpy(v[2], v[1], sizeof(tmp)); std::memcpy(v[1], tmp, sizeof(tmp));
+  ang[0] = srt[0]; ang[1] = srt[2]; ang[2] = srt[1];
+  double e1[3], e2[3], e3[3], d32[3];
+  auto frame = [&]() {
+    sub3(v[0], v[2], e1);
+    double n1 = norm3(e1);
+    for (int k = 0; k < 3; ++k) e1[k] = e1[k] / n1;
+    sub3(v[1], v[2], d32);
+    cross3(e1, d32, e3);
+    double n3 = norm3(e3);
+    for (int k = 0; k < 3; ++k) e3[k] = e3[k] / n3;
+  };
+  frame();
+  double d41[3];
+  sub3(v[3], v[0], d41);
+  if (std::fabs(dot3(d41, e3)) <= numErr) return -1;
+  if (dot3(e3, d41) > 0.) {
+    std::memcpy(tmp, v[0], sizeof(tmp)); std::memcpy(v[0], v[2], sizeof(tmp)); std::memcpy(v[2], tmp, sizeof(tmp));
+    double t = ang[0]; ang[0] = ang[2]; ang[2] = t;
+    frame();
+  }
+  cross3(e3, e1, e2);
+  // P = [e1 e2 e3] (columns), Pinv = P^T
+  Mat3 P = {{e1[0], e2[0], e3[0]}, {e1[1], e2[1], e3[1]}, {e1[2], e2[2], e3[2]}};
+  sub3(v[1], v[2], d32);
+  const double cd = std::cos(ang[2]) * norm3(d32);
+  double D[3];
+  for (int k = 0; k < 3; ++k) D[k] = cd * e1[k] + v[2][k];
+  double vp[3][3], rt[3], q[3];
+  for (int i = 0; i < 3; ++i) {
+    sub3(v[i], D, q);
+    for (int k = 0; k < 3; ++k) vp[i][k] = P[0][k] * q[0] + P[1][k] * q[1] + P[2][k] * q[2];
+  }
+  sub3(r, D, q);
+  for (int k = 0; k < 3; ++k) rt[k] = P[0][k] * q[0] + P[1][k] * q[1] + P[2][k] * q[2];
+  for (int k = 0; k < 3; ++k)
+    if (std::fabs(rt[k]) < threshold) rt[k] = rt[k] < 0 ? -threshold : threshold;
+  // local tensor: only column 3 (vp(1,1) = vp[0][0], vp(2,2) = vp[1][1], vp(1,3) = vp[2][0])
+  const double n13 = Nxz(rt, vp[0][0], vp[1][1]) - Nxz(rt, vp[2][0], vp[1][1]);
+  const double n23 = Nyz(rt, vp[0][0], vp[1][1]) - Nyz(rt, vp[2][0], vp[1][1]);
+  const double n33 = Nzz(rt, vp[0][0], vp[1][1]) - Nzz(rt, vp[2][0], vp[1][1]);
+  // N = (P N_loc) P^T, N_loc(:,3) = (n13, n23, n33)
+  Mat3 PN = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int a2 = 0; a2 < 3; ++a2) PN[a2][2] = P[a2][0] * n13 + P[a2][1] * n23 + P[a2][2] * n33;
+  for (int a2 = 0; a2 < 3; ++a2)
+    for (int b2 = 0; b2 < 3; ++b2) N[a2][b2] = PN[a2][0] * P[b2][0] + PN[a2][1] * P[b2][1] + PN[a2][2] * P[b2][2];
+  return 0;
+}
+
+// getN_tensor_tetrahedron: faces cshift(vert, i-1, 2), i = 1..4
+int tetra_n(const double vert[4][3], const double r[3], Mat3 N) {
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) N[a][b] = 0.0;
+  for (int i = 0; i < 4; ++i) {
+    double v[4][3];
+    for (int k = 0; k < 4; ++k) std::memcpy(v[k], vert[(k + i) % 4], sizeof(v[k]));
+    Mat3 Nt;
+    if (triangle_n(v, r, Nt)) return -1;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) N[a][b] = N[a][b] + Nt[a][b];
+  }
+  return 0;
+}
+
+int tetra_tensor_at(const double* vert12, const double r[3], double N9[9]) {
+  double v[4][3];
+  for (int k = 0; k < 4; ++k)
+    for (int c = 0; c < 3; ++c) v[k][c] = vert12[3 * k + c];
+  Mat3 M;
+  if (tetra_n(v, r, M)) return -1;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) N9[3 * a + b] = M[a][b];
+  return 0;
 }
 
 // H contribution of one tile at p given its tensor (getDotProd + rotations)
@@ -1744,11 +1959,17 @@ void tile_field(const PrismTile& T, const Mat3 N, double h[3]) {
 }
 
 int load_prism_tiles(int nt, const double* size, const double* offset, const double* rot, const double* M,
-                     const int* tile_type, const int* use_sym, const double* sym_ops, std::vector<PrismTile>& T) {
+                     const int* tile_type, const int* use_sym, const double* sym_ops, std::vector<PrismTile>& T,
+                     const double* vertices = nullptr) {
   T.resize(nt);
   for (int i = 0; i < nt; ++i) {
-    if (tile_type && tile_type[i] != 2) return -2;
+    const int type = tile_type ? tile_type[i] : 2;
+    if (type != 2 && !(type == 5 && vertices)) return -2;
     PrismTile& t = T[i];
+    t.type = type;
+    if (type == 5)  // (nt,3,4) Fortran order
+      for (int k = 0; k < 4; ++k)
+        for (int c = 0; c < 3; ++c) t.vert[k][c] = vertices[i + (size_t)nt * (c + 3 * (size_t)k)];
     double ang[3];
     for (int k = 0; k < 3; ++k) {
       t.abc[k] = size[i + (size_t)nt * k];
@@ -1776,10 +1997,19 @@ void or_rotation_matrices(const double* ang, double* R, double* Rinv) {
 }
 
 // getNFromTiles for prism tiles; returns -2 if a tile is not a prism
+int or_tiles_tensor_v(int nt, const double* size, const double* offset, const double* rot, const int* tile_type,
+                      const int* use_sym, const double* sym_ops, const double* vertices, int np, const double* pts,
+                      double* N);
 int or_tiles_tensor(int nt, const double* size, const double* offset, const double* rot, const int* tile_type,
                     const int* use_sym, const double* sym_ops, int np, const double* pts, double* N) {
+  return or_tiles_tensor_v(nt, size, offset, rot, tile_type, use_sym, sym_ops, nullptr, np, pts, N);
+}
+// the same with tetrahedron tiles (tile_type 5, vertices (nt,3,4) Fortran order)
+int or_tiles_tensor_v(int nt, const double* size, const double* offset, const double* rot, const int* tile_type,
+                      const int* use_sym, const double* sym_ops, const double* vertices, int np, const double* pts,
+                      double* N) {
   std::vector<PrismTile> T;
-  int rc = load_prism_tiles(nt, size, offset, rot, nullptr, tile_type, use_sym, sym_ops, T);
+  int rc = load_prism_tiles(nt, size, offset, rot, nullptr, tile_type, use_sym, sym_ops, T, vertices);
   if (rc) return rc;
 #pragma omp parallel for schedule(dynamic, 16)
   for (int p = 0; p < np; ++p) {
@@ -1796,11 +2026,19 @@ int or_tiles_tensor(int nt, const double* size, const double* offset, const doub
 
 // getHFromTiles for prism tiles (H summed over tiles in tile order); N_in,
 // when given, replaces the tensor evaluation (useStoredN)
+int or_tiles_field_v(int nt, const double* size, const double* offset, const double* rot, const double* M,
+                     const int* tile_type, const int* use_sym, const double* sym_ops, const double* vertices, int np,
+                     const double* pts, const double* N_in, double* H);
 int or_tiles_field(int nt, const double* size, const double* offset, const double* rot, const double* M,
                    const int* tile_type, const int* use_sym, const double* sym_ops, int np, const double* pts,
                    const double* N_in, double* H) {
+  return or_tiles_field_v(nt, size, offset, rot, M, tile_type, use_sym, sym_ops, nullptr, np, pts, N_in, H);
+}
+int or_tiles_field_v(int nt, const double* size, const double* offset, const double* rot, const double* M,
+                     const int* tile_type, const int* use_sym, const double* sym_ops, const double* vertices, int np,
+                     const double* pts, const double* N_in, double* H) {
   std::vector<PrismTile> T;
-  int rc = load_prism_tiles(nt, size, offset, rot, M, tile_type, use_sym, sym_ops, T);
+  int rc = load_prism_tiles(nt, size, offset, rot, M, tile_type, use_sym, sym_ops, T, vertices);
   if (rc) return rc;
 #pragma omp parallel for schedule(dynamic, 16)
   for (int p = 0; p < np; ++p) {
@@ -1842,6 +2080,21 @@ int or_tensor_unstructured(int n, const double* pts, const double* abc, const in
   std::vector<double> Kd;
   assemble_unstructured(n, pts, abc, nave, Kd);
   std::memcpy(K6, Kd.data(), sizeof(double) * Kd.size());
+  return 0;
+}
+}  // extern "C"
+
+extern "C" {
+// Tetrahedron tensor at r (vertices vert[k*3 + c], k = 0..3), N row-major 3x3;
+// -1 for a degenerate face (the reference stops).
+int or_tetra_n(const double* vert, const double* r, double* N) {
+  double v[4][3];
+  for (int k = 0; k < 4; ++k)
+    for (int c = 0; c < 3; ++c) v[k][c] = vert[k * 3 + c];
+  Mat3 M;
+  if (tetra_n(v, r, M)) return -1;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) N[3 * a + b] = M[a][b];
   return 0;
 }
 }  // extern "C"

@@ -44,6 +44,7 @@ class OrProblem(C.Structure):
         ("n_ave", C.c_int * 3),
         ("exch_nnz", C.c_int),
         ("exch_rows_start", _ip), ("exch_rows_end", _ip), ("exch_cols", _ip), ("exch_vals", _dp),
+        ("grid_nnod", C.c_int), ("grid_nodes", _dp), ("grid_elements", _ip),
     ]
 
 
@@ -84,6 +85,8 @@ def lib():
         _lib.or_threshold_fraction.restype = C.c_float
         _lib.or_tiles_tensor.argtypes = [C.c_int, _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, _dp, _dp]
         _lib.or_tiles_field.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp]
+        _lib.or_tiles_tensor_v.argtypes = [C.c_int, _dp, _dp, _dp, _ip, _ip, _dp, _dp, C.c_int, _dp, _dp]
+        _lib.or_tiles_field_v.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _dp, _dp, C.c_int, _dp, _dp, _dp]
     return _lib
 
 
@@ -140,6 +143,16 @@ def tensor_unstructured(pts, abc, n_ave=(1, 1, 1)):
     return K
 
 
+def tetra_n(vert, r):
+    """Tetrahedron tensor (MagTense sign) at r; vert (4, 3) vertices; (3, 3)."""
+    v = _c64(np.asarray(vert, dtype=np.float64).reshape(4, 3))
+    rr = _c64(np.asarray(r, dtype=np.float64).reshape(3))
+    N = np.zeros((3, 3))
+    if lib().or_tetra_n(_d(v), _d(rr), _d(N)):
+        raise ValueError("degenerate tetrahedron face")
+    return N
+
+
 def exchange_csr(res, L):
     nnz = lib().or_exchange_csr(int(res[0]), int(res[1]), int(res[2]), float(L[0]), float(L[1]), float(L[2]),
                                 None, None, None)
@@ -160,11 +173,13 @@ class Problem:
     def __init__(self, res, L, Ms=8e5, A0=1.3e-11, K0=0.0, gamma=0.0, alpha=0.02, u_ea=None, solver=2,
                  hext=None, alphat=None, t=None, m0=None, tol=1e-4, thres=1e-6, t_conv=None, conv_tol=1e-4,
                  demag_fp64=False, K_in=None, return_fields=True, demag_approx=1, dem_thres=0.0,
-                 grid_pts=None, grid_abc=None, n_ave=(1, 1, 1), exch=None):
+                 grid_pts=None, grid_abc=None, n_ave=(1, 1, 1), exch=None, grid_nodes=None, grid_elements=None):
         """grid_pts / grid_abc ((n, 3) centres and full edges) select the
         unstructured-prism grid; exch = (rows_start, rows_end, cols, vals) is
         its exchange operator as a 1-based CSR (rows_start / rows_end of
-        mkl_sparse_d_create_csr)."""
+        mkl_sparse_d_create_csr).  grid_nodes ((nnod, 3)) with grid_elements
+        ((n, 4), 1-based node indices) select the tetrahedral grid instead of
+        grid_abc; grid_pts are then the evaluation points."""
         n = int(np.prod(res))
         self.n = n
         self.res = list(res)
@@ -198,7 +213,7 @@ class Problem:
         q.grid_type = 1
         if grid_pts is not None:
             self._gp = _c64(np.asarray(grid_pts, dtype=np.float64).reshape(n, 3))
-            self._ga = _c64(np.asarray(grid_abc, dtype=np.float64).reshape(n, 3))
+            self._ga = _c64(np.zeros((n, 3)) if grid_abc is None else np.asarray(grid_abc, dtype=np.float64).reshape(n, 3))
             rs, re_, cols, vals = exch
             self._ex = [np.ascontiguousarray(rs, dtype=np.int32), np.ascontiguousarray(re_, dtype=np.int32),
                         np.ascontiguousarray(cols, dtype=np.int32), _c64(vals)]
@@ -210,6 +225,13 @@ class Problem:
             q.exch_rows_end = self._ex[1].ctypes.data_as(_ip)
             q.exch_cols = self._ex[2].ctypes.data_as(_ip)
             q.exch_vals = _d(self._ex[3])
+            if grid_nodes is not None:
+                self._gn = _c64(np.asarray(grid_nodes, dtype=np.float64).reshape(-1, 3))
+                self._ge = np.ascontiguousarray(np.asarray(grid_elements).reshape(n, 4), dtype=np.int32)
+                q.grid_type = 2
+                q.grid_nnod = self._gn.shape[0]
+                q.grid_nodes = _d(self._gn)
+                q.grid_elements = self._ge.ctypes.data_as(_ip)
         self.q = q
 
     def dmdt(self, t, m):
@@ -316,30 +338,40 @@ def _tile_args(size, offset, rot, M, tile_type, use_sym, sym_ops):
         f3(sym_ops, 1.0)
 
 
-def tiles_tensor(size, offset, pts, rot=None, tile_type=None, use_sym=None, sym_ops=None):
-    """getNFromTiles for prism tiles: N(n_tiles, n_pts, 3, 3), Fortran order."""
+def _vertices(nt, vertices):
+    if vertices is None:
+        return None
+    return np.asfortranarray(np.asarray(vertices, dtype=np.float64).reshape(nt, 3, 4))
+
+
+def tiles_tensor(size, offset, pts, rot=None, tile_type=None, use_sym=None, sym_ops=None, vertices=None):
+    """getNFromTiles for prism and tetrahedron tiles: N(n_tiles, n_pts, 3, 3),
+    Fortran order; vertices (n_tiles, 3, 4) for tile_type 5."""
     nt, sz, off, rt, _, tt, us, so = _tile_args(size, offset, rot, None, tile_type, use_sym, sym_ops)
+    vv = _vertices(nt, vertices)
     P = np.asfortranarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
     npt = P.shape[0]
     N = np.zeros((nt, npt, 3, 3), order="F")
-    rc = lib().or_tiles_tensor(nt, _d(sz), _d(off), _d(rt), tt.ctypes.data_as(_ip), us.ctypes.data_as(_ip),
-                               _d(so), npt, _d(P), _d(N))
+    rc = lib().or_tiles_tensor_v(nt, _d(sz), _d(off), _d(rt), tt.ctypes.data_as(_ip), us.ctypes.data_as(_ip),
+                                 _d(so), None if vv is None else _d(vv), npt, _d(P), _d(N))
     if rc:
-        raise ValueError("only prism tiles (tile_type 2) are restated")
+        raise ValueError("only prism (2) and tetrahedron (5, with vertices) tiles are restated")
     return N
 
 
-def tiles_field(size, offset, M, pts, rot=None, tile_type=None, use_sym=None, sym_ops=None, N=None):
-    """getHFromTiles for prism tiles: H(n_pts, 3), Fortran order."""
+def tiles_field(size, offset, M, pts, rot=None, tile_type=None, use_sym=None, sym_ops=None, N=None, vertices=None):
+    """getHFromTiles for prism and tetrahedron tiles: H(n_pts, 3), Fortran order."""
     nt, sz, off, rt, mm, tt, us, so = _tile_args(size, offset, rot, M, tile_type, use_sym, sym_ops)
+    vv = _vertices(nt, vertices)
     P = np.asfortranarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
     npt = P.shape[0]
     H = np.zeros((npt, 3), order="F")
     Nin = None if N is None else np.asfortranarray(np.asarray(N, dtype=np.float64))
-    rc = lib().or_tiles_field(nt, _d(sz), _d(off), _d(rt), _d(mm), tt.ctypes.data_as(_ip), us.ctypes.data_as(_ip),
-                              _d(so), npt, _d(P), None if Nin is None else _d(Nin), _d(H))
+    rc = lib().or_tiles_field_v(nt, _d(sz), _d(off), _d(rt), _d(mm), tt.ctypes.data_as(_ip), us.ctypes.data_as(_ip),
+                                _d(so), None if vv is None else _d(vv), npt, _d(P), None if Nin is None else _d(Nin),
+                                _d(H))
     if rc:
-        raise ValueError("only prism tiles (tile_type 2) are restated")
+        raise ValueError("only prism (2) and tetrahedron (5, with vertices) tiles are restated")
     return H
 
 

@@ -105,7 +105,8 @@ def test_unstructured_prisms_marshalling(oracle):
     """grid_type='unstructuredPrisms': set_exchange_matrix writes the 1-based CSR
     the Fortran loader passes to mkl_sparse_d_create_csr (rows_start / rows_end,
     MagTenseMicroMagPyIO.f90:163-180); exch_val keeps float values; the points
-    returned are grid_pts; tetrahedra raise before any GPU call."""
+    returned are grid_pts; tetrahedra need their nodes and elements, checked
+    before any GPU call."""
     import scipy.sparse as sp
 
     res, L = [6, 3, 1], [30e-9, 9e-9, 3e-9]
@@ -130,5 +131,11 @@ def test_unstructured_prisms_marshalling(oracle):
     p.N_ave = np.array([0, 1, 1], dtype=np.int32)
     with pytest.raises(ValueError):
         p._check_supported()
-    with pytest.raises(NotImplementedError):
-        MicromagProblem([4, 1, 1], grid_type="tetrahedron")._check_supported()
+    # tetrahedra: nodes and elements are required
+    pt = MicromagProblem([4, 1, 1], grid_type="tetrahedron", grid_nnod=5)
+    pt.set_exchange_matrix(sp.identity(4) * 0.0)
+    with pytest.raises(ValueError):
+        pt._check_supported()
+    pt.grid_nod[:] = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1]]) * 1e-9
+    pt.grid_ele = np.array([[1, 2, 3, 4], [2, 3, 4, 5], [1, 2, 3, 5], [1, 3, 4, 5]], dtype=np.float64).T
+    pt._check_supported()
