@@ -462,12 +462,16 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
 // loads are ordinary vector loads issued in chunk order, so the compiler's
 // in-order vmcnt wait for chunk j lets chunks j+1 .. j+D-1 stay in flight.
 // Row tiles come from n (receiver rows), k-chunks from np (padded sources).
-template <int D>
+// TOEP: Kt is the translation-invariant fragment table of a gnx x gny x gnz
+// uniform grid; each wave looks up its tile's fragment set from the cell
+// offset and the k-chunks are visited in offset order (as in the LDS kernel),
+// with ordinary loads so the table stays in L2.
+template <int D, bool TOEP>
 __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __restrict__ Kt,
                                                                  const uint4* __restrict__ Mp, int64_t np,
                                                                  int ksplit, float* __restrict__ out, int n,
                                                                  int64_t col_stride, int64_t split_stride,
-                                                                 int ncols) {
+                                                                 int ncols, int gnx, int gny, int gnz) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   __shared__ u32x4 bs[2][12][64];
   const int lane = threadIdx.x & 63;
@@ -498,9 +502,23 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
   const int64_t I = (int64_t)group * 4 + wave;
   const bool tile_ok = I < n_row_tiles;
   // a wave past the last tile loads the group's first tile (kept uniform, never consumed)
-  const u32x4* A = reinterpret_cast<const u32x4*>(Kt) + (tile_ok ? I : (int64_t)group * 4) * nJ * 12 * 64 + lane;
+  const int64_t Il = tile_ok ? I : (int64_t)group * 4;
+  const u32x4* A = reinterpret_cast<const u32x4*>(Kt) + (TOEP ? 0 : Il * nJ * 12 * 64) + lane;
   const u32x4* B = reinterpret_cast<const u32x4*>(Mp) + threadIdx.x;
   const bool two_halves = ncols > 32;
+  // translation-invariant lookup: this wave's tile (x0 tile, iy, iz) and the
+  // group's first tile row (wy, wz), which sets the shared chunk order
+  const int cpr = gnx / 16, tpr = gnx / 32, U = gnx / 8 - 2;
+  int tx0 = 0, tiy = 0, tiz = 0, wy = 0, wz = 0;
+  if (TOEP) {
+    const int irow = (int)(Il / tpr);
+    tx0 = (int)(Il - (int64_t)irow * tpr);
+    tiy = irow % gny;
+    tiz = irow / gny;
+    const int wrow = (int)((int64_t)group * 4 / tpr);
+    wy = wrow % gny;
+    wz = wrow / gny;
+  }
 
   // native vector element types keep both rings in registers
   u32x4 ar[D][12];
@@ -513,13 +531,24 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[h][a][r] = 0.0f;
 
-#define MT_REG_ISSUE(v, slot)                                                      \
-  do {                                                                              \
-    const int J_ = jb + ((v) < nchunks ? (v) : nchunks - 1);                        \
-    const u32x4* a_ = A + (int64_t)J_ * 12 * 64;                                    \
-    _Pragma("unroll") for (int f = 0; f < 12; ++f) ar[slot][f] = __builtin_nontemporal_load(a_ + f * 64); \
-    const u32x4* b_ = B + (int64_t)J_ * 12 * 64;                                    \
-    _Pragma("unroll") for (int q = 0; q < 3; ++q) br[slot][q] = b_[q * 256];        \
+#define MT_REG_ISSUE(v, slot)                                                                            \
+  do {                                                                                                    \
+    int J_ = jb + ((v) < nchunks ? (v) : nchunks - 1);                                                    \
+    const u32x4* a_;                                                                                      \
+    if (TOEP) {                                                                                           \
+      const int x1c_ = J_ % cpr, rs_ = J_ / cpr;                                                          \
+      const int jy_ = (wy - rs_ % gny + gny) % gny, jz_ = (wz - rs_ / gny + gnz) % gnz;                   \
+      J_ = (jz_ * gny + jy_) * cpr + x1c_;                                                                \
+      const int64_t tset_ = ((int64_t)(tiz - jz_ + gnz - 1) * (2 * gny - 1) + (tiy - jy_ + gny - 1)) * U +  \
+                            (2 * tx0 - x1c_ + cpr - 1);                                                   \
+      a_ = A + tset_ * 12 * 64;                                                                           \
+      _Pragma("unroll") for (int f = 0; f < 12; ++f) ar[slot][f] = a_[f * 64];                            \
+    } else {                                                                                              \
+      a_ = A + (int64_t)J_ * 12 * 64;                                                                     \
+      _Pragma("unroll") for (int f = 0; f < 12; ++f) ar[slot][f] = __builtin_nontemporal_load(a_ + f * 64); \
+    }                                                                                                     \
+    const u32x4* b_ = B + (int64_t)J_ * 12 * 64;                                                          \
+    _Pragma("unroll") for (int q = 0; q < 3; ++q) br[slot][q] = b_[q * 256];                              \
   } while (0)
 
   // chunk j -> LDS slot j & 1, barrier, MFMAs from register slot `slot`
@@ -587,16 +616,17 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
   }
 }
 
-template <int D>
+template <int D, bool TOEP = false>
 int launch_reg(const uint8_t* K, const uint8_t* Mpack, int n, int nrows, float* out, int64_t col_stride,
-               int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+               int64_t split_stride, int ncols, int ksplit, hipStream_t stream, int gnx = 0, int gny = 1,
+               int gnz = 1) {
   const int64_t np = round_up(n, kTileM);
   const int groups = (int)((round_up(nrows, kTileM) / kTileM + 3) / 4);
   int64_t blocks = (int64_t)groups * ksplit;
   if (ksplit % 8 != 0 && 8 % ksplit == 0) blocks = round_up(blocks, 8);
-  hipLaunchKernelGGL((gemm_bf16x2_reg_kernel<D>), dim3((unsigned)blocks), dim3(256), 0, stream,
+  hipLaunchKernelGGL((gemm_bf16x2_reg_kernel<D, TOEP>), dim3((unsigned)blocks), dim3(256), 0, stream,
                      reinterpret_cast<const uint4*>(K), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out, nrows,
-                     col_stride, split_stride, ncols);
+                     col_stride, split_stride, ncols, gnx, gny, gnz);
   MT_HIP(hipGetLastError());
   return kOk;
 }
@@ -667,7 +697,8 @@ int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, f
 // kernel with 4 chunks in flight, 7 / 8 = the same with 5 / 3 chunks, 6 =
 // two-tile kernel with a four-stage LDS DMA ring, 1 = register-staged
 // kernel, 5 = four-tile single-stage kernel.  Table: 2 = two-tile four-stage
-// kernel, 3 = four-tile two-stage kernel.
+// kernel, 3 = four-tile two-stage kernel, 4 / 11 = register-ring kernel with
+// 4 / 6 chunks.
 int gemm_variant() {
   static int v = [] {
     const char* e = getenv("MT_GEMM_VARIANT");
@@ -1109,6 +1140,10 @@ int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, con
     return kErrArg;
   }
   const int n = nx * ny * nz;
+  if (gemm_variant() == 4)
+    return launch_reg<4, true>(Ktab, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream, nx, ny, nz);
+  if (gemm_variant() == 11)
+    return launch_reg<6, true>(Ktab, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream, nx, ny, nz);
   if (gemm_variant() != 2 && gemm_variant() != 3) {
     // default: single-stage four-tile kernel, two workgroups per CU (measured at
     // N = 32768, 64 states: 3.19 ms, against 3.31 ms for the two-stage four-tile
