@@ -152,11 +152,13 @@ int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* 
   if (rc) return rc;
   MT_CHECK(bytes >= (1 << 20) && iters >= 1, "need bytes >= 1 MiB and iters >= 1");
   const int64_t n16 = bytes / 16;
-  uint4 *a = nullptr, *b = nullptr;
-  uint32_t* sink = nullptr;
-  MT_HIP(hipMalloc(&a, n16 * 16));
-  MT_HIP(hipMalloc(&sink, 64));
-  if (copy_gbps) MT_HIP(hipMalloc(&b, n16 * 16));
+  DevArr<uint4> A, Bc;
+  DevArr<uint32_t> S;
+  MT_HIP(A.alloc(n16));
+  MT_HIP(S.alloc(16));
+  if (copy_gbps) MT_HIP(Bc.alloc(n16));
+  uint4 *a = A.p, *b = Bc.p;
+  uint32_t* sink = S.p;
   MT_HIP(hipMemset(a, 0x11, n16 * 16));
   hipEvent_t e0, e1;
   MT_HIP(hipEventCreate(&e0));
@@ -183,9 +185,6 @@ int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* 
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(sink);
-  if (b) (void)hipFree(b);
   return kOk;
 }
 
@@ -245,11 +244,13 @@ int apply_bf16x2(const void* K, int n, int nrows, int nx, int ny, int nz, const 
   const int64_t n3 = (int64_t)3 * n;
   const int64_t r3 = (int64_t)3 * nrows;
   int ks = nx > 0 ? gemm_ksplit_for(n, 2) : gemm_ksplit_dense(n, nrows);
-  uint8_t* packed = nullptr;
-  float* part = nullptr;
-  MT_HIP(hipMalloc(&packed, (size_t)packed_m_bytes(n)));
+  DevArr<uint8_t> P8;
+  DevArr<float> PF;
+  MT_HIP(P8.alloc((size_t)packed_m_bytes(n)));
   int cols = nstates < kMaxCols ? nstates : kMaxCols;
-  MT_HIP(hipMalloc(&part, (size_t)ks * cols * r3 * sizeof(float)));
+  MT_HIP(PF.alloc((size_t)ks * cols * r3));
+  uint8_t* packed = P8.p;
+  float* part = PF.p;
   for (int s0 = 0; s0 < nstates && rc == kOk; s0 += kMaxCols) {
     int S = nstates - s0 < kMaxCols ? nstates - s0 : kMaxCols;
     MT_HIP(hipMemsetAsync(packed, 0, (size_t)packed_m_bytes(n), nullptr));
@@ -272,8 +273,6 @@ int apply_bf16x2(const void* K, int n, int nrows, int nx, int ny, int nz, const 
     MT_HIP(hipGetLastError());
   }
   MT_HIP(hipStreamSynchronize(nullptr));
-  MT_HIP(hipFree(packed));
-  MT_HIP(hipFree(part));
   return rc;
 }
 }  // namespace
@@ -338,37 +337,35 @@ int mt_ll_rhs(int nx, int ny, int nz, double Lx, double Ly, double Lz, const dou
       A[0] = ux * ux; A[1] = ux * uy; A[2] = ux * uz; A[3] = uy * uy; A[4] = uy * uz; A[5] = uz * uz;
     }
   }
-  double *djf = nullptr, *dkf = nullptr, *dan = nullptr;
-  float *dms = nullptr, *mf = nullptr, *hb = nullptr;
-  LlgDesc* dd = nullptr;
-  MT_HIP(hipMalloc(&djf, n * sizeof(double)));
-  MT_HIP(hipMalloc(&dkf, n * sizeof(double)));
-  MT_HIP(hipMalloc(&dms, n * sizeof(float)));
-  MT_HIP(hipMalloc(&mf, nstates * n3 * sizeof(float)));
-  MT_HIP(hipMalloc(&hb, nstates * n3 * sizeof(float)));
-  MT_HIP(hipMalloc(&dd, nstates * sizeof(LlgDesc)));
+  DevArr<double> Djf, Dkf, Dan, Scr;
+  DevArr<float> Dms, Mf, Hb;
+  DevArr<LlgDesc> Dd;
+  DevArr<VecDesc> Dvd;
+  MT_HIP(Djf.alloc(n));
+  MT_HIP(Dkf.alloc(n));
+  MT_HIP(Dms.alloc(n));
+  MT_HIP(Mf.alloc((size_t)nstates * n3));
+  MT_HIP(Hb.alloc((size_t)nstates * n3));
+  MT_HIP(Dd.alloc(nstates));
+  MT_HIP(Scr.alloc((size_t)nstates * n3));
+  MT_HIP(Dvd.alloc(nstates));
+  double *djf = Djf.p, *dkf = Dkf.p, *dan = nullptr, *scratch = Scr.p;
+  float *dms = Dms.p, *mf = Mf.p, *hb = Hb.p;
+  LlgDesc* dd = Dd.p;
+  VecDesc* dvd = Dvd.p;
   MT_HIP(hipMemcpy(djf, jf.data(), n * sizeof(double), hipMemcpyHostToDevice));
   MT_HIP(hipMemcpy(dkf, kf.data(), n * sizeof(double), hipMemcpyHostToDevice));
   MT_HIP(hipMemcpy(dms, m32.data(), n * sizeof(float), hipMemcpyHostToDevice));
   if (!an.empty()) {
-    MT_HIP(hipMalloc(&dan, an.size() * sizeof(double)));
+    MT_HIP(Dan.alloc(an.size()));
+    dan = Dan.p;
     MT_HIP(hipMemcpy(dan, an.data(), an.size() * sizeof(double), hipMemcpyHostToDevice));
   }
-  // m (FP64) -> FP32 via the generic vector kernel's FP32 pack
+  // m (FP64) -> FP32 through the vector kernel's FP32 pack (a unit-coefficient copy into scratch)
   std::vector<VecDesc> vd(nstates);
   for (int s = 0; s < nstates; ++s) {
     VecDesc& d = vd[s];
     std::memset(&d, 0, sizeof(d));
-    d.dst = nullptr;
-  }
-  std::vector<float> host_tmp;  // not used: conversion done on device below
-  // Reuse the pack path: copy m into a scratch FP64 vector through a combine with pack.
-  double* scratch = nullptr;
-  VecDesc* dvd = nullptr;
-  MT_HIP(hipMalloc(&scratch, nstates * n3 * sizeof(double)));
-  MT_HIP(hipMalloc(&dvd, nstates * sizeof(VecDesc)));
-  for (int s = 0; s < nstates; ++s) {
-    VecDesc& d = vd[s];
     d.dst = scratch + s * n3;
     d.base = m + s * n3;
     d.bcoef = 1.0;
@@ -403,9 +400,6 @@ int mt_ll_rhs(int nx, int ny, int nz, double Lx, double Ly, double Lz, const dou
     rc = launch_llg(g, dd, nstates, nullptr);
   }
   MT_HIP(hipDeviceSynchronize());
-  (void)hipFree(djf); (void)hipFree(dkf); (void)hipFree(dms); (void)hipFree(mf); (void)hipFree(hb);
-  (void)hipFree(dd); (void)hipFree(scratch); (void)hipFree(dvd);
-  if (dan) (void)hipFree(dan);
   return rc;
 }
 
@@ -528,11 +522,6 @@ int tile_records(const mt_tiles* t, bool need_m, std::vector<double>& rec) {
   return kOk;
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-};
-
 }  // namespace
 
 extern "C" {
@@ -545,16 +534,16 @@ int mt_tiles_demag_tensor(const mt_tiles* tiles, int n_pts, const double* pts, d
   rc = tile_records(tiles, false, rec);
   if (rc) return rc;
   const int nt = tiles->n_tiles;
-  const size_t nbytes = (size_t)nt * n_pts * 9 * sizeof(double);
-  DevBuf drec, dpts, dN;
-  MT_HIP(hipMalloc(&drec.p, rec.size() * sizeof(double)));
-  MT_HIP(hipMalloc(&dpts.p, (size_t)n_pts * 3 * sizeof(double)));
-  MT_HIP(hipMalloc(&dN.p, nbytes));
+  const size_t nN = (size_t)nt * n_pts * 9;
+  DevArr<double> drec, dpts, dN;
+  MT_HIP(drec.alloc(rec.size()));
+  MT_HIP(dpts.alloc((size_t)n_pts * 3));
+  MT_HIP(dN.alloc(nN));
   MT_HIP(hipMemcpy(drec.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
   MT_HIP(hipMemcpy(dpts.p, pts, (size_t)n_pts * 3 * sizeof(double), hipMemcpyHostToDevice));
-  rc = launch_tiles_tensor((const double*)drec.p, nt, (const double*)dpts.p, n_pts, (double*)dN.p, nullptr);
+  rc = launch_tiles_tensor(drec.p, nt, dpts.p, n_pts, dN.p, nullptr);
   if (rc) return rc;
-  MT_HIP(hipMemcpy(N, dN.p, nbytes, hipMemcpyDeviceToHost));
+  MT_HIP(hipMemcpy(N, dN.p, nN * sizeof(double), hipMemcpyDeviceToHost));
   return kOk;
 }
 
@@ -566,19 +555,18 @@ int mt_tiles_h_field(const mt_tiles* tiles, int n_pts, const double* pts, const 
   rc = tile_records(tiles, true, rec);
   if (rc) return rc;
   const int nt = tiles->n_tiles;
-  DevBuf drec, dpts, dN, dH;
-  MT_HIP(hipMalloc(&drec.p, rec.size() * sizeof(double)));
-  MT_HIP(hipMalloc(&dpts.p, (size_t)n_pts * 3 * sizeof(double)));
-  MT_HIP(hipMalloc(&dH.p, (size_t)n_pts * 3 * sizeof(double)));
+  DevArr<double> drec, dpts, dN, dH;
+  MT_HIP(drec.alloc(rec.size()));
+  MT_HIP(dpts.alloc((size_t)n_pts * 3));
+  MT_HIP(dH.alloc((size_t)n_pts * 3));
   MT_HIP(hipMemcpy(drec.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
   MT_HIP(hipMemcpy(dpts.p, pts, (size_t)n_pts * 3 * sizeof(double), hipMemcpyHostToDevice));
   if (N) {
-    const size_t nbytes = (size_t)nt * n_pts * 9 * sizeof(double);
-    MT_HIP(hipMalloc(&dN.p, nbytes));
-    MT_HIP(hipMemcpy(dN.p, N, nbytes, hipMemcpyHostToDevice));
+    const size_t nN = (size_t)nt * n_pts * 9;
+    MT_HIP(dN.alloc(nN));
+    MT_HIP(hipMemcpy(dN.p, N, nN * sizeof(double), hipMemcpyHostToDevice));
   }
-  rc = launch_tiles_field((const double*)drec.p, nt, (const double*)dpts.p, n_pts, (const double*)dN.p,
-                          (double*)dH.p, nullptr);
+  rc = launch_tiles_field(drec.p, nt, dpts.p, n_pts, dN.p, dH.p, nullptr);
   if (rc) return rc;
   MT_HIP(hipMemcpy(H, dH.p, (size_t)n_pts * 3 * sizeof(double), hipMemcpyDeviceToHost));
   return kOk;
@@ -605,13 +593,23 @@ void icl_initDemagMatrices(const float* Kxx, const float* Kxy, const float* Kxz,
   float* T = nullptr;
   if (hipMalloc(&T, 6 * nn * sizeof(float)) != hipSuccess) { set_error("icl_initDemagMatrices: allocation failed"); return; }
   if (hipMalloc(&g_icl_K, 6 * nn * sizeof(float)) != hipSuccess) { (void)hipFree(T); set_error("icl_initDemagMatrices: allocation failed"); return; }
-  (void)hipMalloc(&g_icl_io, 6 * (size_t)N * sizeof(float));
+  bool ok = hipMalloc(&g_icl_io, 6 * (size_t)N * sizeof(float)) == hipSuccess;
   const float* planes[6] = {Kxx, Kxy, Kxz, Kyy, Kyz, Kzz};
-  for (int c = 0; c < 6; ++c) (void)hipMemcpy(T + c * nn, planes[c], nn * sizeof(float), hipMemcpyHostToDevice);
-  dim3 grid((N + 31) / 32, (N + 31) / 32, 6);
-  hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, nullptr, T, N, g_icl_K);
-  (void)hipDeviceSynchronize();
+  for (int c = 0; c < 6 && ok; ++c) ok = hipMemcpy(T + c * nn, planes[c], nn * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+  if (ok) {
+    dim3 grid((N + 31) / 32, (N + 31) / 32, 6);
+    hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, nullptr, T, N, g_icl_K);
+    ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+  }
   (void)hipFree(T);
+  if (!ok) {  // leave no half-initialised state: icl_MVMult_GetH then reports it
+    set_error("icl_initDemagMatrices: device allocation or copy failed");
+    (void)hipFree(g_icl_K); g_icl_K = nullptr;
+    if (g_icl_io) (void)hipFree(g_icl_io);
+    g_icl_io = nullptr;
+    g_icl_n = 0;
+    return;
+  }
   g_icl_n = N;
 }
 

@@ -33,6 +33,20 @@ enum Status {
     }                                                                                             \
   } while (0)
 
+// Device allocation owned by scope: every return path frees it (hipFree
+// waits for the device, so pending work on the buffer completes first).
+template <class T>
+struct DevArr {
+  T* p = nullptr;
+  DevArr() = default;
+  DevArr(const DevArr&) = delete;
+  DevArr& operator=(const DevArr&) = delete;
+  ~DevArr() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t count) { return hipMalloc(&p, count * sizeof(T)); }
+};
+
 #define MT_CHECK(cond, msg)          \
   do {                               \
     if (!(cond)) {                   \

@@ -1244,21 +1244,20 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   if (need32) MT_HIP(hipMalloc(&P.K32, 6 * nn * sizeof(float)));
   if (needsplit) MT_HIP(hipMalloc(&P.Ksplit, (size_t)split_tensor_bytes(n)));
   if (q->K_in) {
+    DevArr<float> tmp;
     float* Kd = P.K32;
-    if (!Kd) MT_HIP(hipMalloc(&Kd, 6 * nn * sizeof(float)));
+    if (!Kd) {
+      MT_HIP(tmp.alloc(6 * nn));
+      Kd = tmp.p;
+    }
     MT_HIP(hipMemcpyAsync(Kd, q->K_in, 6 * nn * sizeof(float), hipMemcpyHostToDevice, P.stream));
     if (needsplit) {
       int rc = launch_split(Kd, n, P.Ksplit, P.stream);
       if (rc) return rc;
     }
-    if (!P.K32) {
-      MT_HIP(hipStreamSynchronize(P.stream));
-      MT_HIP(hipFree(Kd));
-    }
+    MT_HIP(hipStreamSynchronize(P.stream));
   } else if (unstructured) {
     // averaged prism tensor / tetrahedron tensor in FP32, split afterwards for the MFMA product
-    double *dpts = nullptr, *dabc = nullptr;
-    int* delem = nullptr;
     std::vector<int> elem0;
     const size_t geo = tetra ? (size_t)q->grid_nnod * 3 : pts.size();
     if (tetra) {  // 1-based node indices -> 0-based, checked; degenerate elements refused
@@ -1278,40 +1277,40 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
         MT_CHECK(vol != 0.0, "grid_elements: degenerate tetrahedron (zero volume)");
       }
     }
+    DevArr<float> tmp;
+    DevArr<double> dpts, dabc;
+    DevArr<int> delem;
     float* Kd = P.K32;
-    if (!Kd) MT_HIP(hipMalloc(&Kd, 6 * nn * sizeof(float)));
-    MT_HIP(hipMalloc(&dpts, pts.size() * sizeof(double)));
-    MT_HIP(hipMalloc(&dabc, geo * sizeof(double)));
-    MT_HIP(hipMemcpyAsync(dpts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
-    MT_HIP(hipMemcpyAsync(dabc, tetra ? q->grid_nodes : q->grid_abc, geo * sizeof(double), hipMemcpyHostToDevice,
+    if (!Kd) {
+      MT_HIP(tmp.alloc(6 * nn));
+      Kd = tmp.p;
+    }
+    MT_HIP(dpts.alloc(pts.size()));
+    MT_HIP(dabc.alloc(geo));
+    MT_HIP(hipMemcpyAsync(dpts.p, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
+    MT_HIP(hipMemcpyAsync(dabc.p, tetra ? q->grid_nodes : q->grid_abc, geo * sizeof(double), hipMemcpyHostToDevice,
                           P.stream));
     int rc = kOk;
     if (tetra) {
-      MT_HIP(hipMalloc(&delem, elem0.size() * sizeof(int)));
-      MT_HIP(hipMemcpyAsync(delem, elem0.data(), elem0.size() * sizeof(int), hipMemcpyHostToDevice, P.stream));
-      rc = launch_assemble_tetra(dpts, dabc, delem, n, Kd, nullptr, P.stream);
+      MT_HIP(delem.alloc(elem0.size()));
+      MT_HIP(hipMemcpyAsync(delem.p, elem0.data(), elem0.size() * sizeof(int), hipMemcpyHostToDevice, P.stream));
+      rc = launch_assemble_tetra(dpts.p, dabc.p, delem.p, n, Kd, nullptr, P.stream);
     } else {
-      rc = launch_assemble_avg(dpts, dabc, n, q->n_ave, Kd, nullptr, P.stream);
+      rc = launch_assemble_avg(dpts.p, dabc.p, n, q->n_ave, Kd, nullptr, P.stream);
     }
     if (rc == kOk && P.Ksplit) rc = launch_split(Kd, n, P.Ksplit, P.stream);
     MT_HIP(hipStreamSynchronize(P.stream));
-    MT_HIP(hipFree(dpts));
-    MT_HIP(hipFree(dabc));
-    if (delem) MT_HIP(hipFree(delem));
-    if (!P.K32) MT_HIP(hipFree(Kd));
     if (rc) return rc;
   } else if (P.K32 || P.Ksplit) {
-    double *dpts = nullptr, *dabc = nullptr;
-    MT_HIP(hipMalloc(&dpts, pts.size() * sizeof(double)));
-    MT_HIP(hipMalloc(&dabc, 3 * sizeof(double)));
+    DevArr<double> dpts, dabc;
+    MT_HIP(dpts.alloc(pts.size()));
+    MT_HIP(dabc.alloc(3));
     double abc[3] = {P.dx, P.dy, P.dz};
-    MT_HIP(hipMemcpyAsync(dpts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
-    MT_HIP(hipMemcpyAsync(dabc, abc, sizeof(abc), hipMemcpyHostToDevice, P.stream));
-    int rc = launch_assemble(dpts, dabc, 0, nullptr, dpts, n, n, P.K32, nullptr, P.Ksplit, P.stream);
+    MT_HIP(hipMemcpyAsync(dpts.p, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
+    MT_HIP(hipMemcpyAsync(dabc.p, abc, sizeof(abc), hipMemcpyHostToDevice, P.stream));
+    int rc = launch_assemble(dpts.p, dabc.p, 0, nullptr, dpts.p, n, n, P.K32, nullptr, P.Ksplit, P.stream);
     if (rc) return rc;
     MT_HIP(hipStreamSynchronize(P.stream));
-    MT_HIP(hipFree(dpts));
-    MT_HIP(hipFree(dabc));
   }
   if (q->K_out) {
     MT_CHECK(P.K32 != nullptr, "K_out needs the FP32 tensor (precision fp32 or keep_fp32_tensor)");
