@@ -412,7 +412,7 @@ def main():
         achieved = alg_bytes / (gemm_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("dense"),
-                    "kernel": "gemm_bf16x2_reg_kernel<4>", "kernel_ms": gemm_ms,
+                    "kernel": "gemm_bf16x2_reg_kernel<4, false>", "kernel_ms": gemm_ms,
                     "alg_bytes_per_launch": alg_bytes, "measured_read_peak": rd.value,
                     "frac_of_measured_read_peak": achieved / rd.value}
     else:

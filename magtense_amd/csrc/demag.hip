@@ -742,12 +742,13 @@ int gemm_ksplit_dense(int n, int nrows) {
   const int v = gemm_variant();
   if (v == 1 || v == 5 || v == 6) return gemm_ksplit_rows(n, nrows, 2);
   // register-ring kernel: one resident workgroup (four row tiles) per CU; the
-  // smallest k-split that gives every CU a workgroup (measured: N = 32768
-  // k-split 4 / 8 / 16 -> 4.19 / 4.33 / 4.52 ms; a 4096-row block 8 -> 0.69 ms
-  // per call, 4 (half the CUs idle) -> 0.99 ms)
+  // smallest k-split that gives every CU a workgroup (measured at N = 32768:
+  // k-split 1 / 2 / 4 -> 4.21 / 4.23 / 4.27 ms in one run, 4 / 8 / 16 -> 4.19 /
+  // 4.33 / 4.52 ms in another, and fewer partial sums for the LL kernel to read;
+  // a 4096-row block at 8 -> 0.69 ms per call, 4 (half the CUs idle) -> 0.99 ms)
   const int nJ = (int)(round_up(n, kTileM) / kTileK);
   const int groups = (int)((round_up(nrows, kTileM) / kTileM + 3) / 4);
-  int ks = 4;
+  int ks = 1;
   while (groups * ks < 256 && ks * 2 <= nJ) ks *= 2;
   return clamp_ksplit(ks, nJ);
 }
