@@ -13,9 +13,10 @@ fragment-tiled tensor (HBM-bound).  Beside it, on the same states: the same
 product over the uniform grid's translation-invariant table (140 MB, L2/MALL
 resident), the plain per-state FP32 mat-vec path (the drop-in), the product
 at one row shard's shape (N/8 receivers x N sources, the per-GPU work of the
-row-sharded design), the FP64 field error, the 16384-prism tensor assembly,
-and the FP64 CPU oracle timed on a bounded sample on the host cores.  Prints
-one JSON line.
+row-sharded design), the gradient wall time of the differentiation
+configuration (RK4 adjoint, N = 4096, 2000 steps), the FP64 field error, the
+16384-prism tensor assembly, and the FP64 CPU oracle timed on a bounded
+sample on the host cores.  Prints one JSON line.
 """
 
 import argparse
@@ -255,6 +256,39 @@ def shard_shape(W, shards=8, shard=3, reps=10):
                      "is not built")
 
 
+def adjoint_c5(seed=5, nsteps=2000, dt=1e-13, every=100):
+    """The differentiation configuration: 128 x 32 x 1 film (N = 4096), per-cell
+    Ms ~ U[7e5, 9e5] A/m, FP32 demag, 2000 fixed RK4 steps of 0.1 ps; loss =
+    mean m_z(T) + |m(T) - m_target|; gradients w.r.t. the applied field (3) and
+    Ms (4096) by the discrete adjoint with a checkpoint every 100 steps.  Wall
+    times of the forward pass and of forward + backward."""
+    import torch
+    from magtense_amd.adjoint import ll_rk4
+
+    res, Lg = [128, 32, 1], [500e-9, 125e-9, 3e-9]
+    n = 4096
+    rng = np.random.default_rng(seed)
+    unit = lambda a: a / np.linalg.norm(a, axis=1, keepdims=True)  # noqa: E731
+    # smooth near-uniform start (from random unit vectors the exchange-driven
+    # 2000-step trajectory is chaotic and the gradients meaningless)
+    x, y = np.meshgrid(np.arange(res[0]) / res[0], np.arange(res[1]) / res[1])
+    m0 = np.stack([np.ones(n), 0.2 * np.sin(2 * np.pi * x).ravel(), 0.1 * np.cos(2 * np.pi * y).ravel()], axis=1)
+    m0 = torch.tensor(unit(m0))
+    target = torch.tensor(unit(rng.normal(size=(n, 3))))
+    ms = torch.tensor(rng.uniform(7e5, 9e5, n), requires_grad=True)
+    h = torch.tensor(np.array([-24.6, 4.3, 0.0]) / 1000 / MU0, requires_grad=True)
+    t0 = time.perf_counter()
+    mT = ll_rk4(m0, h, ms, res, Lg, dt, nsteps, checkpoint_every=every)
+    t_fwd = time.perf_counter() - t0
+    loss = mT[:, 2].mean() + torch.linalg.norm(mT - target)
+    loss.backward()
+    t_all = time.perf_counter() - t0
+    return dict(workload=f"128x32x1 film, N = {n}, {nsteps} RK4 steps of {dt:g} s, checkpoint every {every}, FP32 demag",
+                forward_s=t_fwd, forward_backward_s=t_all, forward_rhs_per_s=4 * nsteps / t_fwd,
+                grad_h_norm=float(h.grad.norm()), grad_ms_norm=float(ms.grad.norm()),
+                note="errors vs central differences and the FP64 oracle adjoint: tests/test_gpu_adjoint.py")
+
+
 def lattice_prisms(seed=4, side=128):
     """16384 prisms: side x side lattice of pitch 5 nm, centres jittered by
     U[-0.5, 0.5] nm in x and y, edges U[0.8, 1.2] * (5, 5, 2) nm."""
@@ -317,6 +351,8 @@ def main():
     ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--no-shard", action="store_true",
                     help="skip the product at one row shard's shape (N/8 rows x N sources)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiation configuration (RK4 adjoint gradient wall time)")
     ap.add_argument("--table-steps", type=int, default=30,
                     help="steps of the translation-invariant table path timed beside the dense tensor (0: skip)")
     ap.add_argument("--table", action="store_true",
@@ -398,6 +434,8 @@ def main():
         log(f"fp32 plain: {args.plain_steps} steps in {wall2:.3f} s = {plain:.4f} steps/s")
     shard = None if args.no_shard else shard_shape(W)
     log(f"row shard shape: {shard}")
+    adj = None if args.no_adjoint else adjoint_c5()
+    log(f"adjoint (differentiation configuration): {adj}")
     rd, cp = C.c_double(0), C.c_double(0)
     _lib.check(L.mt_measure_bandwidth(8 << 30, 10, C.byref(rd), C.byref(cp)))
     log(f"measured streaming bandwidth: read {rd.value:.0f} GB/s, copy {cp.value:.0f} GB/s")
@@ -451,6 +489,7 @@ def main():
         "tensor_assembly": asm,
         "table_path": table,
         "row_shard": shard,
+        "adjoint": adj,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
