@@ -183,6 +183,17 @@ def field_error_vs_fp64(W, n_rows=1024, n_states=64, seed=7):
                 bf16x2_max_rel=float(np.abs(gbf - ref).max() / np.abs(ref).max()))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(W, rows=2048):
     """FP64 oracle (FP32 demag semantics, 9 gemv-equivalents per RHS) on the host
     cores: the demag products of one full batch step (7 RHS x all B states) for
@@ -207,6 +218,7 @@ def cpu_baseline(W, rows=2048):
     t_batch_step = t_sample * n / rows
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=1.0 / t_batch_step, unit="LL batch-steps/s (64 states)", cores=cores, kind="port",
+                cpu_model=cpu_model(), nproc=os.cpu_count(),
                 sample=(f"FP64 oracle demag products (FP32 tensor, FP64-exact sums, reference gemv order) of one "
                         f"batch step (7 RHS x {W['B']} states) for {rows} of {n} receiver rows: {t_sample:.2f} s "
                         f"on {cores} threads; extrapolated x{n / rows:g} rows (tensor rows took {t_tensor:.1f} s, "
@@ -333,8 +345,17 @@ def tensor_assembly(reps=3, cpu_rows=64):
         t0 = time.perf_counter()
         O.tensor_points(c, abc, c[rows])
         tc = time.perf_counter() - t0
+        # one thread, as the reference assembles (omp_set_num_threads(1),
+        # LandauLifshitzEquationSolver.f90:927)
+        prev = O.set_threads(1)
+        t0 = time.perf_counter()
+        O.tensor_points(c, abc, c[rows[:cpu_rows // 4]])
+        t1 = time.perf_counter() - t0
+        O.set_threads(prev)
         out["cpu_pairs_per_s"] = cpu_rows * n / tc
-        out["cpu_sample"] = f"oracle prism tensor, {cpu_rows} rows x {n} sources, {tc:.2f} s"
+        out["cpu_pairs_per_s_1thread"] = (cpu_rows // 4) * n / t1
+        out["cpu_sample"] = (f"oracle prism tensor, {cpu_rows} rows x {n} sources, {tc:.2f} s on {prev} threads; "
+                             f"{cpu_rows // 4} rows on 1 thread, {t1:.2f} s")
     return out
 
 

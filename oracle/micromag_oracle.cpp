@@ -41,6 +41,7 @@
 #include <cstdio>
 #include <cstring>
 #include <vector>
+#include <omp.h>
 
 namespace {
 
@@ -2096,5 +2097,15 @@ int or_tetra_n(const double* vert, const double* r, double* N) {
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b) N[3 * a + b] = M[a][b];
   return 0;
+}
+}  // extern "C"
+
+extern "C" {
+// OpenMP threads used by the oracle's parallel loops (tensor assembly, FP32
+// demag rows); returns the previous setting.
+int or_set_threads(int n) {
+  const int prev = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return prev;
 }
 }  // extern "C"

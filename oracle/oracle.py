@@ -143,6 +143,11 @@ def tensor_unstructured(pts, abc, n_ave=(1, 1, 1)):
     return K
 
 
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops; returns the previous count."""
+    return int(lib().or_set_threads(int(n)))
+
+
 def tetra_n(vert, r):
     """Tetrahedron tensor (MagTense sign) at r; vert (4, 3) vertices; (3, 3)."""
     v = _c64(np.asarray(vert, dtype=np.float64).reshape(4, 3))
