@@ -115,128 +115,14 @@ __global__ void __launch_bounds__(256) matvec_f32_kernel(const float* __restrict
 //     32sb + l%32, same k mapping.
 //   C: 32x32 FP32 per (alpha, sb); lane l, register r -> column l%32,
 //     row 8(r/4) + 4(l/32) + r%4.
-// Grid: x = row group (WAVES row tiles) * KSPLIT + k-split index, so that with
-// KSPLIT a multiple of 8 every XCD streams one fixed slice of the B operand
-// (which then stays resident in that XCD's L2).  One workgroup shares each
-// B chunk through LDS among its WAVES waves.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 __device__ inline bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-template <int WAVES, int SBLK>
-__global__ void __launch_bounds__(WAVES * 64) gemm_bf16x2_kernel(const uint4* __restrict__ Kt,
-                                                                  const uint4* __restrict__ Mp, int64_t np,
-                                                                  int ksplit, float* __restrict__ out, int n,
-                                                                  int64_t col_stride, int64_t split_stride,
-                                                                  int ncols) {
-  constexpr int MF = 3 * SBLK * 2;  // B fragments per k-chunk used by this kernel
-  __shared__ uint4 lds[2][MF][64];
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nJ = (int)(np / kTileK);
-  const int ks = blockIdx.x % ksplit;
-  const int group = blockIdx.x / ksplit;
-  const int64_t I = (int64_t)group * WAVES + wave;  // row tile of this wave
-  const int n_row_tiles = (int)(np / kTileM);
-  const bool active = I < n_row_tiles;
-  const int jb = (int)((int64_t)nJ * ks / ksplit);
-  const int je = (int)((int64_t)nJ * (ks + 1) / ksplit);
-
-  floatx16 acc[3][SBLK];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int sb = 0; sb < SBLK; ++sb)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[a][sb][r] = 0.0f;
-
-  // B chunk J: fragments f = (beta*2 + sb)*2 + part at Mp[(J*12 + f)*64 + lane]
-  auto load_b = [&](int J, uint4* regs) {
-    for (int q = threadIdx.x; q < MF * 64; q += WAVES * 64) {
-      int f = q / 64, l = q % 64;
-      int beta = f / (SBLK * 2), rem = f % (SBLK * 2);
-      int sb = rem / 2, part = rem % 2;
-      int gf = (beta * 2 + sb) * 2 + part;
-      regs[q / (WAVES * 64)] = Mp[((int64_t)J * 12 + gf) * 64 + l];
-    }
-  };
-  constexpr int BREGS = (MF * 64 + WAVES * 64 - 1) / (WAVES * 64);
-  auto store_b = [&](int buf, const uint4* regs) {
-    for (int q = threadIdx.x; q < MF * 64; q += WAVES * 64) lds[buf][q / 64][q % 64] = regs[q / (WAVES * 64)];
-  };
-  const uint4* Krow = Kt + (I * nJ) * 12 * 64 + lane;
-  uint4 kc[12], kn[12], bpre[BREGS];
-  if (jb < je) {
-    load_b(jb, bpre);
-    store_b(0, bpre);
-    if (active)
-#pragma unroll
-      for (int f = 0; f < 12; ++f) kc[f] = Krow[((int64_t)jb * 12 + f) * 64];
-  }
-  __syncthreads();
-  for (int J = jb; J < je; ++J) {
-    const int buf = (J - jb) & 1;
-    const bool more = J + 1 < je;
-    if (more) {
-      load_b(J + 1, bpre);
-      if (active)
-#pragma unroll
-        for (int f = 0; f < 12; ++f) kn[f] = Krow[((int64_t)(J + 1) * 12 + f) * 64];
-    }
-    if (active) {
-      bf16x8 bh[3][SBLK], bl[3][SBLK];
-#pragma unroll
-      for (int beta = 0; beta < 3; ++beta)
-#pragma unroll
-        for (int sb = 0; sb < SBLK; ++sb) {
-          bh[beta][sb] = as_bf16x8(lds[buf][(beta * SBLK + sb) * 2 + 0][lane]);
-          bl[beta][sb] = as_bf16x8(lds[buf][(beta * SBLK + sb) * 2 + 1][lane]);
-        }
-      const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int beta = 0; beta < 3; ++beta) {
-          const int c = comp[a][beta];
-          bf16x8 kh = as_bf16x8(kc[c * 2 + 0]);
-          bf16x8 kl = as_bf16x8(kc[c * 2 + 1]);
-#pragma unroll
-          for (int sb = 0; sb < SBLK; ++sb) {
-            acc[a][sb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kh, bh[beta][sb], acc[a][sb], 0, 0, 0);
-            acc[a][sb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kh, bl[beta][sb], acc[a][sb], 0, 0, 0);
-            acc[a][sb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kl, bh[beta][sb], acc[a][sb], 0, 0, 0);
-          }
-        }
-    }
-    if (more) {
-      store_b(buf ^ 1, bpre);
-      if (active)
-#pragma unroll
-        for (int f = 0; f < 12; ++f) kc[f] = kn[f];
-    }
-    __syncthreads();
-  }
-  if (!active) return;
-  // epilogue: partial sums (positive K m) for this k-split
-  float* o = out + (int64_t)ks * split_stride;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int sb = 0; sb < SBLK; ++sb) {
-      const int col = 32 * sb + (lane & 31);
-      if (col >= ncols) continue;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-        if (row < n) o[col * col_stride + (int64_t)a * n + row] = acc[a][sb][r];
-      }
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Split-BF16 GEMM, pipelined version.  A workgroup of four waves owns two
+// Split-BF16 GEMM, two-tile LDS-ring version.  A workgroup of four waves owns two
 // 32-row tiles; waves (tile, sb) split the 64 state columns in two halves.
 // Every k-chunk (16 columns of the tensor) is streamed HBM -> LDS with
 // global_load_lds_dwordx4 (no VGPRs held by loads in flight) through a ring of
@@ -694,17 +580,14 @@ int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, f
 
 // Kernel choice for measurements (MT_GEMM_VARIANT): 0 (default) picks the
 // fastest measured kernel for each storage.  Dense tensor: 0 = register-ring
-// kernel with 4 chunks in flight, 7 / 8 = the same with 5 / 3 chunks, 6 =
-// two-tile kernel with a four-stage LDS DMA ring, 1 = register-staged
-// kernel, 5 = four-tile single-stage kernel.  Table: 2 = two-tile four-stage
-// kernel, 3 = four-tile two-stage kernel, 4 / 11 = register-ring kernel with
-// 4 / 6 chunks.
+// kernel with 4 chunks in flight, 6 = two-tile kernel with a four-stage LDS
+// DMA ring.  Table: 0 = four-tile single-stage LDS kernel, 2 = two-tile
+// four-stage kernel, 4 = register-ring kernel.
+// Read at every launch (a getenv per GEMM), so that one process can compare
+// the kernels.
 int gemm_variant() {
-  static int v = [] {
-    const char* e = getenv("MT_GEMM_VARIANT");
-    return e ? atoi(e) : 0;
-  }();
-  return v;
+  const char* e = getenv("MT_GEMM_VARIANT");
+  return e ? atoi(e) : 0;
 }
 
 namespace {
@@ -739,8 +622,7 @@ int gemm_ksplit_rows(int n, int nrows, int tiles_per_group) {
 }
 
 int gemm_ksplit_dense(int n, int nrows) {
-  const int v = gemm_variant();
-  if (v == 1 || v == 5 || v == 6) return gemm_ksplit_rows(n, nrows, 2);
+  if (gemm_variant() == 6) return gemm_ksplit_rows(n, nrows, 2);
   // register-ring kernel: one resident workgroup (four row tiles) per CU; the
   // smallest k-split that gives every CU a workgroup (measured at N = 32768:
   // k-split 1 / 2 / 4 -> 4.21 / 4.23 / 4.27 ms in one run, 4 / 8 / 16 -> 4.19 /
@@ -753,54 +635,10 @@ int gemm_ksplit_dense(int n, int nrows) {
   return clamp_ksplit(ks, nJ);
 }
 
-template <bool TOEP>
-__global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __restrict__ Kt,
-                                                                 const uint4* __restrict__ Mp, int64_t np,
-                                       int ksplit, float* __restrict__ out, int n, int64_t col_stride,
-                                       int64_t split_stride, int ncols, int gnx, int gny, int gnz);
-
-int launch_gemm_bf16x2_rows(const uint8_t* Krows, const uint8_t* Mpack, int n, int nrows, float* out,
-                            int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
-
-int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
-                       int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
-  if (gemm_variant() == 5) {
-    // measurement option: the single-stage four-tile kernel on the dense tensor
-    int64_t np4 = round_up(n, kTileM);
-    const int groups4 = (int)((np4 / kTileM + 3) / 4);
-    const int blocks4 = (groups4 * ksplit + 7) / 8 * 8;
-    hipLaunchKernelGGL((gemm_bf16x2_t4s_kernel<false>), dim3((unsigned)blocks4), dim3(256), 0, stream,
-                       reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np4, ksplit,
-                       out, n, col_stride, split_stride, ncols, 0, 1, 1);
-    MT_HIP(hipGetLastError());
-    return kOk;
-  }
-  if (gemm_variant() != 1)
-    return launch_gemm_bf16x2_rows(Ksplit, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream);
-  constexpr int WAVES = 4;
-  int64_t np = round_up(n, kTileM);
-  int groups = (int)((np / kTileM + WAVES - 1) / WAVES);
-  dim3 grid((unsigned)(groups * ksplit));
-  if (ncols <= 32)
-    hipLaunchKernelGGL((gemm_bf16x2_kernel<WAVES, 1>), grid, dim3(WAVES * 64), 0, stream,
-                       reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out,
-                       n, col_stride, split_stride, ncols);
-  else
-    hipLaunchKernelGGL((gemm_bf16x2_kernel<WAVES, 2>), grid, dim3(WAVES * 64), 0, stream,
-                       reinterpret_cast<const uint4*>(Ksplit), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out,
-                       n, col_stride, split_stride, ncols);
-  MT_HIP(hipGetLastError());
-  return kOk;
-}
-
 int launch_gemm_bf16x2_rows(const uint8_t* Krows, const uint8_t* Mpack, int n, int nrows, float* out,
                             int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
-  switch (gemm_variant()) {
-    case 6: break;
-    case 7: return launch_reg<5>(Krows, Mpack, n, nrows, out, col_stride, split_stride, ncols, ksplit, stream);
-    case 8: return launch_reg<3>(Krows, Mpack, n, nrows, out, col_stride, split_stride, ncols, ksplit, stream);
-    default: return launch_reg<4>(Krows, Mpack, n, nrows, out, col_stride, split_stride, ncols, ksplit, stream);
-  }
+  if (gemm_variant() != 6)
+    return launch_reg<4>(Krows, Mpack, n, nrows, out, col_stride, split_stride, ncols, ksplit, stream);
   // variant 6: the two-tile kernel with the four-stage LDS DMA ring
   const int64_t np = round_up(n, kTileM);
   const int groups2 = (int)((round_up(nrows, kTileM) / kTileM + 1) / 2);
@@ -811,179 +649,20 @@ int launch_gemm_bf16x2_rows(const uint8_t* Krows, const uint8_t* Mpack, int n, i
   return kOk;
 }
 
-// Four-tile variant for the translation-invariant table: a workgroup of four
-// waves owns four 32-row tiles, each wave one tile and all 64 states (six
-// accumulators).  Tensor and state fragments of a k-chunk (48 + 12 KiB) are
-// DMA'd into one of two LDS slots one chunk ahead; each wave reads the next
-// chunk's 24 fragments from LDS into a second register set while its 54
-// MFMAs of the current chunk execute, so LDS traffic overlaps the matrix
-// pipe.  (The table is L2-resident, so the kernel is bound by the MFMA and
-// LDS pipes rather than by HBM.)
-template <bool TOEP>
-__global__ void __launch_bounds__(256, 1) gemm_bf16x2_t4_kernel(const uint4* __restrict__ Kt,
-                                                                const uint4* __restrict__ Mp, int64_t np,
-                                                                int ksplit, float* __restrict__ out, int n,
-                                                                int64_t col_stride, int64_t split_stride, int ncols,
-                                                                int gnx, int gny, int gnz) {
-  constexpr int FR = 60;     // fragments per slot: 4 tiles x 12 + 12 state fragments
-  constexpr int LOADS = 15;  // per wave per chunk: 12 tensor + 3 state fragments
-  __shared__ uint4 smem[2][FR][64];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nJ = (int)(np / kTileK);
-  const int n_row_tiles = (int)(np / kTileM);
-  const int groups = (n_row_tiles + 3) / 4;
-  // XCD-aware order (block b runs on XCD b % 8).  With the table and a k-split
-  // dividing 8, XCD x takes k-split x % ksplit for every group: its resident
-  // workgroups then walk the same offset sequence (below) and share the table
-  // slabs in its L2.  Otherwise each XCD takes a contiguous range of work.
-  int ks, group;
-  const bool by_ks = TOEP && ksplit <= 8 && 8 % ksplit == 0;
-  if (by_ks) {
-    const int xpk = 8 / ksplit;  // XCDs per k-split
-    const int x = (int)(blockIdx.x % 8);
-    ks = x % ksplit;
-    group = (int)(blockIdx.x / 8) * xpk + x / ksplit;
-    if (group >= groups) return;  // whole workgroup, before any barrier
-  } else {
-    const int total = groups * ksplit;
-    const int per_xcd = (total + 7) / 8;
-    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (lin >= total) return;
-    ks = lin % ksplit;
-    group = lin / ksplit;
-  }
-  const int jb = (int)((int64_t)nJ * ks / ksplit);
-  const int je = (int)((int64_t)nJ * (ks + 1) / ksplit);
-  const int64_t I = (int64_t)group * 4 + wave;
-  const bool tile_ok = I < n_row_tiles;
-  const int64_t Il = tile_ok ? I : 0;  // loads stay uniform; never consumed
-  const int cpr = gnx / 16, tpr = gnx / 32, U = gnx / 8 - 2;
-  int tx0 = 0, tiy = 0, tiz = 0, wy = 0, wz = 0;
-  if (TOEP) {
-    const int irow = (int)(Il / tpr);
-    tx0 = (int)(Il - (int64_t)irow * tpr);
-    tiy = irow % gny;
-    tiz = irow / gny;
-    // the workgroup's first tile sets the chunk order shared by its four waves
-    const int64_t I0 = (int64_t)group * 4 < n_row_tiles ? (int64_t)group * 4 : 0;
-    const int wrow = (int)(I0 / tpr);
-    wy = wrow % gny;
-    wz = wrow / gny;
-  }
-  // Chunk v of the k range -> k-chunk J.  With the table the k-chunks are
-  // visited by cell offset: v = ((s ny + r) cpr + x1c) takes source row
-  // jy = (wy - r) mod ny in plane jz = (wz - s) mod nz, so every workgroup
-  // with the same k-split reads the same (dy, dz) slab at the same step.
-  // For each tile this is a permutation of the k-chunks.
-  auto chunk_of = [&](int v) {
-    if (!TOEP) return v;
-    const int x1c = v % cpr;
-    const int rs = v / cpr;
-    const int r = rs % gny, sp = rs / gny;
-    const int jy = (wy - r + gny) % gny, jz = (wz - sp + gnz) % gnz;
-    return (jz * gny + jy) * cpr + x1c;
-  };
-  auto issue = [&](int v, int slot) {
-    const int J = chunk_of(v);
-    int64_t base;
-    if (TOEP) {
-      const int jrow = J / cpr;
-      const int x1c = J - jrow * cpr;
-      const int jy = jrow % gny, jz = jrow / gny;
-      base = (((int64_t)(tiz - jz + gnz - 1) * (2 * gny - 1) + (tiy - jy + gny - 1)) * U + (2 * tx0 - x1c + cpr - 1)) * 12;
-    } else {
-      base = (Il * nJ + J) * 12;
-    }
-#pragma unroll
-    for (int q = 0; q < 12; ++q) {
-      if (TOEP)
-        lds_dma16(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
-      else  // the dense tensor is read once per pass
-        lds_dma16_nt(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const int f = wave * 3 + q;  // 0..11
-      lds_dma16(Mp + ((int64_t)J * 12 + f) * 64 + lane, lds_addr(&smem[slot][48 + f][0]));
-    }
-  };
-  floatx16 acc[2][3];
-#pragma unroll
-  for (int sb = 0; sb < 2; ++sb)
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[sb][a][r] = 0.0f;
-
-  // register sets: 12 tensor fragments of this wave's tile, 12 state fragments
-  uint4 ra[2][12], rb[2][12];
-  auto read_slot = [&](int slot, int set) {
-#pragma unroll
-    for (int q = 0; q < 12; ++q) ra[set][q] = smem[slot][wave * 12 + q][lane];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) rb[set][q] = smem[slot][48 + q][lane];
-  };
-  auto mfmas = [&](int set) {
-    const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-#pragma unroll
-    for (int beta = 0; beta < 3; ++beta)
-#pragma unroll
-      for (int t = 0; t < 3; ++t)  // hi*hi, hi*lo, lo*hi
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-          for (int sb = 0; sb < 2; ++sb) {
-            const int c = comp[a][beta];
-            const bf16x8 kf = as_bf16x8(ra[set][2 * c + (t == 2 ? 1 : 0)]);
-            const bf16x8 mf = as_bf16x8(rb[set][(beta * 2 + sb) * 2 + (t == 1 ? 1 : 0)]);
-            acc[sb][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kf, mf, acc[sb][a], 0, 0, 0);
-          }
-  };
-
-  const int nchunks = je - jb;
-  if (nchunks > 0) {
-    issue(jb, 0);
-    issue(jb + (nchunks > 1 ? 1 : 0), 1);
-    __builtin_amdgcn_s_waitcnt(vmcnt_imm(LOADS));  // my chunk-0 loads
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-    read_slot(0, 0);
-    for (int j = 0; j < nchunks; j += 2) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int jj = j + h;
-        if (jj < nchunks) {
-          // my loads of chunk jj+1 have landed and my reads of slot jj%2 are done
-          __builtin_amdgcn_s_waitcnt(0xC07F & 0xFFFF);  // lgkmcnt(0)
-          __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-          __builtin_amdgcn_s_barrier();
-          asm volatile("" ::: "memory");
-          const int jn = jj + 2 < nchunks ? jj + 2 : nchunks - 1;
-          issue(jb + jn, h);                            // slot jj % 2 == h
-          read_slot(h ^ 1, h ^ 1);                      // chunk jj+1 (or a repeat at the end)
-          if (tile_ok) mfmas(h);
-        }
-      }
-    }
-    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // no LDS write may outlive the workgroup
-  }
-  if (!tile_ok) return;
-  float* o = out + (int64_t)ks * split_stride;
-#pragma unroll
-  for (int sb = 0; sb < 2; ++sb) {
-    const int col = 32 * sb + (lane & 31);
-    if (col >= ncols) continue;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-        if (row < n) o[col * col_stride + (int64_t)a * n + row] = acc[sb][a][r];
-      }
-  }
+int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
+                       int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+  return launch_gemm_bf16x2_rows(Ksplit, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream);
 }
 
+// Four-tile kernel for the translation-invariant table: a workgroup of four
+// waves owns four 32-row tiles, each wave one tile and all 64 states (six
+// accumulators).  Tensor and state fragments of a k-chunk (48 + 12 KiB) are
+// DMA'd into a single LDS slot; each wave copies its 24 fragments of the chunk
+// into registers, the workgroup releases the slot and DMAs the next chunk, and
+// the chunk's 54 MFMAs run while that copy is in flight.  Two workgroups per CU
+// interleave, so one's MFMAs cover the other's LDS reads.  (The table is
+// L2-resident, so the kernel is bound by the MFMA and LDS pipes and the
+// L2 -> LDS transfer rather than by HBM.)
 template <bool TOEP>
 __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __restrict__ Kt,
                                                                 const uint4* __restrict__ Mp, int64_t np,
@@ -991,7 +670,6 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __
                                                                 int64_t col_stride, int64_t split_stride, int ncols,
                                                                 int gnx, int gny, int gnz) {
   constexpr int FR = 60;     // fragments per slot: 4 tiles x 12 + 12 state fragments
-  constexpr int LOADS = 15;  // per wave per chunk: 12 tensor + 3 state fragments
   __shared__ uint4 smem[1][FR][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1143,12 +821,10 @@ int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, con
   const int n = nx * ny * nz;
   if (gemm_variant() == 4)
     return launch_reg<4, true>(Ktab, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream, nx, ny, nz);
-  if (gemm_variant() == 11)
-    return launch_reg<6, true>(Ktab, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream, nx, ny, nz);
-  if (gemm_variant() != 2 && gemm_variant() != 3) {
+  if (gemm_variant() != 2) {
     // default: single-stage four-tile kernel, two workgroups per CU (measured at
-    // N = 32768, 64 states: 3.19 ms, against 3.31 ms for the two-stage four-tile
-    // kernel and 3.50 ms for the two-tile four-stage kernel)
+    // N = 32768, 64 states: 3.19 ms, against 3.31 ms for a two-stage version of
+    // it and 3.50 ms for the two-tile four-stage kernel)
     const int groups4 = (n / kTileM + 3) / 4;
     const int blocks4 = (groups4 * ksplit + 7) / 8 * 8;
     hipLaunchKernelGGL((gemm_bf16x2_t4s_kernel<true>), dim3((unsigned)blocks4), dim3(256), 0, stream,
@@ -1157,16 +833,7 @@ int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, con
     MT_HIP(hipGetLastError());
     return kOk;
   }
-  if (gemm_variant() == 3) {
-    // four-tile kernel; the k-split only changes how many partial sums the LL kernel adds
-    const int groups4 = (n / kTileM + 3) / 4;
-    const int blocks4 = (groups4 * ksplit + 7) / 8 * 8;
-    hipLaunchKernelGGL((gemm_bf16x2_t4_kernel<true>), dim3((unsigned)blocks4), dim3(256), 0, stream,
-                       reinterpret_cast<const uint4*>(Ktab), reinterpret_cast<const uint4*>(Mpack), (int64_t)n,
-                       ksplit, out, n, col_stride, split_stride, ncols, nx, ny, nz);
-    MT_HIP(hipGetLastError());
-    return kOk;
-  }
+  // variant 2: the two-tile kernel with the four-stage LDS DMA ring
   const int groups = (n / kTileM + 1) / 2;
   const int blocks = (groups * ksplit + 7) / 8 * 8;  // a multiple of the XCD count (see the kernel's mapping)
   hipLaunchKernelGGL((gemm_bf16x2_lds_kernel<4, true>), dim3((unsigned)blocks), dim3(256), 0, stream,
