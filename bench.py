@@ -136,6 +136,21 @@ def time_steps(L, h, steps, B):
     return wall, (L.mt_solver_steps_total(h) - s0) / B
 
 
+def step_breakdown(st, gemm_launches, steps):
+    """(c) LL right-hand side and (d) stepper vector operations of a timed run:
+    event time and algorithmic HBM bytes per launch and per batch step."""
+    def part(ms, launches, nbytes, kernel):
+        per = max(launches, 1)
+        return {"kernel": kernel, "launches_per_batch_step": launches / steps,
+                "ms_per_batch_step": ms / steps, "ms_per_launch": ms / per,
+                "alg_bytes_per_launch": nbytes / per,
+                "achieved_gbs": nbytes / max(ms, 1e-9) / 1e6,
+                "frac": nbytes / max(ms, 1e-9) / 1e6 / HBM_PEAK_GBS}
+    return {"batch_steps_timed": steps, "gemm_launches_per_batch_step": gemm_launches / steps,
+            "rhs": part(st.llg_ms, st.llg_launches, st.llg_bytes, "llg_kernel"),
+            "stepper": part(st.vec_ms, st.vec_launches, st.vec_bytes, "vec_kernel + finalize_kernel")}
+
+
 def field_error_vs_fp64(W, n_rows=1024, n_states=64, seed=7):
     """Relative demag-field error of the split-BF16 and FP32 products vs an FP64
     product on sampled receiver rows (FP64 tensor rows from the GPU kernel)."""
@@ -409,14 +424,22 @@ def main():
     # kernel duration with HIP events on the solver's stream (separate short run)
     L.mt_solver_set_timing(h, 1)
     sta = stats(L, h, B)
-    time_steps(L, h, 2, B)
+    _, timed_steps = time_steps(L, h, 2, B)
     stb = stats(L, h, B)
     L.mt_solver_set_timing(h, 0)
     gemm_launches = stb[0].demag_passes - sta[0].demag_passes
     gemm_ms = stb[0].demag_ms / max(gemm_launches, 1)
+    breakdown = step_breakdown(stb[0], gemm_launches, timed_steps)
     L.mt_solver_destroy(h)
     del keep
     steps_per_s = batch_steps / wall
+    ms_step = 1e3 * wall / batch_steps
+    breakdown["other_ms_per_batch_step"] = (ms_step - passes / batch_steps * gemm_ms - breakdown["rhs"]["ms_per_batch_step"]
+                                            - breakdown["stepper"]["ms_per_batch_step"])
+    breakdown["note"] = ("rhs = (c) LL kernel, stepper = (d) RKSUITE vector operations with their descriptor uploads, "
+                         "timed with HIP events over 2 batch steps; other = the default run's step time less the "
+                         "GEMM passes and those two: host scheduling and launch gaps")
+    log(f"step breakdown: {json.dumps(breakdown)}")
     log(f"bf16x2: setup {t_setup:.2f} s, {args.steps} steps in {wall:.3f} s = {steps_per_s:.2f} steps/s; "
         f"demag passes {passes}, rejected {rejected}, gemm {gemm_ms:.3f} ms/launch")
     alg_bytes = 6.0 * n * n * 4 + 2 * 3 * n * B * 4
@@ -459,7 +482,10 @@ def main():
     log(f"adjoint (differentiation configuration): {adj}")
     rd, cp = C.c_double(0), C.c_double(0)
     _lib.check(L.mt_measure_bandwidth(8 << 30, 10, C.byref(rd), C.byref(cp)))
-    log(f"measured streaming bandwidth: read {rd.value:.0f} GB/s, copy {cp.value:.0f} GB/s")
+    rdr = C.c_double(0)
+    _lib.check(L.mt_measure_bandwidth_pattern(8 << 30, 10, 1, C.byref(rdr), None))
+    log(f"measured streaming bandwidth: read {rd.value:.0f} GB/s (constant bytes), {rdr.value:.0f} GB/s "
+        f"(random bits), copy {cp.value:.0f} GB/s")
     ferr = None if args.no_field_error else field_error_vs_fp64(W)
     log(f"field error vs FP64: {ferr}")
     asm = None if args.no_assembly else tensor_assembly(cpu_rows=0 if args.no_cpu else 64)
@@ -473,7 +499,9 @@ def main():
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("dense"),
                     "kernel": "gemm_bf16x2_reg_kernel<4, false>", "kernel_ms": gemm_ms,
                     "alg_bytes_per_launch": alg_bytes, "measured_read_peak": rd.value,
-                    "frac_of_measured_read_peak": achieved / rd.value}
+                    "frac_of_measured_read_peak": achieved / rd.value,
+                    "measured_read_peak_random_bits": rdr.value,
+                    "frac_of_measured_read_peak_random_bits": achieved / rdr.value}
     else:
         # the translation-invariant table (140 MB) stays in L2/MALL: the matrix pipe bounds it
         achieved = alg_flops / (gemm_ms * 1e-3) / 1e12
@@ -509,6 +537,7 @@ def main():
         "field_error_vs_fp64": ferr,
         "tensor_assembly": asm,
         "table_path": table,
+        "step_breakdown": breakdown,
         "row_shard": shard,
         "adjoint": adj,
         "roofline": roofline,

@@ -63,6 +63,10 @@ int mt_device_count(void);
  * published 8 TB/s and the known-size launches that calibrate HBM counters.
  * Either output may be null. */
 int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* copy_gbps);
+/* Same probe over a buffer of constant bytes (pattern 0, what
+ * mt_measure_bandwidth reads) or of pseudo-random bits (pattern 1: HBM data
+ * lines toggle as with live data, which costs power and hence clock). */
+int mt_measure_bandwidth_pattern(long long bytes, int iters, int pattern, double* read_gbps, double* copy_gbps);
 
 /* ---- (a) prism tensor ----------------------------------------------------------
  * K = -N_standard (MagTense sign) of prism j (centre src[3j..], full edges
@@ -242,7 +246,14 @@ typedef struct {
   long long f_calls, steps, bad_steps, flag3, flag4;
   double h_last, t_reached;
   long long evals, ticks, demag_passes;
+  /* mt_solver_set_timing on: event-timed milliseconds since it was switched on
+   * of the demag products, of the stepper's vector operations (descriptor
+   * uploads included) and of the LL right-hand-side kernel, their launch counts
+   * and the algorithmic HBM bytes of the last two (every vector a launch reads
+   * or writes counted once) */
   double demag_ms;
+  double vec_ms, llg_ms, vec_bytes, llg_bytes;
+  long long vec_launches, llg_launches;
 } mt_stats;
 
 typedef struct mt_solver mt_solver;

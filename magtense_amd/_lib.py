@@ -64,6 +64,8 @@ class MtStats(C.Structure):
         ("h_last", C.c_double), ("t_reached", C.c_double),
         ("evals", C.c_longlong), ("ticks", C.c_longlong), ("demag_passes", C.c_longlong),
         ("demag_ms", C.c_double),
+        ("vec_ms", C.c_double), ("llg_ms", C.c_double), ("vec_bytes", C.c_double), ("llg_bytes", C.c_double),
+        ("vec_launches", C.c_longlong), ("llg_launches", C.c_longlong),
     ]
 
 
@@ -78,7 +80,7 @@ class MtTiles(C.Structure):
 
 # every symbol declared in include/magtense_hip.h
 EXPORTS = [
-    "mt_last_error", "mt_version", "mt_device_count", "mt_measure_bandwidth", "mt_prism_tensor", "mt_split_tensor_bytes",
+    "mt_last_error", "mt_version", "mt_device_count", "mt_measure_bandwidth", "mt_measure_bandwidth_pattern", "mt_prism_tensor", "mt_split_tensor_bytes",
     "mt_packed_m_bytes", "mt_split_tensor", "mt_demag_apply_f32", "mt_demag_apply_bf16x2", "mt_ll_rhs",
     "mt_solver_create", "mt_solver_advance", "mt_solver_run", "mt_solver_advance_total", "mt_solver_steps_total", "mt_solver_results", "mt_solver_state",
     "mt_solver_set_timing", "mt_solver_destroy", "mt_run_micromag", "mt_rk4_create", "mt_rk4_forward",
@@ -115,6 +117,8 @@ def load():
     lib.mt_packed_m_bytes.restype = C.c_longlong
     vp = C.c_void_p
     lib.mt_measure_bandwidth.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.mt_measure_bandwidth_pattern.argtypes = [C.c_longlong, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                                 C.POINTER(C.c_double)]
     lib.mt_prism_tensor.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp]
     lib.mt_split_tensor.argtypes = [vp, C.c_int, vp]
     lib.mt_demag_apply_f32.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_float]

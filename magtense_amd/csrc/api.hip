@@ -106,6 +106,24 @@ __global__ void copy_bw_kernel(const uint4* __restrict__ src, uint4* __restrict_
     dst[i] = src[i];
 }
 
+// Pseudo-random bits (the murmur3 32-bit finalizer of the word index): the
+// probe buffer then toggles as many HBM data lines as live tensor data does.
+__global__ void fill_random_kernel(uint4* __restrict__ p, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t w[4];
+    for (int k = 0; k < 4; ++k) {
+      uint32_t x = (uint32_t)(i * 4 + k) * 0x9E3779B9u + 0x7F4A7C15u;
+      x ^= x >> 16;
+      x *= 0x85EBCA6Bu;
+      x ^= x >> 13;
+      x *= 0xC2B2AE35u;
+      x ^= x >> 16;
+      w[k] = x;
+    }
+    p[i] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
 int device_ok() {
   int nd = 0;
   if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) {
@@ -148,9 +166,14 @@ long long mt_split_tensor_bytes(int n) { return split_tensor_bytes(n); }
 long long mt_packed_m_bytes(int n) { return packed_m_bytes(n); }
 
 int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* copy_gbps) {
+  return mt_measure_bandwidth_pattern(bytes, iters, 0, read_gbps, copy_gbps);
+}
+
+int mt_measure_bandwidth_pattern(long long bytes, int iters, int pattern, double* read_gbps, double* copy_gbps) {
   int rc = device_ok();
   if (rc) return rc;
   MT_CHECK(bytes >= (1 << 20) && iters >= 1, "need bytes >= 1 MiB and iters >= 1");
+  MT_CHECK(pattern == 0 || pattern == 1, "pattern: 0 constant bytes, 1 pseudo-random bits");
   const int64_t n16 = bytes / 16;
   DevArr<uint4> A, Bc;
   DevArr<uint32_t> S;
@@ -159,7 +182,12 @@ int mt_measure_bandwidth(long long bytes, int iters, double* read_gbps, double* 
   if (copy_gbps) MT_HIP(Bc.alloc(n16));
   uint4 *a = A.p, *b = Bc.p;
   uint32_t* sink = S.p;
-  MT_HIP(hipMemset(a, 0x11, n16 * 16));
+  if (pattern == 0) {
+    MT_HIP(hipMemset(a, 0x11, n16 * 16));
+  } else {
+    hipLaunchKernelGGL(fill_random_kernel, dim3(4096), dim3(256), 0, nullptr, a, n16);
+    MT_HIP(hipGetLastError());
+  }
   hipEvent_t e0, e1;
   MT_HIP(hipEventCreate(&e0));
   MT_HIP(hipEventCreate(&e1));

@@ -30,10 +30,19 @@ __device__ inline uint16_t bf16_bits(float v) {
 }
 __device__ inline float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-__global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc* __restrict__ descs) {
-  const LlgDesc& D = descs[blockIdx.y];
+// One thread per (cell, state).  The grid is 1-D with a multiple of 8 blocks;
+// block b runs on XCD b % 8, and XCD x takes the contiguous x-th eighth of the
+// (state, cell block) range, so the exchange stencil's neighbour rows and
+// planes are read through the L2 that already holds them.
+__global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc* __restrict__ descs,
+                                                  int cell_blocks, int nstates) {
+  const int total = cell_blocks * nstates;
+  const int per_xcd = (int)(gridDim.x / 8);
+  const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (w >= total) return;
+  const LlgDesc& D = descs[w / cell_blocks];
   const int n = g.n;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (w % cell_blocks) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double* m = D.m;
   const int ix = i % g.nx;
@@ -52,19 +61,37 @@ __global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc
       for (int k = kb; k < ke; ++k) sum += g.xv[k] * v[g.xc[k]];
       hj[c] = (-2.0 * sum) * g.jfact[i];
     }
-  } else for (int c = 0; c < 3; ++c) {
-    const double* v = m + (size_t)c * n;
-    double vi = v[i];
-    mc[c] = vi;
-    // Laplacian with the reference's boundary rows: (-1, 1) and (1, -1)
-    double lx = ix == 0 ? (v[i + 1] - vi) : (ix == g.nx - 1 ? (v[i - 1] - vi) : (v[i - 1] - 2.0 * vi + v[i + 1]));
-    double ly = iy == 0 ? (v[i + sy] - vi) : (iy == g.ny - 1 ? (v[i - sy] - vi) : (v[i - sy] - 2.0 * vi + v[i + sy]));
-    double lap = lx * g.idx2 + ly * g.idy2;
-    if (g.nz > 1) {
-      double lz = iz == 0 ? (v[i + sz] - vi) : (iz == g.nz - 1 ? (v[i - sz] - vi) : (v[i - sz] - 2.0 * vi + v[i + sz]));
-      lap += lz * g.idz2;
+  } else {
+    // Laplacian with the reference's boundary rows: (-1, 1) and (1, -1).  The
+    // neighbour indices are clamped to the cell itself at a boundary, so every
+    // load is issued unconditionally before the per-row formula is selected.
+    const int xm = ix > 0 ? i - 1 : i, xp = ix < g.nx - 1 ? i + 1 : i;
+    const int ym = iy > 0 ? i - sy : i, yp = iy < g.ny - 1 ? i + sy : i;
+    const bool z3 = g.nz > 1;
+    const int zm = z3 && iz > 0 ? i - sz : i, zp = z3 && iz < g.nz - 1 ? i + sz : i;
+    double c0[3], cxm[3], cxp[3], cym[3], cyp[3], czm[3], czp[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double* v = m + (size_t)c * n;
+      c0[c] = v[i];
+      cxm[c] = v[xm]; cxp[c] = v[xp];
+      cym[c] = v[ym]; cyp[c] = v[yp];
+      czm[c] = v[zm]; czp[c] = v[zp];
     }
-    hj[c] = (-2.0 * lap) * g.jfact[i];
+    const double jf = g.jfact[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double vi = c0[c];
+      mc[c] = vi;
+      double lx = ix == 0 ? (cxp[c] - vi) : (ix == g.nx - 1 ? (cxm[c] - vi) : (cxm[c] - 2.0 * vi + cxp[c]));
+      double ly = iy == 0 ? (cyp[c] - vi) : (iy == g.ny - 1 ? (cym[c] - vi) : (cym[c] - 2.0 * vi + cyp[c]));
+      double lap = lx * g.idx2 + ly * g.idy2;
+      if (z3) {
+        double lz = iz == 0 ? (czp[c] - vi) : (iz == g.nz - 1 ? (czm[c] - vi) : (czm[c] - 2.0 * vi + czp[c]));
+        lap += lz * g.idz2;
+      }
+      hj[c] = (-2.0 * lap) * jf;
+    }
   }
   // demag: S = sum_k partial (K m); field = fl32(fl32(-S) * Ms_i)
   if (D.hpart64) {
@@ -115,96 +142,179 @@ __device__ inline double nan_to_inf(double v) { return isnan(v) ? INFINITY : v; 
 
 // Generic vector op.  A block covers 2048 consecutive elements of one state;
 // thread t handles elements t + 256q (q < 8), so every load and store of the
-// combination and the reductions is coalesced.  The BF16 pack needs eight
-// consecutive elements per 16-byte store and goes through LDS.
+// combination and the reductions is coalesced.  Each operand vector is loaded
+// for all eight elements before any of them is used: a wave keeps eight
+// 512-byte loads in flight per operand instead of waiting on one at a time.
+// Per element the arithmetic (and its order) is that of the scalar form
+//   v = post_scale * (bconst + bcoef*base + sum_k coef[k]*src[k]).
+// The BF16 pack needs eight consecutive elements per 16-byte store and goes
+// through LDS.
+constexpr int kVecE = 8;
+
+__device__ inline void load_e(const double* p, int e0, int n3, double (&x)[kVecE]) {
+#pragma unroll
+  for (int q = 0; q < kVecE; ++q) {
+    const int e = e0 + 256 * q;
+    x[q] = e < n3 ? p[e] : 0.0;
+  }
+}
+
 __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ descs, int n, int n3,
                                                   int blocks_per_state, double* __restrict__ partials,
                                                   PackSpec pack) {
   const VecDesc& D = descs[blockIdx.y];
   const int t0 = blockIdx.x * 2048;
+  const int e0 = t0 + threadIdx.x;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
   const int kind = D.rkind;
   const bool do_pack = D.pack_col >= 0 && pack.mode != kPackNone && D.dst;
   __shared__ float tile[2048];
-#pragma unroll 2
-  for (int q = 0; q < 8; ++q) {
-    const int e = t0 + threadIdx.x + 256 * q;
-    if (e >= n3) break;
-    double v = 0.0;
-    if (D.dst) {
-      v = D.bconst;
-      if (D.base) v += D.bcoef * D.base[e];
-      for (int k = 0; k < D.nterm; ++k) v += D.coef[k] * D.src[k][e];
-      v = D.post_scale * v;
-      D.dst[e] = v;
+  bool ok[kVecE];
+#pragma unroll
+  for (int q = 0; q < kVecE; ++q) ok[q] = e0 + 256 * q < n3;
+  double v[kVecE], x[kVecE];
+#pragma unroll
+  for (int q = 0; q < kVecE; ++q) v[q] = 0.0;
+  if (D.dst) {
+#pragma unroll
+    for (int q = 0; q < kVecE; ++q) v[q] = D.bconst;
+    if (D.base) {
+      const double bc = D.bcoef;
+      load_e(D.base, e0, n3, x);
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q) v[q] += bc * x[q];
     }
-    if (do_pack) {
-      if (pack.mode == kPackF32) pack.f32[(int64_t)D.pack_col * pack.col_stride + e] = (float)v;
-      else tile[threadIdx.x + 256 * q] = (float)v;
+    for (int k = 0; k < D.nterm; ++k) {
+      const double c = D.coef[k];
+      load_e(D.src[k], e0, n3, x);
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q) v[q] += c * x[q];
     }
-    switch (kind) {
-      case kRedNone:
-        break;
-      case kRedStepB: {  // max |sum c_k s_k| / max(0.5(|y|+|ynew|), thresh)
-        double s = 0.0;
-        for (int k = 0; k < D.rn; ++k) s += D.rcoef[k] * D.rsrc[k][e];
-        double w = fmax(0.5 * (fabs(D.ra[e]) + fabs(D.rb[e])), D.thresh);
-        r0 = fmax(r0, nan_to_inf(fabs(s) / w));
-        break;
-      }
-      case kRedFinal: {  // errest = scale * sum c_k s_k; w = max(0.5(|y|+|ynew|), thresh)
-        double s = 0.0;
-        for (int k = 0; k < D.rn; ++k) s += D.rcoef[k] * D.rsrc[k][e];
-        s = D.rscale * s;
-        double w = fmax(0.5 * (fabs(D.ra[e]) + fabs(D.rb[e])), D.thresh);
-        if (D.rd) D.rd[e] = s;
-        if (D.rw) D.rw[e] = w;
-        r0 = fmax(r0, nan_to_inf(fabs(s / w)));
-        break;
-      }
-      case kRedStepA: {  // w = max(w, |ynew|); max |y|/w, |ynew|/w, |ynew-y|/w, |f-yp|/w
-        double ya = D.ra[e], yn = D.rb[e];
-        double w = fmax(D.rw[e], fabs(yn));
-        D.rw[e] = w;
-        r0 = fmax(r0, nan_to_inf(fabs(ya) / w));
-        r1 = fmax(r1, nan_to_inf(fabs(yn) / w));
-        r2 = fmax(r2, nan_to_inf(fabs(yn - ya) / w));
-        r3 = fmax(r3, nan_to_inf(fabs(D.rc[e] - D.rd[e]) / w));
-        break;
-      }
-      case kRedYpNorm: {  // w = max(|y|, thresh); max over y != 0 of |yp|/w
-        double ya = D.ra[e];
-        double w = fmax(fabs(ya), D.thresh);
-        D.rw[e] = w;
-        if (ya != 0.0) r0 = fmax(r0, nan_to_inf(fabs(D.rb[e]) / w));
-        break;
-      }
-      case kRedWeightsAvg: {  // w = max(0.5(|a|+|b|), thresh)
-        D.rw[e] = fmax(0.5 * (fabs(D.ra[e]) + fabs(D.rb[e])), D.thresh);
-        break;
-      }
-      case kRedWdot: {  // sums of (p/w)(q/w) for up to three pairs (rsrc[2k], rsrc[2k+1])
-        double w = D.rw[e];
-        if (D.rn > 0) r0 += (D.rsrc[0][e] / w) * (D.rsrc[1][e] / w);
-        if (D.rn > 1) r1 += (D.rsrc[2][e] / w) * (D.rsrc[3][e] / w);
-        if (D.rn > 2) r2 += (D.rsrc[4][e] / w) * (D.rsrc[5][e] / w);
-        break;
-      }
-      case kRedMaxDiff: {
-        r0 = fmax(r0, nan_to_inf(fabs(D.ra[e] - D.rb[e])));
-        break;
-      }
-      default:
-        break;
+    const double ps = D.post_scale;
+#pragma unroll
+    for (int q = 0; q < kVecE; ++q) {
+      v[q] = ps * v[q];
+      if (ok[q]) D.dst[e0 + 256 * q] = v[q];
     }
+  }
+  if (do_pack) {
+    if (pack.mode == kPackF32) {
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q)
+        if (ok[q]) pack.f32[(int64_t)D.pack_col * pack.col_stride + e0 + 256 * q] = (float)v[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q) tile[threadIdx.x + 256 * q] = (float)v[q];
+    }
+  }
+  switch (kind) {
+    case kRedNone:
+      break;
+    case kRedStepB:    // max |sum c_k s_k| / max(0.5(|y|+|ynew|), thresh)
+    case kRedFinal: {  // errest = scale * sum c_k s_k; w = max(0.5(|y|+|ynew|), thresh)
+      double s[kVecE], a[kVecE];
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q) s[q] = 0.0;
+      for (int k = 0; k < D.rn; ++k) {
+        const double c = D.rcoef[k];
+        load_e(D.rsrc[k], e0, n3, x);
+#pragma unroll
+        for (int q = 0; q < kVecE; ++q) s[q] += c * x[q];
+      }
+      load_e(D.ra, e0, n3, a);
+      load_e(D.rb, e0, n3, x);
+      const double th = D.thresh, sc = D.rscale;
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q) {
+        if (!ok[q]) continue;
+        const double w = fmax(0.5 * (fabs(a[q]) + fabs(x[q])), th);
+        if (kind == kRedStepB) {
+          r0 = fmax(r0, nan_to_inf(fabs(s[q]) / w));
+        } else {
+          const double sv = sc * s[q];
+          const int e = e0 + 256 * q;
+          if (D.rd) D.rd[e] = sv;
+          if (D.rw) D.rw[e] = w;
+          r0 = fmax(r0, nan_to_inf(fabs(sv / w)));
+        }
+      }
+      break;
+    }
+    case kRedStepA: {  // w = max(w, |ynew|); max |y|/w, |ynew|/w, |ynew-y|/w, |f-yp|/w
+      double ya[kVecE], yn[kVecE], wo[kVecE], fc[kVecE];
+      load_e(D.ra, e0, n3, ya);
+      load_e(D.rb, e0, n3, yn);
+      load_e(D.rw, e0, n3, wo);
+      load_e(D.rc, e0, n3, fc);
+      load_e(D.rd, e0, n3, x);
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q) {
+        if (!ok[q]) continue;
+        const double w = fmax(wo[q], fabs(yn[q]));
+        D.rw[e0 + 256 * q] = w;
+        r0 = fmax(r0, nan_to_inf(fabs(ya[q]) / w));
+        r1 = fmax(r1, nan_to_inf(fabs(yn[q]) / w));
+        r2 = fmax(r2, nan_to_inf(fabs(yn[q] - ya[q]) / w));
+        r3 = fmax(r3, nan_to_inf(fabs(fc[q] - x[q]) / w));
+      }
+      break;
+    }
+    case kRedYpNorm: {  // w = max(|y|, thresh); max over y != 0 of |yp|/w
+      double ya[kVecE];
+      load_e(D.ra, e0, n3, ya);
+      load_e(D.rb, e0, n3, x);
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q) {
+        if (!ok[q]) continue;
+        const double w = fmax(fabs(ya[q]), D.thresh);
+        D.rw[e0 + 256 * q] = w;
+        if (ya[q] != 0.0) r0 = fmax(r0, nan_to_inf(fabs(x[q]) / w));
+      }
+      break;
+    }
+    case kRedWeightsAvg: {  // w = max(0.5(|a|+|b|), thresh)
+      double a[kVecE];
+      load_e(D.ra, e0, n3, a);
+      load_e(D.rb, e0, n3, x);
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q)
+        if (ok[q]) D.rw[e0 + 256 * q] = fmax(0.5 * (fabs(a[q]) + fabs(x[q])), D.thresh);
+      break;
+    }
+    case kRedWdot: {  // sums of (p/w)(q/w) for up to three pairs (rsrc[2k], rsrc[2k+1])
+      double w[kVecE], p[kVecE];
+      load_e(D.rw, e0, n3, w);
+      double* acc[3] = {&r0, &r1, &r2};
+      for (int k = 0; k < D.rn && k < 3; ++k) {
+        load_e(D.rsrc[2 * k], e0, n3, p);
+        load_e(D.rsrc[2 * k + 1], e0, n3, x);
+        double r = *acc[k];
+#pragma unroll
+        for (int q = 0; q < kVecE; ++q)
+          if (ok[q]) r += (p[q] / w[q]) * (x[q] / w[q]);
+        *acc[k] = r;
+      }
+      break;
+    }
+    case kRedMaxDiff: {
+      double a[kVecE];
+      load_e(D.ra, e0, n3, a);
+      load_e(D.rb, e0, n3, x);
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q)
+        if (ok[q]) r0 = fmax(r0, nan_to_inf(fabs(a[q] - x[q])));
+      break;
+    }
+    default:
+      break;
   }
   if (do_pack && pack.mode == kPackBf16) {
     __syncthreads();
     const int l0 = threadIdx.x * 8;  // eight consecutive elements of this block's tile
-    const int e0 = t0 + l0;
-    if (e0 < n3) {
-      if ((n & 7) == 0 && e0 + 8 <= n3) {
-        const int c = e0 / n, j0 = e0 - c * n;
+    const int p0 = t0 + l0;
+    if (p0 < n3) {
+      if ((n & 7) == 0 && p0 + 8 <= n3) {
+        const int c = p0 / n, j0 = p0 - c * n;
         uint32_t hw[4], lw[4];
         for (int q = 0; q < 4; ++q) {
           float f0 = tile[l0 + 2 * q], f1 = tile[l0 + 2 * q + 1];
@@ -221,8 +331,8 @@ __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ de
         p[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
         p[kFragBytes / 16] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
       } else {
-        for (int q = 0; q < 8 && e0 + q < n3; ++q) {
-          int e = e0 + q;
+        for (int q = 0; q < 8 && p0 + q < n3; ++q) {
+          int e = p0 + q;
           int c = e / n, j = e - c * n;
           float f = tile[l0 + q];
           uint16_t hi = bf16_bits(f);
@@ -243,9 +353,9 @@ __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ de
   // block reduction of four values
   __shared__ double red[4][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double v[4] = {r0, r1, r2, r3};
+  const double rv[4] = {r0, r1, r2, r3};
   for (int k = 0; k < 4; ++k) {
-    double x = v[k];
+    double x = rv[k];
     for (int off = 32; off > 0; off >>= 1) {
       double o = __shfl_xor(x, off, 64);
       x = is_sum ? x + o : fmax(x, o);
@@ -284,8 +394,14 @@ __global__ void finalize_kernel(const VecDesc* __restrict__ descs, int nstates, 
 }  // namespace
 
 int launch_llg(const LlgGrid& g, const LlgDesc* d_descs, int nstates, hipStream_t stream) {
-  dim3 grid((unsigned)((g.n + 255) / 256), (unsigned)nstates);
-  hipLaunchKernelGGL(llg_kernel, grid, dim3(256), 0, stream, g, d_descs);
+  const int cell_blocks = (g.n + 255) / 256;
+  const int64_t total = (int64_t)cell_blocks * nstates;
+  const int64_t blocks = (total + 7) / 8 * 8;
+  if (blocks > INT32_MAX) {
+    set_error("LL kernel: too many cells x states for one launch");
+    return kErrArg;
+  }
+  hipLaunchKernelGGL(llg_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, g, d_descs, cell_blocks, nstates);
   MT_HIP(hipGetLastError());
   return kOk;
 }

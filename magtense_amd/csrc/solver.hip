@@ -285,6 +285,7 @@ struct Solver {
   uint8_t* Ksplit = nullptr;
   int *xs = nullptr, *xe = nullptr, *xc = nullptr;  // loaded exchange operator (unstructured grids)
   double* xv = nullptr;
+  long long exch_nnz_host = 0;
   uint8_t* Ktab = nullptr;    // translation-invariant fragment table (bf16x2 on uniform grids)
   int demag_approx = 1;       // 1 dense, 2 threshold, 4 threshold fraction
   SparseDemag sparse;
@@ -311,9 +312,13 @@ struct Solver {
   // statistics
   long long ticks = 0, eval_ticks = 0, demag_passes = 0;
   double gemm_ms = 0.0;
+  // timing mode: per-kernel event times and algorithmic bytes of the LL
+  // right-hand side and of the stepper's vector operations
+  double vec_ms = 0.0, llg_ms = 0.0, vec_bytes = 0.0, llg_bytes = 0.0;
+  long long vec_launches = 0, llg_launches = 0;
   bool timing = false;
   long long total_target = 0;   // > 0: stop once the states' accepted steps sum to this
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev[7] = {};  // timing: vec-only, pre-vec, GEMM, LL kernel, post-vec boundaries
 
   ~Solver() { release(); }
 
@@ -326,16 +331,84 @@ struct Solver {
     sparse.release();
     if (h_results) (void)hipHostFree(h_results);
     if (arena_h) (void)hipHostFree(arena_h);
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t& e : ev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     xs = xe = xc = nullptr; xv = nullptr;
     K32 = nullptr; Ksplit = nullptr; Ktab = nullptr; jfact = kfact = aniso = nullptr; ms32 = nullptr; pool = yout = fout = nullptr;
     packf = nullptr; packb = nullptr; hbuf = nullptr; partials = results = nullptr; arena_d = nullptr;
-    h_results = nullptr; arena_h = nullptr; stream = nullptr; ev0 = ev1 = nullptr;
+    h_results = nullptr; arena_h = nullptr; stream = nullptr;
+    for (hipEvent_t& e : ev) e = nullptr;
   }
 
   size_t n3() const { return (size_t)3 * n; }
+
+  // Algorithmic HBM bytes of one state's vector operation: every distinct
+  // vector read or written once (a reduction operand that the combination just
+  // wrote is not read again), plus the demag input pack (BF16 hi+lo or FP32).
+  double vec_desc_bytes(const VecDesc& d, int pack_mode) const {
+    const void* rd[kMaxTerms + 16];
+    const void* wr[4];
+    int nr = 0, nw = 0;
+    auto add = [](const void** set, int& k, const void* p) {
+      if (!p) return;
+      for (int i = 0; i < k; ++i)
+        if (set[i] == p) return;
+      set[k++] = p;
+    };
+    if (d.dst) {
+      add(wr, nw, d.dst);
+      add(rd, nr, d.base);
+      for (int k = 0; k < d.nterm; ++k) add(rd, nr, d.src[k]);
+    }
+    auto rd_red = [&](const void* p) {
+      if (p && p != d.dst) add(rd, nr, p);
+    };
+    switch (d.rkind) {
+      case kRedStepB:
+      case kRedFinal:
+        for (int k = 0; k < d.rn; ++k) rd_red(d.rsrc[k]);
+        rd_red(d.ra); rd_red(d.rb);
+        if (d.rkind == kRedFinal) { add(wr, nw, d.rd); add(wr, nw, d.rw); }
+        break;
+      case kRedStepA:
+        rd_red(d.ra); rd_red(d.rb); rd_red(d.rc); rd_red(d.rd); rd_red(d.rw); add(wr, nw, d.rw);
+        break;
+      case kRedYpNorm:
+      case kRedWeightsAvg:
+        rd_red(d.ra); rd_red(d.rb); add(wr, nw, d.rw);
+        break;
+      case kRedWdot:
+        rd_red(d.rw);
+        for (int k = 0; k < 2 * d.rn && k < 8; ++k) rd_red(d.rsrc[k]);
+        break;
+      case kRedMaxDiff:
+        rd_red(d.ra); rd_red(d.rb);
+        break;
+      default:
+        break;
+    }
+    double bytes = 8.0 * (double)n3() * (nr + nw);
+    if (d.dst && d.pack_col >= 0 && pack_mode != kPackNone) bytes += 4.0 * (double)n3();
+    return bytes;
+  }
+
+  // Algorithmic HBM bytes of one LL-kernel launch over S states: per state the
+  // argument, the demag sums (FP32 partials or FP64) and dm/dt (plus the four
+  // stored fields when asked); per launch the per-cell coefficient arrays.
+  double llg_launch_bytes(const std::vector<LlgDesc>& ld) const {
+    const double c3 = (double)n3();
+    double bytes = 8.0 * n + 4.0 * n;  // jfact, ms32
+    if (grid.aniso) bytes += 48.0 * n + 8.0 * n;
+    if (grid.xs) bytes += 8.0 * n + 12.0 * (double)exch_nnz_host;
+    for (const LlgDesc& L : ld) {
+      bytes += 8.0 * c3;
+      bytes += L.hpart64 ? 8.0 * c3 + 8.0 * n : 4.0 * c3 * L.nsplit;
+      if (L.out) bytes += 8.0 * c3;
+      if (L.fields) bytes += 32.0 * c3;
+    }
+    return bytes;
+  }
 
   // descriptor arena: pinned host staging + device copy; reset after syncs
   template <typename T>
@@ -350,6 +423,33 @@ struct Solver {
     MT_HIP(hipMemcpyAsync(arena_d + off, arena_h + off, bytes, hipMemcpyHostToDevice, stream));
     *dptr = reinterpret_cast<const T*>(arena_d + off);
     arena_off = off + bytes;
+    return kOk;
+  }
+
+  // Several descriptor arrays with one host -> device copy: stage_reserve makes
+  // room for `bytes` (plus alignment) without a reset in between, stage_put
+  // lays an array out in the pinned arena, stage_flush copies what was put
+  // since the reservation.
+  size_t stage_from = 0;
+  int stage_reserve(size_t bytes) {
+    if (round_up((int64_t)arena_off, 256) + bytes > arena_size) {
+      MT_HIP(hipStreamSynchronize(stream));
+      arena_off = 0;
+    }
+    stage_from = round_up((int64_t)arena_off, 256);
+    return kOk;
+  }
+  template <typename T>
+  const T* stage_put(const std::vector<T>& v) {
+    const size_t off = round_up((int64_t)arena_off, 256);
+    std::memcpy(arena_h + off, v.data(), v.size() * sizeof(T));
+    arena_off = off + v.size() * sizeof(T);
+    return reinterpret_cast<const T*>(arena_d + off);
+  }
+  int stage_flush() {
+    if (arena_off > stage_from)
+      MT_HIP(hipMemcpyAsync(arena_d + stage_from, arena_h + stage_from, arena_off - stage_from,
+                            hipMemcpyHostToDevice, stream));
     return kOk;
   }
 
@@ -964,12 +1064,22 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       d[k] = vecs[k]->op.vec;
       if (d[k].rkind != kRedNone && d[k].rkind != kRedWeightsAvg) vec_reduce = true;
     }
+    if (timing) MT_HIP(hipEventRecord(ev[0], stream));
     const VecDesc* dd = nullptr;
     int rc = stage_descs(d, &dd);
     if (rc) return rc;
     rc = launch_vec(dd, (int)d.size(), n, partials, res_vec, nopack, vec_reduce, stream);
     if (rc) return rc;
     need_sync |= vec_reduce;
+    if (timing) {
+      MT_HIP(hipEventRecord(ev[1], stream));
+      MT_HIP(hipEventSynchronize(ev[1]));
+      float ms = 0;
+      MT_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      vec_ms += ms;
+      vec_launches += 1;
+      for (const VecDesc& v : d) vec_bytes += vec_desc_bytes(v, kPackNone);
+    }
   }
   // right-hand sides: groups of up to 64 columns per demag pass
   bool pre_reduce_any = false, post_reduce_any = false;
@@ -1002,14 +1112,19 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     PackSpec pk;
     if (precision == MT_PREC_BF16X2) pk = PackSpec{kPackBf16, nullptr, 0, packb};
     else pk = PackSpec{kPackF32, packf, (int64_t)N3, nullptr};
-    const VecDesc* dpre = nullptr;
-    const LlgDesc* dl = nullptr;
-    int rc = stage_descs(pre, &dpre);
+    if (timing) MT_HIP(hipEventRecord(ev[2], stream));
+    // the group's three descriptor arrays go over in one copy
+    int rc = stage_reserve(3 * 256 + (size_t)S * (2 * sizeof(VecDesc) + sizeof(LlgDesc)));
+    if (rc) return rc;
+    const VecDesc* dpre = stage_put(pre);
+    const LlgDesc* dl = stage_put(ld);
+    const VecDesc* dpost = post_red ? stage_put(post) : nullptr;
+    rc = stage_flush();
     if (rc) return rc;
     rc = launch_vec(dpre, S, n, partials + (size_t)B * vec_blocks_per_state((int)N3) * 4, res_pre + g0 * 4, pk,
                     pre_red, stream);
     if (rc) return rc;
-    if (timing) MT_HIP(hipEventRecord(ev0, stream));
+    if (timing) MT_HIP(hipEventRecord(ev[3], stream));
     if (precision == MT_PREC_BF16X2) {
       if (Ktab)
         rc = launch_gemm_bf16x2_toeplitz(Ktab, nx, ny, nz, packb, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit,
@@ -1022,27 +1137,33 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       rc = launch_matvec_f32(K32, n, packf, (int64_t)N3, hbuf, (int64_t)N3, S, stream);
     }
     if (rc) return rc;
-    if (timing) {
-      MT_HIP(hipEventRecord(ev1, stream));
-    }
+    if (timing) MT_HIP(hipEventRecord(ev[4], stream));
     demag_passes += 1;
-    rc = stage_descs(ld, &dl);
-    if (rc) return rc;
     rc = launch_llg(grid, dl, S, stream);
     if (rc) return rc;
+    if (timing) MT_HIP(hipEventRecord(ev[5], stream));
     if (post_red) {
-      const VecDesc* dpost = nullptr;
-      rc = stage_descs(post, &dpost);
-      if (rc) return rc;
       rc = launch_vec(dpost, S, n, partials + (size_t)2 * B * vec_blocks_per_state((int)N3) * 4, res_post + g0 * 4,
                       nopack, true, stream);
       if (rc) return rc;
     }
     if (timing) {
-      MT_HIP(hipEventSynchronize(ev1));
-      float ms = 0;
-      MT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-      gemm_ms += ms;
+      MT_HIP(hipEventRecord(ev[6], stream));
+      MT_HIP(hipEventSynchronize(ev[6]));
+      float ms[4] = {0, 0, 0, 0};
+      MT_HIP(hipEventElapsedTime(&ms[0], ev[2], ev[3]));
+      MT_HIP(hipEventElapsedTime(&ms[1], ev[3], ev[4]));
+      MT_HIP(hipEventElapsedTime(&ms[2], ev[4], ev[5]));
+      MT_HIP(hipEventElapsedTime(&ms[3], ev[5], ev[6]));
+      gemm_ms += ms[1];
+      llg_ms += ms[2];
+      llg_launches += 1;
+      llg_bytes += llg_launch_bytes(ld);
+      vec_ms += ms[0] + (post_red ? ms[3] : 0.0f);
+      vec_launches += post_red ? 2 : 1;
+      for (const VecDesc& v : pre) vec_bytes += vec_desc_bytes(v, pk.mode);
+      if (post_red)
+        for (const VecDesc& v : post) vec_bytes += vec_desc_bytes(v, kPackNone);
     }
     pre_reduce_any |= pre_red;
     post_reduce_any |= post_red;
@@ -1220,14 +1341,14 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       MT_CHECK(hxs[r] >= 0 && hxs[r] <= hxe[r] && hxe[r] <= q->exch_nnz,
                "exchange operator: rows_start / rows_end out of range (1-based, within exch_nnz)");
     }
+    P.exch_nnz_host = q->exch_nnz;
     for (int k = 0; k < q->exch_nnz; ++k) {
       hxc[k] = q->exch_cols[k] - 1;
       MT_CHECK(hxc[k] >= 0 && hxc[k] < n, "exchange operator: column index out of range (1-based)");
     }
   }
   MT_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
-  MT_HIP(hipEventCreate(&P.ev0));
-  MT_HIP(hipEventCreate(&P.ev1));
+  for (hipEvent_t& e : P.ev) MT_HIP(hipEventCreate(&e));
   // tensor
   const size_t nn = (size_t)n * n;
   const bool need32 = P.precision == MT_PREC_FP32 || q->keep_fp32_tensor;
@@ -1500,6 +1621,12 @@ int mt_solver_results(mt_solver* h, double* t_out, double* y_out, double* fields
       o.ticks = P.ticks;
       o.demag_passes = P.demag_passes;
       o.demag_ms = P.gemm_ms;
+      o.vec_ms = P.vec_ms;
+      o.llg_ms = P.llg_ms;
+      o.vec_bytes = P.vec_bytes;
+      o.llg_bytes = P.llg_bytes;
+      o.vec_launches = P.vec_launches;
+      o.llg_launches = P.llg_launches;
     }
   }
   return kOk;
@@ -1519,7 +1646,8 @@ int mt_solver_state(mt_solver* h, double* y) {
 int mt_solver_set_timing(mt_solver* h, int on) {
   mt::Solver& P = *reinterpret_cast<mt::Solver*>(h);
   P.timing = on != 0;
-  P.gemm_ms = 0.0;
+  P.gemm_ms = P.vec_ms = P.llg_ms = P.vec_bytes = P.llg_bytes = 0.0;
+  P.vec_launches = P.llg_launches = 0;
   return 0;
 }
 

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -74,3 +76,33 @@ def test_status_codes_match_library():
     assert set(enum) == set(names)
     for k, v in enum.items():
         assert declared[names[k]] == v, k
+
+
+@pytest.mark.parametrize("cname,pyname", [("mt_micromag_problem", "MtProblem"), ("mt_stats", "MtStats"),
+                                          ("mt_tiles", "MtTiles")])
+def test_struct_layouts_match_header(tmp_path, cname, pyname):
+    """The ctypes mirrors have the size and field offsets the C compiler gives
+    the header's structs (fields in the same order under the same names)."""
+    import ctypes as C
+    import shutil
+    import subprocess
+
+    from magtense_amd import _lib
+
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    py = getattr(_lib, pyname)
+    names = [f[0] for f in py._fields_]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "magtense_hip.h"', "int main(void) {",
+             f'  printf("%zu\\n", sizeof({cname}));']
+    lines += [f'  printf("%zu\\n", offsetof({cname}, {nm}));' for nm in names]
+    lines += ["  return 0;", "}"]
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == C.sizeof(py), (cname, out[0], C.sizeof(py))
+    for nm, off in zip(names, out[1:]):
+        assert getattr(py, nm).offset == off, (cname, nm, off, getattr(py, nm).offset)
