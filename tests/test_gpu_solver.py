@@ -70,6 +70,36 @@ def test_batch_equals_individual_runs(oracle):
         assert np.abs(M_b[b] - yr).max() < 1e-4
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16x2"])
+def test_batch_beyond_one_demag_group(oracle, precision):
+    """70 states: every tick runs two demag passes (64 + 6 columns).  Each
+    state's column of the product, its LL kernel row and its step control are
+    independent of the others, so a state in the batch reproduces its own
+    one-state run bit for bit, on both sides of the split; in FP32 it also
+    follows the oracle run (split-BF16 against the oracle: the single-run tests
+    above, from starts whose trajectories do not amplify the 4e-6 field error)."""
+    from magtense_amd.micromag import MicromagProblem
+
+    res, L = [8, 4, 1], [40e-9, 20e-9, 3e-9]
+    n, B = 32, 70
+    m0 = np.stack([_random_m(n, 200 + b) for b in range(B)])
+    ang = np.linspace(0, 2 * np.pi, B, endpoint=False)
+    fields = np.stack([2e4 * np.cos(ang), 2e4 * np.sin(ang), np.zeros(B)], axis=1)
+    rows = np.repeat(fields[:, None, :], 10, axis=1)
+    p = MicromagProblem(res, L, alpha=4.42e3, gamma=2.21e5, tol=1e-6, precision=precision)
+    _, M_b = p.run_simulation_batch(5e-11, 6, rows, 10, m0)
+    assert M_b.shape == (B, 6, n, 3)
+    for b in (0, 63, 64, 69):
+        _, M_1 = p.run_simulation_batch(5e-11, 6, rows[b:b + 1], 10, m0[b:b + 1])
+        assert np.array_equal(M_b[b], M_1[0]), (precision, b, np.abs(M_b[b] - M_1[0]).max())
+        if precision == "fp32":
+            _, yr, _, _ = _oracle_run(oracle, res, L, m0[b], fields[b], 5e-11, 6, 10, alpha=4.42e3,
+                                      gamma=2.21e5, tol=1e-6)
+            err = np.abs(M_b[b] - yr).max()
+            print(f"state {b}: max |m - oracle| {err:.2e}")
+            assert err < 1e-4, (b, err)
+
+
 def test_explicit_solver_rows(oracle):
     """solver='explicit': each field row relaxes from the previous row's final state."""
     from magtense_amd.micromag import MicromagProblem
