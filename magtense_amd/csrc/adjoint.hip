@@ -24,6 +24,8 @@
 
 #include <cstring>
 #include <memory>
+#include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "mt_common.h"
@@ -210,6 +212,14 @@ struct Rk4 {
   uint8_t* desc_h = nullptr;  // pinned
   int ring_next = 0;
   size_t desc_size = 0;
+  // Descriptor cache: the fixed-step loops issue the same descriptor blocks
+  // step after step (same vectors, same coefficients), so each distinct block
+  // is copied to device memory once and reused; a copy is an SDMA transfer the
+  // compute queue waits for, about 20 us each.
+  static constexpr size_t kCacheBytes = (size_t)32 << 20;
+  uint8_t* dcache_d = nullptr;
+  size_t dcache_used = 0;
+  std::unordered_map<std::string, size_t> dcache;
   int nck = 0;
   bool forward_done = false;
 
@@ -219,7 +229,7 @@ struct Rk4 {
     };
     f(K32); f(K64); f(jfact); f(kfact); f(aniso); f(ms32); f(ms64); f(packd); f(hbuf); f(ckpt); f(seg); f(segS);
     f(work);
-    f(grad_ms); f(hext_part); f(desc_d);
+    f(grad_ms); f(hext_part); f(desc_d); f(dcache_d);
     if (desc_h) (void)hipHostFree(desc_h);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -227,12 +237,13 @@ struct Rk4 {
   size_t n3() const { return (size_t)3 * n; }
   void* hcol(void* base, int b) const { return static_cast<uint8_t*>(base) + (size_t)b * n3() * es; }
 
-  // hbuf = K * packd for all B states
-  int demag_pass() {
+  // out (default hbuf) = K * packd for all B states, columns n3 apart
+  int demag_pass(void* out = nullptr) {
+    if (!out) out = hbuf;
     if (fp64)
-      return launch_matvec_f64(K64, n, static_cast<const double*>(packd), (int64_t)n3(), static_cast<double*>(hbuf),
+      return launch_matvec_f64(K64, n, static_cast<const double*>(packd), (int64_t)n3(), static_cast<double*>(out),
                                (int64_t)n3(), B, stream);
-    return launch_matvec_f32(K32, n, static_cast<const float*>(packd), (int64_t)n3(), static_cast<float*>(hbuf),
+    return launch_matvec_f32(K32, n, static_cast<const float*>(packd), (int64_t)n3(), static_cast<float*>(out),
                              (int64_t)n3(), B, stream);
   }
   int nblocks() const { return (n + 255) / 256; }
@@ -247,19 +258,50 @@ struct Rk4 {
       set_error("descriptor block too large");
       return kErrArg;
     }
+    std::string key(reinterpret_cast<const char*>(v.data()), bytes);
+    auto hit = dcache.find(key);
+    if (hit != dcache.end()) {
+      *out = reinterpret_cast<const T*>(dcache_d + hit->second);
+      return kOk;
+    }
     if (ring_next == kSlots) {
       MT_HIP(hipStreamSynchronize(stream));
       ring_next = 0;
     }
     size_t off = (size_t)ring_next++ * kSlotBytes;
     std::memcpy(desc_h + off, v.data(), bytes);
+    const size_t cbytes = (bytes + 255) / 256 * 256;
+    if (dcache_d && dcache_used + cbytes <= kCacheBytes) {
+      // cached blocks are written once and only read afterwards
+      MT_HIP(hipMemcpyAsync(dcache_d + dcache_used, desc_h + off, bytes, hipMemcpyHostToDevice, stream));
+      *out = reinterpret_cast<const T*>(dcache_d + dcache_used);
+      dcache.emplace(std::move(key), dcache_used);
+      dcache_used += cbytes;
+      return kOk;
+    }
     MT_HIP(hipMemcpyAsync(desc_d + off, desc_h + off, bytes, hipMemcpyHostToDevice, stream));
     *out = reinterpret_cast<const T*>(desc_d + off);
     return kOk;
   }
 
-  // k[b] = f(Y[b]) for every state; S (K Y) is written to S_out[b] when given
-  int rhs(double* const* Y, double* const* k, void* const* S_out) {
+  // k[b] = f(Y[b]) for every state; S (K Y) is written to S_out[b] when given.
+  // packed: the FP32 demag input already holds Y (packed by the combination
+  // that formed it).
+  int rhs(double* const* Y, double* const* k, void* const* S_out, bool packed = false) {
+    int rc = packed && !fp64 ? kOk : pack_input(Y);
+    if (rc) return rc;
+    // recording: when the states' S_out columns are laid out like the demag
+    // buffer's, the product is written there directly instead of copied
+    bool direct = S_out != nullptr;
+    for (int b = 1; b < B && direct; ++b)
+      direct = static_cast<uint8_t*>(S_out[b]) == static_cast<uint8_t*>(S_out[0]) + (size_t)b * n3() * es;
+    void* out = direct ? S_out[0] : hbuf;
+    rc = demag_pass(out);
+    if (rc) return rc;
+    return llg(Y, k, direct ? nullptr : S_out, out);
+  }
+
+  int pack_input(double* const* Y) {
     std::vector<VecDesc> vd(B);
     for (int b = 0; b < B; ++b) {
       VecDesc& d = vd[b];
@@ -275,37 +317,39 @@ struct Rk4 {
     int rc = upload(vd, &dvd, 0);
     if (rc) return rc;
     PackSpec pk{fp64 ? kPackNone : kPackF32, fp64 ? nullptr : static_cast<float*>(packd), (int64_t)n3(), nullptr};
-    rc = launch_vec(dvd, B, n, nullptr, nullptr, pk, false, stream);
-    if (rc) return rc;
-    rc = demag_pass();
-    if (rc) return rc;
+    return launch_vec(dvd, B, n, nullptr, nullptr, pk, false, stream);
+  }
+
+  int llg(double* const* Y, double* const* k, void* const* S_out, void* sums) {
     std::vector<LlgDesc> ld(B);
     for (int b = 0; b < B; ++b) {
       LlgDesc& L = ld[b];
       std::memset(&L, 0, sizeof(L));
       L.m = Y[b];
       L.out = k[b];
-      if (fp64) L.hpart64 = static_cast<const double*>(hcol(hbuf, b));
-      else L.hpart = static_cast<const float*>(hcol(hbuf, b));
+      if (fp64) L.hpart64 = static_cast<const double*>(hcol(sums, b));
+      else L.hpart = static_cast<const float*>(hcol(sums, b));
       L.nsplit = 1;
       for (int c = 0; c < 3; ++c) L.hext[c] = hext[b * 3 + c];
       L.alpha = alpha;
       L.gamma = gamma;
     }
     const LlgDesc* dl = nullptr;
-    rc = upload(ld, &dl, 1);
+    int rc = upload(ld, &dl, 1);
     if (rc) return rc;
     rc = launch_llg(grid, dl, B, stream);
     if (rc) return rc;
     if (S_out)
       for (int b = 0; b < B; ++b)
-        MT_HIP(hipMemcpyAsync(S_out[b], hcol(hbuf, b), n3() * es, hipMemcpyDeviceToDevice, stream));
+        MT_HIP(hipMemcpyAsync(S_out[b], hcol(sums, b), n3() * es, hipMemcpyDeviceToDevice, stream));
     return kOk;
   }
 
-  // dst[b] = base[b] + sum_k c_k src_k[b]
+  // dst[b] = base[b] + sum_k c_k src_k[b]; pack: also write dst into the FP32
+  // demag input (not in the FP64 mode, whose input is a plain copy)
   int combine(double* const* dst, double* const* base, std::vector<std::pair<double* const*, double>> terms,
-              double bcoef = 1.0) {
+              double bcoef = 1.0, bool pack = false) {
+    pack = pack && !fp64;
     std::vector<VecDesc> vd(B);
     for (int b = 0; b < B; ++b) {
       VecDesc& d = vd[b];
@@ -314,7 +358,7 @@ struct Rk4 {
       d.base = base ? base[b] : nullptr;
       d.bcoef = bcoef;
       d.post_scale = 1.0;
-      d.pack_col = -1;
+      d.pack_col = pack ? b : -1;
       for (auto& t : terms) {
         d.src[d.nterm] = t.first[b];
         d.coef[d.nterm] = t.second;
@@ -324,8 +368,9 @@ struct Rk4 {
     const VecDesc* dvd = nullptr;
     int rc = upload(vd, &dvd, 2);
     if (rc) return rc;
-    PackSpec nop{kPackNone, nullptr, 0, nullptr};
-    return launch_vec(dvd, B, n, nullptr, nullptr, nop, false, stream);
+    PackSpec pk{pack ? kPackF32 : kPackNone, pack ? static_cast<float*>(packd) : nullptr, pack ? (int64_t)n3() : 0,
+                nullptr};
+    return launch_vec(dvd, B, n, nullptr, nullptr, pk, false, stream);
   }
 
   // pointer tables
@@ -343,7 +388,7 @@ struct Rk4 {
   int step(double* const* y, double* const* ynext, double* const* Yst /*4*B*/, void* const* Sst /*4*B*/) {
     const size_t N3 = n3();
     auto W = [&](int k) { return vecs(work + (size_t)k * B * N3, N3); };
-    auto k1 = W(0), k2 = W(1), k3 = W(2), k4 = W(3), tmp = W(4);
+    auto k1 = W(0), k2 = W(1), k3 = W(2), k4 = W(3);
     std::vector<double*> Y1(y, y + B), Y2 = Yst ? std::vector<double*>(Yst + B, Yst + 2 * B) : W(5),
                                        Y3 = Yst ? std::vector<double*>(Yst + 2 * B, Yst + 3 * B) : W(6),
                                        Y4 = Yst ? std::vector<double*>(Yst + 3 * B, Yst + 4 * B) : W(7);
@@ -354,23 +399,23 @@ struct Rk4 {
     }
     rc = rhs(Y1.data(), k1.data(), Sst ? Sst : nullptr);
     if (rc) return rc;
-    rc = combine(Y2.data(), Y1.data(), {{k1.data(), 0.5 * dt}});
+    // each stage input is packed for its demag pass by the combination that forms it
+    rc = combine(Y2.data(), Y1.data(), {{k1.data(), 0.5 * dt}}, 1.0, true);
     if (rc) return rc;
-    rc = rhs(Y2.data(), k2.data(), Sst ? Sst + B : nullptr);
+    rc = rhs(Y2.data(), k2.data(), Sst ? Sst + B : nullptr, true);
     if (rc) return rc;
-    rc = combine(Y3.data(), Y1.data(), {{k2.data(), 0.5 * dt}});
+    rc = combine(Y3.data(), Y1.data(), {{k2.data(), 0.5 * dt}}, 1.0, true);
     if (rc) return rc;
-    rc = rhs(Y3.data(), k3.data(), Sst ? Sst + 2 * B : nullptr);
+    rc = rhs(Y3.data(), k3.data(), Sst ? Sst + 2 * B : nullptr, true);
     if (rc) return rc;
-    rc = combine(Y4.data(), Y1.data(), {{k3.data(), dt}});
+    rc = combine(Y4.data(), Y1.data(), {{k3.data(), dt}}, 1.0, true);
     if (rc) return rc;
-    rc = rhs(Y4.data(), k4.data(), Sst ? Sst + 3 * B : nullptr);
+    rc = rhs(Y4.data(), k4.data(), Sst ? Sst + 3 * B : nullptr, true);
     if (rc) return rc;
-    // y' = y + h/6 (k1 + 2 k2 + 2 k3 + k4)
-    rc = combine(tmp.data(), Y1.data(),
-                 {{k1.data(), dt / 6.0}, {k2.data(), dt / 3.0}, {k3.data(), dt / 3.0}, {k4.data(), dt / 6.0}});
-    if (rc) return rc;
-    return combine(ynext, tmp.data(), {});
+    // y' = y + h/6 (k1 + 2 k2 + 2 k3 + k4); the kernel reads every operand
+    // element before writing it, so ynext may be y itself
+    return combine(ynext, Y1.data(),
+                   {{k1.data(), dt / 6.0}, {k2.data(), dt / 3.0}, {k3.data(), dt / 3.0}, {k4.data(), dt / 6.0}});
   }
 
   int forward(const double* m0_host, double* mT_host) {
@@ -648,6 +693,7 @@ int mt_rk4_create(const mt_micromag_problem* q, double dt, long long nsteps, int
   P.desc_size = Rk4::kSlotBytes * Rk4::kSlots;
   MT_HIP(hipMalloc(&P.desc_d, P.desc_size));
   MT_HIP(hipHostMalloc(&P.desc_h, P.desc_size, hipHostMallocDefault));
+  MT_HIP(hipMalloc(&P.dcache_d, Rk4::kCacheBytes));
   MT_CHECK(P.B * sizeof(VecDesc) <= Rk4::kSlotBytes, "too many states for one descriptor block");
   MT_HIP(hipStreamSynchronize(P.stream));
   *out = reinterpret_cast<mt_rk4*>(R.release());

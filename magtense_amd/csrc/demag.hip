@@ -33,6 +33,12 @@ namespace {
 // ---------------------------------------------------------------------------
 // Plain FP32 per-state mat-vec: block = RB receiver rows of one state.
 // ---------------------------------------------------------------------------
+__device__ inline float4 ldnt4(const float* p) {
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 template <int RB, bool VEC4>
 __global__ void __launch_bounds__(256) matvec_f32_kernel(const float* __restrict__ K, int n,
                                                          const float* __restrict__ m, int64_t m_stride,
@@ -55,13 +61,15 @@ __global__ void __launch_bounds__(256) matvec_f32_kernel(const float* __restrict
       for (int r = 0; r < RB; ++r) {
         int i = i0 + r;
         if (i >= n) break;
+        // the tensor is streamed once per state: non-temporal loads, so that it
+        // does not displace the state vectors from L2
         const float* row = K + (size_t)i * n + j;
-        float4 kxx = *reinterpret_cast<const float4*>(row);
-        float4 kxy = *reinterpret_cast<const float4*>(row + nn);
-        float4 kxz = *reinterpret_cast<const float4*>(row + 2 * nn);
-        float4 kyy = *reinterpret_cast<const float4*>(row + 3 * nn);
-        float4 kyz = *reinterpret_cast<const float4*>(row + 4 * nn);
-        float4 kzz = *reinterpret_cast<const float4*>(row + 5 * nn);
+        float4 kxx = ldnt4(row);
+        float4 kxy = ldnt4(row + nn);
+        float4 kxz = ldnt4(row + 2 * nn);
+        float4 kyy = ldnt4(row + 3 * nn);
+        float4 kyz = ldnt4(row + 4 * nn);
+        float4 kzz = ldnt4(row + 5 * nn);
         acc[r][0] += kxx.x * vx.x + kxy.x * vy.x + kxz.x * vz.x + kxx.y * vx.y + kxy.y * vy.y + kxz.y * vz.y +
                      kxx.z * vx.z + kxy.z * vy.z + kxz.z * vz.z + kxx.w * vx.w + kxy.w * vy.w + kxz.w * vz.w;
         acc[r][1] += kxy.x * vx.x + kyy.x * vy.x + kyz.x * vz.x + kxy.y * vx.y + kyy.y * vy.y + kyz.y * vz.y +
