@@ -271,7 +271,7 @@ int apply_bf16x2(const void* K, int n, int nrows, int nx, int ny, int nz, const 
   int rc = kOk;
   const int64_t n3 = (int64_t)3 * n;
   const int64_t r3 = (int64_t)3 * nrows;
-  int ks = nx > 0 ? gemm_ksplit_for(n, 2) : gemm_ksplit_dense(n, nrows);
+  int ks = nx > 0 ? gemm_ksplit_table(nx, ny, nz) : gemm_ksplit_dense(n, nrows);
   DevArr<uint8_t> P8;
   DevArr<float> PF;
   MT_HIP(P8.alloc((size_t)packed_m_bytes(n)));

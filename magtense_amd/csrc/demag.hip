@@ -589,8 +589,9 @@ int launch_matvec_f32(const float* K, int n, const float* m, int64_t m_stride, f
 // Kernel choice for measurements (MT_GEMM_VARIANT): 0 (default) picks the
 // fastest measured kernel for each storage.  Dense tensor: 0 = register-ring
 // kernel with 4 chunks in flight, 6 = two-tile kernel with a four-stage LDS
-// DMA ring.  Table: 0 = four-tile single-stage LDS kernel, 2 = two-tile
-// four-stage kernel, 4 = register-ring kernel.
+// DMA ring.  Table: 0 = entry-reuse kernel where nx % 128 == 0, else the
+// four-tile single-stage LDS kernel, 3 = that four-tile kernel everywhere,
+// 2 = two-tile four-stage kernel, 4 = register-ring kernel.
 // Read at every launch (a getenv per GEMM), so that one process can compare
 // the kernels.
 int gemm_variant() {
@@ -820,6 +821,161 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __
   }
 }
 
+// Entry-reuse kernel for the translation-invariant table (nx % 128 == 0):
+// a workgroup of four waves owns four x-consecutive 32-row tiles of one grid
+// row (x tiles tx0 .. tx0+3) and walks the 16-column chunks of each source row
+// in x order.  Wave w at chunk x1c needs the table entry (12 fragments) of x
+// offset e = 2w - x1c + cpr - 1 (relative to 2 tx0), so over a source row the
+// four waves need cpr + 6 distinct entries where the four-tile kernel loads
+// 4 cpr.  Entries live in an eight-slot LDS ring (slot e mod 8): a step brings
+// one new entry (four at the first two chunks of a row) and one state chunk,
+// DMA'd one step ahead so that the transfer runs under the current step's
+// MFMAs; one barrier per step.  Slot safety (cpr % 8 == 0): the entries of
+// steps g and g+1 are eight consecutive values of e at most, so the DMA for
+// step g+1 (issued after the barrier that opens step g) writes only slots that
+// step g does not read, and the slots it overwrites were last read at step g-1,
+// whose reads every wave finished before that barrier.
+__global__ void __launch_bounds__(256, 1) gemm_bf16x2_tw_kernel(const uint4* __restrict__ Kt,
+                                                                const uint4* __restrict__ Mp, int ksplit,
+                                                                float* __restrict__ out, int n, int64_t col_stride,
+                                                                int64_t split_stride, int ncols, int gnx, int gny,
+                                                                int gnz) {
+  __shared__ uint4 ent[8][12][64];  // 96 KiB: table entries, slot e mod 8
+  __shared__ uint4 bst[2][12][64];  // 24 KiB: state chunks, slot g mod 2
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int cpr = gnx / 16, tpr = gnx / 32, U = gnx / 8 - 2;
+  const int rows = gny * gnz;
+  const int groups = n / kTileM / 4;
+  // XCD-aware order as in the four-tile kernel (block b runs on XCD b % 8)
+  int ks, group;
+  if (ksplit <= 8 && 8 % ksplit == 0) {
+    const int x = (int)(blockIdx.x % 8);
+    ks = x % ksplit;
+    group = (int)(blockIdx.x / 8) * (8 / ksplit) + x / ksplit;
+  } else {
+    const int per_xcd = (groups * ksplit + 7) / 8;
+    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    ks = lin % ksplit;
+    group = lin < groups * ksplit ? lin / ksplit : groups;
+  }
+  if (group >= groups) return;  // whole workgroup, before any barrier
+  const int sb0 = rows * ks / ksplit, sb1 = rows * (ks + 1) / ksplit;  // source rows of this split
+  const int I0 = group * 4;
+  const int irow = I0 / tpr;
+  const int tx0 = I0 - irow * tpr;
+  const int tiy = irow % gny, tiz = irow / gny;
+  const int G = (sb1 - sb0) * cpr;
+  // source row s (visited by cell offset, as in the four-tile kernel) ->
+  // source cell row (jy, jz)
+  auto src_row = [&](int s, int& jy, int& jz) {
+    const int r = s % gny, sp = s / gny;
+    jy = (tiy - r + gny) % gny;
+    jz = (tiz - sp + gnz) % gnz;
+  };
+  auto issue = [&](int g) {
+    const int s = sb0 + g / cpr, x1c = g % cpr;
+    int jy, jz;
+    src_row(s, jy, jz);
+    const int64_t slab = ((int64_t)(tiz - jz + gnz - 1) * (2 * gny - 1) + (tiy - jy + gny - 1)) * U + 2 * tx0;
+    const int e0 = cpr - 1 - x1c;    // the step's lowest entry
+    const int ne = x1c < 2 ? 4 : 1;  // entries this step is the first to use
+    for (int k = 0; k < ne; ++k) {
+      const int e = e0 + 2 * k;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int f = wave * 3 + q;
+        lds_dma16(Kt + ((slab + e) * 12 + f) * 64 + lane, lds_addr(&ent[e & 7][f][0]));
+      }
+    }
+    const int64_t J = ((int64_t)jz * gny + jy) * cpr + x1c;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int f = wave * 3 + q;
+      lds_dma16(Mp + (J * 12 + f) * 64 + lane, lds_addr(&bst[g & 1][f][0]));
+    }
+  };
+  auto fetch = [&](int g, uint4 (&ra)[12], uint4 (&rb)[12]) {
+    const int slot = (cpr - 1 - g % cpr + 2 * wave) & 7;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) ra[q] = ent[slot][q][lane];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) rb[q] = bst[g & 1][q][lane];
+  };
+  floatx16 acc[2][3];
+#pragma unroll
+  for (int sb = 0; sb < 2; ++sb)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[sb][a][r] = 0.0f;
+  auto mfmas = [&](const uint4 (&ra)[12], const uint4 (&rb)[12]) {
+    const int comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+    for (int beta = 0; beta < 3; ++beta)
+#pragma unroll
+      for (int t = 0; t < 3; ++t)  // hi*hi, hi*lo, lo*hi
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int sb = 0; sb < 2; ++sb) {
+            const int c = comp[a][beta];
+            const bf16x8 kf = as_bf16x8(ra[2 * c + (t == 2 ? 1 : 0)]);
+            const bf16x8 mf = as_bf16x8(rb[(beta * 2 + sb) * 2 + (t == 1 ? 1 : 0)]);
+            acc[sb][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kf, mf, acc[sb][a], 0, 0, 0);
+          }
+  };
+  // step g: wait for my step-g DMA and my step-(g-1) LDS reads, then for
+  // everyone's; DMA step g+1; read step g's fragments and run its MFMAs (the
+  // compiler waits for each fragment just before its first use)
+  uint4 ra[12], rb[12];
+  if (G > 0) {
+    issue(0);
+    for (int g = 0; g < G; ++g) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+      if (g + 1 < G) issue(g + 1);
+      fetch(g, ra, rb);
+      mfmas(ra, rb);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // no LDS write may outlive the workgroup
+  }
+  float* o = out + (int64_t)ks * split_stride;
+  const int64_t I = I0 + wave;
+#pragma unroll
+  for (int sb = 0; sb < 2; ++sb) {
+    const int col = 32 * sb + (lane & 31);
+    if (col >= ncols) continue;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+        o[col * col_stride + (int64_t)a * n + row] = acc[sb][a][r];
+      }
+  }
+}
+
+// The entry-reuse kernel applies to grids with nx % 128 == 0 (four x tiles per
+// workgroup in one grid row, cpr % 8 == 0) and k-splits that divide the source
+// rows.
+bool table_window_ok(int nx, int ny, int nz, int ksplit) {
+  return nx % 128 == 0 && ny >= 1 && nz >= 1 && ksplit >= 1 && (ny * nz) % ksplit == 0 && gemm_variant() == 0;
+}
+
+int gemm_ksplit_table(int nx, int ny, int nz) {
+  const int n = nx * ny * nz;
+  if (nx % 128 != 0 || gemm_variant() != 0) return gemm_ksplit_for(n, 2);
+  // one resident workgroup per CU: the smallest k-split giving every CU a
+  // workgroup (1 at 256 x 64 x 2), within the divisors of the source rows
+  const int rows = ny * nz, groups = n / kTileM / 4;
+  int ks = 1;
+  while (groups * ks < 256 && rows % (ks * 2) == 0) ks *= 2;
+  if (forced_ksplit() > 0 && rows % forced_ksplit() == 0) ks = forced_ksplit();
+  return ks;
+}
+
 int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, const uint8_t* Mpack, float* out,
                                 int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
   if (nx % 32 != 0 || nx < 32 || ny < 1 || nz < 1) {
@@ -827,6 +983,15 @@ int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, con
     return kErrArg;
   }
   const int n = nx * ny * nz;
+  if (table_window_ok(nx, ny, nz, ksplit)) {
+    const int groups = n / kTileM / 4;
+    const int blocks = (groups * ksplit + 7) / 8 * 8;
+    hipLaunchKernelGGL(gemm_bf16x2_tw_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4*>(Ktab), reinterpret_cast<const uint4*>(Mpack), ksplit, out, n,
+                       col_stride, split_stride, ncols, nx, ny, nz);
+    MT_HIP(hipGetLastError());
+    return kOk;
+  }
   if (gemm_variant() == 4)
     return launch_reg<4, true>(Ktab, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream, nx, ny, nz);
   if (gemm_variant() != 2) {

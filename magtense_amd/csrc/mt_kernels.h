@@ -39,6 +39,7 @@ int launch_matvec_f64(const double* K, int n, const double* m, int64_t m_stride,
                       int nstates, hipStream_t stream);
 int gemm_ksplit_for(int n, int tiles_per_group);  // k-split of the batched GEMM (two-tile groups)
 int gemm_ksplit_rows(int n, int nrows, int tiles_per_group);  // same for nrows receiver rows of n sources
+int gemm_ksplit_table(int nx, int ny, int nz);  // k-split of the kernel launch_gemm_bf16x2_toeplitz picks
 int gemm_ksplit_dense(int n, int nrows);  // k-split of the dense-tensor kernel launch_gemm_bf16x2{,_rows} picks
 // out[ks*split_stride + col*col_stride + c*n + i] = k-split partial of K m
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,

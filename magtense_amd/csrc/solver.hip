@@ -1510,7 +1510,7 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   }
   P.hcols = std::min(P.B, kMaxCols);
   if (P.precision == MT_PREC_BF16X2) {
-    P.ksplit = q->ksplit > 0 ? q->ksplit : (P.Ktab ? gemm_ksplit_for(n, 2) : gemm_ksplit_dense(n, n));
+    P.ksplit = q->ksplit > 0 ? q->ksplit : (P.Ktab ? gemm_ksplit_table(P.nx, P.ny, P.nz) : gemm_ksplit_dense(n, n));
     MT_HIP(hipMalloc(&P.packb, (size_t)packed_m_bytes(n)));
     MT_HIP(hipMemsetAsync(P.packb, 0, (size_t)packed_m_bytes(n), P.stream));
     MT_HIP(hipMalloc(&P.hbuf, (size_t)P.ksplit * P.hcols * n3 * sizeof(float)));

@@ -4,7 +4,8 @@
 * Dense tensor: variant 6 (two-tile kernel, four-stage LDS DMA ring) on the
   whole tensor and on row blocks, n = 540 (the last tile holds 28 rows).
 * Table: variant 2 (two-tile four-stage kernel) and variant 4 (register-ring
-  kernel) on a 64 x 4 x 3 grid.
+  kernel) on a 64 x 4 x 3 grid; variant 3 (the four-tile kernel) against the
+  default entry-reuse kernel on 128 x 4 x 2 and 256 x 2 x 3 grids.
 
 Each is within 1e-6 relative (norm-wise; the kernels differ in k-split and
 k-chunk order, hence in FP32 summation order) of the default kernel and within
@@ -85,11 +86,10 @@ def test_dense_lds_ring_variant(oracle, monkeypatch):
             assert e_base < 1e-6 and e_64 < 1e-5, (S, r0, nr, e_base, e_64)
 
 
-@pytest.mark.parametrize("variant", ["2", "4"])
-def test_table_variants(oracle, monkeypatch, variant):
+@pytest.mark.parametrize("variant,res", [("2", [64, 4, 3]), ("4", [64, 4, 3]), ("3", [128, 4, 2]), ("3", [256, 2, 3])])
+def test_table_variants(oracle, monkeypatch, variant, res):
     import torch
 
-    res = [64, 4, 3]
     Lb, n, _, tab, K = _tensors(oracle, res, table=True)
     for S in (1, 5, 64):
         m = np.random.default_rng(60 + S).normal(size=(S, 3, n)).astype(np.float32)

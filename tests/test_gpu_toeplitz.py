@@ -1,7 +1,9 @@
 """The translation-invariant split-BF16 table (uniform grids, nx % 32 == 0)
 against the dense fragment-tiled tensor and the FP64 product.
 
-The table holds the same FP32 values split the same way; the products differ
+Grids with nx % 128 == 0 take the entry-reuse kernel (k-split 1 or 2 here),
+the others the four-tile kernel.  The table holds the same FP32 values split
+the same way; the products differ
 only by the order in which k-chunks are summed (the table kernel visits them by
 cell offset) and by ulp-level FP64 differences in the offsets (bar 1e-6
 relative, norm-wise); both stay within 1e-5 of the FP64 product.  The solver
@@ -28,7 +30,7 @@ def _fp64_product(K, m):
     return out
 
 
-@pytest.mark.parametrize("res", [[32, 8, 1], [64, 4, 3], [32, 2, 2], [96, 3, 1]])
+@pytest.mark.parametrize("res", [[32, 8, 1], [64, 4, 3], [32, 2, 2], [96, 3, 1], [128, 4, 2], [256, 3, 3], [128, 2, 3]])
 def test_table_matches_dense_split(oracle, res):
     import torch
     from magtense_amd import _lib
