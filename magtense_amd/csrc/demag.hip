@@ -873,30 +873,42 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_tw_kernel(const uint4* __r
     jy = (tiy - r + gny) % gny;
     jz = (tiz - sp + gnz) % gnz;
   };
-  auto issue = [&](int g) {
-    const int s = sb0 + g / cpr, x1c = g % cpr;
+  // DMA of the next step to issue (source row is, chunk ix): the row's slab
+  // and chunk base change only at row starts, so the divisions stay off the
+  // per-step path
+  int is = sb0, ix = 0;
+  int64_t islab = 0, iJ0 = 0;
+  auto row_setup = [&]() {
     int jy, jz;
-    src_row(s, jy, jz);
-    const int64_t slab = ((int64_t)(tiz - jz + gnz - 1) * (2 * gny - 1) + (tiy - jy + gny - 1)) * U + 2 * tx0;
-    const int e0 = cpr - 1 - x1c;    // the step's lowest entry
-    const int ne = x1c < 2 ? 4 : 1;  // entries this step is the first to use
+    src_row(is, jy, jz);
+    islab = ((int64_t)(tiz - jz + gnz - 1) * (2 * gny - 1) + (tiy - jy + gny - 1)) * U + 2 * tx0;
+    iJ0 = ((int64_t)jz * gny + jy) * cpr;
+  };
+  row_setup();
+  auto issue = [&](int g) {  // g: the step number of (is, ix)
+    const int e0 = cpr - 1 - ix;    // the step's lowest entry
+    const int ne = ix < 2 ? 4 : 1;  // entries this step is the first to use
     for (int k = 0; k < ne; ++k) {
       const int e = e0 + 2 * k;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         const int f = wave * 3 + q;
-        lds_dma16(Kt + ((slab + e) * 12 + f) * 64 + lane, lds_addr(&ent[e & 7][f][0]));
+        lds_dma16(Kt + ((islab + e) * 12 + f) * 64 + lane, lds_addr(&ent[e & 7][f][0]));
       }
     }
-    const int64_t J = ((int64_t)jz * gny + jy) * cpr + x1c;
+    const int64_t J = iJ0 + ix;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const int f = wave * 3 + q;
       lds_dma16(Mp + (J * 12 + f) * 64 + lane, lds_addr(&bst[g & 1][f][0]));
     }
+    if (++ix == cpr) {
+      ix = 0;
+      if (++is < sb1) row_setup();
+    }
   };
-  auto fetch = [&](int g, uint4 (&ra)[12], uint4 (&rb)[12]) {
-    const int slot = (cpr - 1 - g % cpr + 2 * wave) & 7;
+  auto fetch = [&](int x1c, int g, uint4 (&ra)[12], uint4 (&rb)[12]) {
+    const int slot = (cpr - 1 - x1c + 2 * wave) & 7;
 #pragma unroll
     for (int q = 0; q < 12; ++q) ra[q] = ent[slot][q][lane];
 #pragma unroll
@@ -926,18 +938,21 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_tw_kernel(const uint4* __r
           }
   };
   // step g: wait for my step-g DMA and my step-(g-1) LDS reads, then for
-  // everyone's; DMA step g+1; read step g's fragments and run its MFMAs (the
-  // compiler waits for each fragment just before its first use)
+  // everyone's; read step g's fragments, DMA step g+1 (into slots step g does
+  // not read) and run step g's MFMAs (the compiler waits for each fragment
+  // just before its first use)
   uint4 ra[12], rb[12];
   if (G > 0) {
     issue(0);
+    int x1c = 0;  // chunk of step g within its source row
     for (int g = 0; g < G; ++g) {
       __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_s_barrier();
       asm volatile("" ::: "memory");
+      fetch(x1c, g, ra, rb);
       if (g + 1 < G) issue(g + 1);
-      fetch(g, ra, rb);
       mfmas(ra, rb);
+      if (++x1c == cpr) x1c = 0;
     }
     __builtin_amdgcn_s_waitcnt(0);  // no LDS write may outlive the workgroup
   }
