@@ -459,7 +459,7 @@ def main():
         del keept
         tms = sb[0].demag_ms / max(sb[0].demag_passes - sa[0].demag_passes, 1)
         tf = alg_flops / (tms * 1e-3) / 1e12
-        table = dict(steps_per_s=bst / wallt, steps=args.table_steps, gemm_ms=tms, kernel="gemm_bf16x2_t4s_kernel<true>",
+        table = dict(steps_per_s=bst / wallt, steps=args.table_steps, gemm_ms=tms, kernel="gemm_bf16x2_tw_kernel",
                      roofline={"bound": "mfma", "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                                "frac": tf / BF16_PEAK_TFLOPS, "mfma_issued_frac": 3.0 * tf / BF16_PEAK_TFLOPS,
                                "traffic": traffic_from_profile("table")},
@@ -507,7 +507,7 @@ def main():
         achieved = alg_flops / (gemm_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic_from_profile("table"),
-                    "kernel": "gemm_bf16x2_t4s_kernel<true>", "kernel_ms": gemm_ms,
+                    "kernel": "gemm_bf16x2_tw_kernel", "kernel_ms": gemm_ms,
                     "alg_flops_per_launch": alg_flops,
                     "mfma_issued_frac": 3.0 * achieved / BF16_PEAK_TFLOPS,
                     "note": "split-BF16 issues 3 MFMA products per algorithmic flop (hi*hi + hi*lo + lo*hi)"}
