@@ -672,7 +672,6 @@ int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float
 // interleave, so one's MFMAs cover the other's LDS reads.  (The table is
 // L2-resident, so the kernel is bound by the MFMA and LDS pipes and the
 // L2 -> LDS transfer rather than by HBM.)
-template <bool TOEP>
 __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __restrict__ Kt,
                                                                 const uint4* __restrict__ Mp, int64_t np,
                                                                 int ksplit, float* __restrict__ out, int n,
@@ -685,12 +684,12 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __
   const int nJ = (int)(np / kTileK);
   const int n_row_tiles = (int)(np / kTileM);
   const int groups = (n_row_tiles + 3) / 4;
-  // XCD-aware order (block b runs on XCD b % 8).  With the table and a k-split
-  // dividing 8, XCD x takes k-split x % ksplit for every group: its resident
-  // workgroups then walk the same offset sequence (below) and share the table
-  // slabs in its L2.  Otherwise each XCD takes a contiguous range of work.
+  // XCD-aware order (block b runs on XCD b % 8).  With a k-split dividing 8,
+  // XCD x takes k-split x % ksplit for every group: its resident workgroups
+  // then walk the same offset sequence (below) and share the table slabs in
+  // its L2.  Otherwise each XCD takes a contiguous range of work.
   int ks, group;
-  const bool by_ks = TOEP && ksplit <= 8 && 8 % ksplit == 0;
+  const bool by_ks = ksplit <= 8 && 8 % ksplit == 0;
   if (by_ks) {
     const int xpk = 8 / ksplit;  // XCDs per k-split
     const int x = (int)(blockIdx.x % 8);
@@ -712,7 +711,7 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __
   const int64_t Il = tile_ok ? I : 0;  // loads stay uniform; never consumed
   const int cpr = gnx / 16, tpr = gnx / 32, U = gnx / 8 - 2;
   int tx0 = 0, tiy = 0, tiz = 0, wy = 0, wz = 0;
-  if (TOEP) {
+  {
     const int irow = (int)(Il / tpr);
     tx0 = (int)(Il - (int64_t)irow * tpr);
     tiy = irow % gny;
@@ -723,13 +722,12 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __
     wy = wrow % gny;
     wz = wrow / gny;
   }
-  // Chunk v of the k range -> k-chunk J.  With the table the k-chunks are
-  // visited by cell offset: v = ((s ny + r) cpr + x1c) takes source row
-  // jy = (wy - r) mod ny in plane jz = (wz - s) mod nz, so every workgroup
-  // with the same k-split reads the same (dy, dz) slab at the same step.
-  // For each tile this is a permutation of the k-chunks.
+  // Chunk v of the k range -> k-chunk J.  The k-chunks are visited by cell
+  // offset: v = ((s ny + r) cpr + x1c) takes source row jy = (wy - r) mod ny
+  // in plane jz = (wz - s) mod nz, so every workgroup with the same k-split
+  // reads the same (dy, dz) slab at the same step.  For each tile this is a
+  // permutation of the k-chunks.
   auto chunk_of = [&](int v) {
-    if (!TOEP) return v;
     const int x1c = v % cpr;
     const int rs = v / cpr;
     const int r = rs % gny, sp = rs / gny;
@@ -738,22 +736,13 @@ __global__ void __launch_bounds__(256, 2) gemm_bf16x2_t4s_kernel(const uint4* __
   };
   auto issue = [&](int v, int slot) {
     const int J = chunk_of(v);
-    int64_t base;
-    if (TOEP) {
-      const int jrow = J / cpr;
-      const int x1c = J - jrow * cpr;
-      const int jy = jrow % gny, jz = jrow / gny;
-      base = (((int64_t)(tiz - jz + gnz - 1) * (2 * gny - 1) + (tiy - jy + gny - 1)) * U + (2 * tx0 - x1c + cpr - 1)) * 12;
-    } else {
-      base = (Il * nJ + J) * 12;
-    }
+    const int jrow = J / cpr;
+    const int x1c = J - jrow * cpr;
+    const int jy = jrow % gny, jz = jrow / gny;
+    const int64_t base =
+        (((int64_t)(tiz - jz + gnz - 1) * (2 * gny - 1) + (tiy - jy + gny - 1)) * U + (2 * tx0 - x1c + cpr - 1)) * 12;
 #pragma unroll
-    for (int q = 0; q < 12; ++q) {
-      if (TOEP)
-        lds_dma16(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
-      else  // the dense tensor is read once per pass
-        lds_dma16_nt(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
-    }
+    for (int q = 0; q < 12; ++q) lds_dma16(Kt + (base + q) * 64 + lane, lds_addr(&smem[slot][wave * 12 + q][0]));
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const int f = wave * 3 + q;  // 0..11
@@ -1015,7 +1004,7 @@ int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, con
     // it and 3.50 ms for the two-tile four-stage kernel)
     const int groups4 = (n / kTileM + 3) / 4;
     const int blocks4 = (groups4 * ksplit + 7) / 8 * 8;
-    hipLaunchKernelGGL((gemm_bf16x2_t4s_kernel<true>), dim3((unsigned)blocks4), dim3(256), 0, stream,
+    hipLaunchKernelGGL(gemm_bf16x2_t4s_kernel, dim3((unsigned)blocks4), dim3(256), 0, stream,
                        reinterpret_cast<const uint4*>(Ktab), reinterpret_cast<const uint4*>(Mpack), (int64_t)n,
                        ksplit, out, n, col_stride, split_stride, ncols, nx, ny, nz);
     MT_HIP(hipGetLastError());
