@@ -159,6 +159,26 @@ __device__ inline void load_e(const double* p, int e0, int n3, double (&x)[kVecE
   }
 }
 
+// acc[q] += coef(k) * ptr(k)[e_q] for k = 0 .. cnt-1, in k order.  Operand
+// k+1's loads are issued before operand k is accumulated, so two operands'
+// loads are in flight; the last iteration reloads its own operand (cached) to
+// keep the loop body free of branches, so the wait before each accumulation
+// leaves exactly the next operand's eight loads outstanding.
+template <typename PF, typename CF>
+__device__ inline void accumulate(double (&acc)[kVecE], int cnt, PF ptr, CF coef, int e0, int n3) {
+  if (cnt <= 0) return;
+  double x[kVecE], y[kVecE];
+  load_e(ptr(0), e0, n3, x);
+  for (int k = 0; k < cnt; ++k) {
+    load_e(ptr(k + 1 < cnt ? k + 1 : k), e0, n3, y);
+    const double c = coef(k);
+#pragma unroll
+    for (int q = 0; q < kVecE; ++q) acc[q] += c * x[q];
+#pragma unroll
+    for (int q = 0; q < kVecE; ++q) x[q] = y[q];
+  }
+}
+
 __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ descs, int n, int n3,
                                                   int blocks_per_state, double* __restrict__ partials,
                                                   PackSpec pack) {
@@ -178,18 +198,11 @@ __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ de
   if (D.dst) {
 #pragma unroll
     for (int q = 0; q < kVecE; ++q) v[q] = D.bconst;
-    if (D.base) {
-      const double bc = D.bcoef;
-      load_e(D.base, e0, n3, x);
-#pragma unroll
-      for (int q = 0; q < kVecE; ++q) v[q] += bc * x[q];
-    }
-    for (int k = 0; k < D.nterm; ++k) {
-      const double c = D.coef[k];
-      load_e(D.src[k], e0, n3, x);
-#pragma unroll
-      for (int q = 0; q < kVecE; ++q) v[q] += c * x[q];
-    }
+    // base first, then the terms in order
+    const int nb = D.base ? 1 : 0;
+    accumulate(
+        v, nb + D.nterm, [&](int k) { return k < nb ? D.base : D.src[k - nb]; },
+        [&](int k) { return k < nb ? D.bcoef : D.coef[k - nb]; }, e0, n3);
     const double ps = D.post_scale;
 #pragma unroll
     for (int q = 0; q < kVecE; ++q) {
@@ -215,12 +228,8 @@ __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ de
       double s[kVecE], a[kVecE];
 #pragma unroll
       for (int q = 0; q < kVecE; ++q) s[q] = 0.0;
-      for (int k = 0; k < D.rn; ++k) {
-        const double c = D.rcoef[k];
-        load_e(D.rsrc[k], e0, n3, x);
-#pragma unroll
-        for (int q = 0; q < kVecE; ++q) s[q] += c * x[q];
-      }
+      accumulate(
+          s, D.rn, [&](int k) { return D.rsrc[k]; }, [&](int k) { return D.rcoef[k]; }, e0, n3);
       load_e(D.ra, e0, n3, a);
       load_e(D.rb, e0, n3, x);
       const double th = D.thresh, sc = D.rscale;
@@ -284,15 +293,16 @@ __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ de
     case kRedWdot: {  // sums of (p/w)(q/w) for up to three pairs (rsrc[2k], rsrc[2k+1])
       double w[kVecE], p[kVecE];
       load_e(D.rw, e0, n3, w);
-      double* acc[3] = {&r0, &r1, &r2};
       for (int k = 0; k < D.rn && k < 3; ++k) {
         load_e(D.rsrc[2 * k], e0, n3, p);
         load_e(D.rsrc[2 * k + 1], e0, n3, x);
-        double r = *acc[k];
+        double r = 0.0;
 #pragma unroll
         for (int q = 0; q < kVecE; ++q)
           if (ok[q]) r += (p[q] / w[q]) * (x[q] / w[q]);
-        *acc[k] = r;
+        if (k == 0) r0 = r;
+        else if (k == 1) r1 = r;
+        else r2 = r;
       }
       break;
     }
