@@ -414,6 +414,10 @@ struct Solver {
   template <typename T>
   int stage_descs(const std::vector<T>& v, const T** dptr) {
     size_t bytes = v.size() * sizeof(T);
+    if (bytes > arena_size) {
+      set_error("descriptor arena too small for one launch");
+      return kErrArg;
+    }
     size_t off = round_up((int64_t)arena_off, 256);
     if (off + bytes > arena_size) {
       MT_HIP(hipStreamSynchronize(stream));
@@ -432,6 +436,10 @@ struct Solver {
   // since the reservation.
   size_t stage_from = 0;
   int stage_reserve(size_t bytes) {
+    if (bytes > arena_size) {
+      set_error("descriptor arena too small for one demag pass");
+      return kErrArg;
+    }
     if (round_up((int64_t)arena_off, 256) + bytes > arena_size) {
       MT_HIP(hipStreamSynchronize(stream));
       arena_off = 0;
@@ -1524,7 +1532,10 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   MT_HIP(hipMalloc(&P.partials, (size_t)3 * P.B * bps * 4 * sizeof(double)));
   MT_HIP(hipMalloc(&P.results, (size_t)3 * P.B * 4 * sizeof(double)));
   MT_HIP(hipHostMalloc(&P.h_results, (size_t)3 * P.B * 4 * sizeof(double), hipHostMallocDefault));
-  P.arena_size = (size_t)8 << 20;
+  // one tick stages at most one vector-only launch over all states and one
+  // demag pass of up to 64 states (three descriptor arrays)
+  P.arena_size = std::max((size_t)8 << 20, (size_t)(P.B + 2 * kMaxCols) * sizeof(VecDesc) +
+                                               (size_t)kMaxCols * sizeof(LlgDesc) + 4096);
   MT_HIP(hipHostMalloc(&P.arena_h, P.arena_size, hipHostMallocDefault));
   MT_HIP(hipMalloc(&P.arena_d, P.arena_size));
   P.t_out_h.assign((size_t)P.B * P.n_h * P.nt, 0.0);
