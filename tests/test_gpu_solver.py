@@ -72,7 +72,7 @@ def test_batch_equals_individual_runs(oracle):
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x2"])
 def test_batch_beyond_one_demag_group(oracle, precision):
-    """70 states: every tick runs two demag passes (64 + 6 columns).  Each
+    """130 states: every tick runs three demag passes (64 + 64 + 2 columns).  Each
     state's column of the product, its LL kernel row and its step control are
     independent of the others, so a state in the batch reproduces its own
     one-state run bit for bit, on both sides of the split; in FP32 it also
@@ -81,7 +81,7 @@ def test_batch_beyond_one_demag_group(oracle, precision):
     from magtense_amd.micromag import MicromagProblem
 
     res, L = [8, 4, 1], [40e-9, 20e-9, 3e-9]
-    n, B = 32, 70
+    n, B = 32, 130
     m0 = np.stack([_random_m(n, 200 + b) for b in range(B)])
     ang = np.linspace(0, 2 * np.pi, B, endpoint=False)
     fields = np.stack([2e4 * np.cos(ang), 2e4 * np.sin(ang), np.zeros(B)], axis=1)
@@ -89,7 +89,7 @@ def test_batch_beyond_one_demag_group(oracle, precision):
     p = MicromagProblem(res, L, alpha=4.42e3, gamma=2.21e5, tol=1e-6, precision=precision)
     _, M_b = p.run_simulation_batch(5e-11, 6, rows, 10, m0)
     assert M_b.shape == (B, 6, n, 3)
-    for b in (0, 63, 64, 69):
+    for b in (0, 63, 64, 128, 129):
         _, M_1 = p.run_simulation_batch(5e-11, 6, rows[b:b + 1], 10, m0[b:b + 1])
         assert np.array_equal(M_b[b], M_1[0]), (precision, b, np.abs(M_b[b] - M_1[0]).max())
         if precision == "fp32":
