@@ -316,6 +316,110 @@ def adjoint_c5(seed=5, nsteps=2000, dt=1e-13, every=100):
                 note="errors vs central differences and the FP64 oracle adjoint: tests/test_gpu_adjoint.py")
 
 
+def next_rows(W, reps=5):
+    """The widened rows beside the hot path, timed on the GPU (wall time of the C-ABI
+    and Python-API calls, inputs resident where the API takes device arrays):
+    * the threshold-sparse demag product (threshold fraction 0.9) on the benchmark film
+      (N = 32768), 64 states: build time, stored entries, time per product;
+    * the magnetostatic field of the 16384-prism lattice at its 16384 centres
+      (`magstatics.get_H_field`, host arrays in and out);
+    * the tetrahedron tile tensor of 2048 tetrahedra at 2048 points
+      (`magstatics.get_demag_tensor`, tile type 5, host arrays);
+    * the demag-tensor file: save and load of the 128 x 32 film's six FP32 N^2 records."""
+    import tempfile
+
+    import torch
+    from magtense_amd import _lib, micromag
+    from magtense_amd import magstatics as ms
+
+    L = _lib.lib()
+    out = {}
+    n, res, Lg = W["n"], W["res"], W["L"]
+    d = [Lg[k] / res[k] for k in range(3)]
+    ax = [np.array([-Lg[k] / 2 + i * d[k] + d[k] / 2 for i in range(res[k])]) for k in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    dp = torch.as_tensor(pts).cuda()
+    dabc = torch.as_tensor(np.array(d)).cuda()
+    K32 = torch.empty(6 * n * n, dtype=torch.float32, device="cuda")
+    _lib.check(L.mt_prism_tensor(dp.data_ptr(), dabc.data_ptr(), 0, None, dp.data_ptr(), n, n, K32.data_ptr(),
+                                 None, None))
+    torch.cuda.synchronize()
+    h = C.c_void_p()
+    t0 = time.perf_counter()
+    _lib.check(L.mt_demag_sparse_create(K32.data_ptr(), n, 4, 0.9, C.byref(h)))
+    t_build = time.perf_counter() - t0
+    del K32
+    torch.cuda.empty_cache()
+    try:
+        nnz = (C.c_longlong * 6)()
+        thr = C.c_float()
+        _lib.check(L.mt_demag_sparse_info(h, nnz, C.byref(thr)))
+        S = W["B"]
+        dm = torch.as_tensor(W["m0"][:S].reshape(S, 3 * n).astype(np.float32)).cuda()
+        dh = torch.empty(S * 3 * n, dtype=torch.float32, device="cuda")
+        _lib.check(L.mt_demag_apply_sparse(h, dm.data_ptr(), dh.data_ptr(), S, -1.0))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.check(L.mt_demag_apply_sparse(h, dm.data_ptr(), dh.data_ptr(), S, -1.0))
+        torch.cuda.synchronize()
+        ms_apply = (time.perf_counter() - t0) / reps * 1e3
+    finally:
+        L.mt_demag_sparse_destroy(h)
+    stored = float(sum(nnz))
+    alg = stored * 8.0 + 2.0 * S * 3 * n * 4  # value + column per stored entry, states in, fields out
+    out["sparse_demag"] = {"workload": f"{res[0]}x{res[1]}x{res[2]} film, N = {n}, {S} states, threshold fraction 0.9",
+                           "build_s": t_build, "stored_entries": stored, "fraction_of_dense": stored / (6.0 * n * n),
+                           "threshold": thr.value, "ms_per_product": ms_apply,
+                           "alg_bytes_per_product": alg, "achieved_gbs": alg / (ms_apply * 1e-3) / 1e9}
+    # magnetostatics: prism lattice field
+    c, abc = lattice_prisms()
+    nt = c.shape[0]
+    rng = np.random.default_rng(11)
+    t = ms.Tiles(nt, size=abc.tolist(), offset=c.tolist(), tile_type=2, M_rem=1.2 / MU0,
+                 easy_axis=rng.normal(size=(nt, 3)).tolist())
+    ms.get_H_field(t, c[:64])
+    t0 = time.perf_counter()
+    H = ms.get_H_field(t, c)
+    t_h = time.perf_counter() - t0
+    out["magnetostatic_field"] = {"workload": f"{nt} prism tiles (5 nm lattice, jittered) at their {nt} centres",
+                                  "s": t_h, "pairs_per_s": nt * nt / t_h, "finite": bool(np.isfinite(H).all())}
+    # tetrahedron tiles: tensor of 2048 tetrahedra at 2048 points
+    ntt = 2048
+    tt = ms.Tiles(ntt)
+    tt.tile_type = [5] * ntt
+    base = np.array([[0, 0, 0], [4e-9, 0, 0], [0, 3e-9, 0], [0, 0, 2e-9]], dtype=float)
+    grid = np.stack(np.meshgrid(np.arange(16), np.arange(16), np.arange(8), indexing="ij"), -1).reshape(-1, 3)
+    verts = np.zeros((ntt, 3, 4))
+    for i in range(ntt):
+        verts[i] = (base + grid[i] * 5e-9 + rng.uniform(-0.3e-9, 0.3e-9, (4, 3))).T
+    tt.vertices = verts
+    tp = grid * 5e-9 + 1e-9
+    ms.get_demag_tensor(tt, tp[:16])
+    t0 = time.perf_counter()
+    N = ms.get_demag_tensor(tt, tp)
+    t_n = time.perf_counter() - t0
+    out["tetrahedron_tensor"] = {"workload": f"{ntt} tetrahedron tiles at {tp.shape[0]} points, N(n_tiles, n_pts, 3, 3) FP64",
+                                 "s": t_n, "pairs_per_s": ntt * tp.shape[0] / t_n,
+                                 "finite": bool(np.isfinite(N).all())}
+    # demag-tensor file (six FP32 N^2 records) for the 128 x 32 x 1 film
+    nf = 4096
+    Kf = np.random.default_rng(3).normal(size=(6, nf, nf)).astype(np.float32)
+    with tempfile.TemporaryDirectory() as tmp:
+        f = os.path.join(tmp, "K.dat")
+        t0 = time.perf_counter()
+        micromag.save_demag_tensor(f, Kf)
+        t_w = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        K2 = micromag.load_demag_tensor(f, nf)
+        t_r = time.perf_counter() - t0
+    out["tensor_file"] = {"workload": f"N = {nf}: six FP32 {nf}^2 records ({Kf.nbytes / 1e6:.1f} MB)",
+                          "save_gbs": Kf.nbytes / t_w / 1e9, "load_gbs": Kf.nbytes / t_r / 1e9,
+                          "round_trip_exact": bool(np.array_equal(K2, Kf))}
+    return out
+
+
 def lattice_prisms(seed=4, side=128):
     """16384 prisms: side x side lattice of pitch 5 nm, centres jittered by
     U[-0.5, 0.5] nm in x and y, edges U[0.8, 1.2] * (5, 5, 2) nm."""
@@ -387,6 +491,8 @@ def main():
     ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--no-shard", action="store_true",
                     help="skip the product at one row shard's shape (N/8 rows x N sources)")
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the widened rows (sparse demag, magnetostatic field, tetrahedron tiles, tensor file)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation configuration (RK4 adjoint gradient wall time)")
     ap.add_argument("--table-steps", type=int, default=30,
@@ -480,6 +586,8 @@ def main():
     log(f"row shard shape: {shard}")
     adj = None if args.no_adjoint else adjoint_c5()
     log(f"adjoint (differentiation configuration): {adj}")
+    nxt = None if args.no_next else next_rows(W)
+    log(f"widened rows: {nxt}")
     rd, cp = C.c_double(0), C.c_double(0)
     _lib.check(L.mt_measure_bandwidth(8 << 30, 10, C.byref(rd), C.byref(cp)))
     rdr = C.c_double(0)
@@ -540,6 +648,7 @@ def main():
         "step_breakdown": breakdown,
         "row_shard": shard,
         "adjoint": adj,
+        "next_rows": nxt,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -163,11 +163,16 @@ struct CsrView {
   const float* vals;
 };
 
-// many states: wave per receiver row, lane = state (mT, hT are [3n][S])
+// many states: wave per receiver row, lane = state (mT, hT are [3n][S]).
+// The row's entries come in 64 at a time with one coalesced load of columns
+// and values (lane k holds entry k); each entry is then broadcast from its
+// lane (v_readlane) and every lane gathers its state's component.  Eight
+// entries per unrolled group keep eight independent gathers in flight; the
+// sums still run over the entries in CSR order.
 __global__ void __launch_bounds__(256) spmm_rows_kernel(CsrView A, int n, int S, const float* __restrict__ mT,
                                                         float pref, float* __restrict__ hT) {
   const int lane = threadIdx.x & 63;
-  // wave-uniform row: the CSR walk runs on scalar registers and scalar loads
+  // wave-uniform row: the CSR walk runs on scalar registers
   const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (i >= n) return;
   const int s = lane < S ? lane : 0;
@@ -179,12 +184,29 @@ __global__ void __launch_bounds__(256) spmm_rows_kernel(CsrView A, int n, int S,
     const int a = ca[c], b = cb[c];
     const int64_t base = A.coff[c];
     const int64_t q0 = base + A.rowptr[(size_t)c * (n + 1) + i], q1 = base + A.rowptr[(size_t)c * (n + 1) + i + 1];
+    const float* mb = mT + (size_t)b * n * S + s;
+    const float* ma = mT + (size_t)a * n * S + s;
     float sa = 0.0f, sb = 0.0f;
-    for (int64_t q = q0; q < q1; ++q) {
-      const int j = A.cols[q];
-      const float v = A.vals[q];
-      sa += v * mT[((size_t)b * n + j) * S + s];
-      if (a != b) sb += v * mT[((size_t)a * n + j) * S + s];
+    for (int64_t q = q0; q < q1; q += 64) {
+      const int cnt = (int)(q1 - q < 64 ? q1 - q : 64);
+      const int jl = lane < cnt ? A.cols[q + lane] : 0;
+      const int vl = lane < cnt ? __float_as_int(A.vals[q + lane]) : 0;
+      int k = 0;
+      for (; k + 8 <= cnt; k += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int j = __builtin_amdgcn_readlane(jl, k + u);
+          const float v = __int_as_float(__builtin_amdgcn_readlane(vl, k + u));
+          sa += v * mb[(size_t)j * S];
+          if (a != b) sb += v * ma[(size_t)j * S];
+        }
+      }
+      for (; k < cnt; ++k) {
+        const int j = __builtin_amdgcn_readlane(jl, k);
+        const float v = __int_as_float(__builtin_amdgcn_readlane(vl, k));
+        sa += v * mb[(size_t)j * S];
+        if (a != b) sb += v * ma[(size_t)j * S];
+      }
     }
     acc[a] += sa;
     if (a != b) acc[b] += sb;
