@@ -166,9 +166,10 @@ struct CsrView {
 // many states: wave per receiver row, lane = state (mT, hT are [3n][S]).
 // The row's entries come in 64 at a time with one coalesced load of columns
 // and values (lane k holds entry k); each entry is then broadcast from its
-// lane (v_readlane) and every lane gathers its state's component.  Eight
-// entries per unrolled group keep eight independent gathers in flight; the
-// sums still run over the entries in CSR order.
+// lane (v_readlane) and every lane gathers its state's component.  Sixteen
+// entries per unrolled group keep sixteen independent gathers in flight
+// (measured: eight 12.3 ms, sixteen 10.2 ms, thirty-two 10.6 ms per product on
+// the 32768-cell film); the sums still run over the entries in CSR order.
 __global__ void __launch_bounds__(256) spmm_rows_kernel(CsrView A, int n, int S, const float* __restrict__ mT,
                                                         float pref, float* __restrict__ hT) {
   const int lane = threadIdx.x & 63;
@@ -192,9 +193,9 @@ __global__ void __launch_bounds__(256) spmm_rows_kernel(CsrView A, int n, int S,
       const int jl = lane < cnt ? A.cols[q + lane] : 0;
       const int vl = lane < cnt ? __float_as_int(A.vals[q + lane]) : 0;
       int k = 0;
-      for (; k + 8 <= cnt; k += 8) {
+      for (; k + 16 <= cnt; k += 16) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < 16; ++u) {
           const int j = __builtin_amdgcn_readlane(jl, k + u);
           const float v = __int_as_float(__builtin_amdgcn_readlane(vl, k + u));
           sa += v * mb[(size_t)j * S];
