@@ -365,6 +365,14 @@ def next_rows(W, reps=5):
             _lib.check(L.mt_demag_apply_sparse(h, dm.data_ptr(), dh.data_ptr(), S, -1.0))
         torch.cuda.synchronize()
         ms_apply = (time.perf_counter() - t0) / reps * 1e3
+        # one state (the reference's per-state product)
+        _lib.check(L.mt_demag_apply_sparse(h, dm.data_ptr(), dh.data_ptr(), 1, -1.0))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.check(L.mt_demag_apply_sparse(h, dm.data_ptr(), dh.data_ptr(), 1, -1.0))
+        torch.cuda.synchronize()
+        ms_apply1 = (time.perf_counter() - t0) / reps * 1e3
     finally:
         L.mt_demag_sparse_destroy(h)
     stored = float(sum(nnz))
@@ -372,7 +380,9 @@ def next_rows(W, reps=5):
     out["sparse_demag"] = {"workload": f"{res[0]}x{res[1]}x{res[2]} film, N = {n}, {S} states, threshold fraction 0.9",
                            "build_s": t_build, "stored_entries": stored, "fraction_of_dense": stored / (6.0 * n * n),
                            "threshold": thr.value, "ms_per_product": ms_apply,
-                           "alg_bytes_per_product": alg, "achieved_gbs": alg / (ms_apply * 1e-3) / 1e9}
+                           "alg_bytes_per_product": alg, "achieved_gbs": alg / (ms_apply * 1e-3) / 1e9,
+                           "ms_per_product_one_state": ms_apply1,
+                           "one_state_gbs": (stored * 8.0 + 2.0 * 3 * n * 4) / (ms_apply1 * 1e-3) / 1e9}
     # magnetostatics: prism lattice field
     c, abc = lattice_prisms()
     nt = c.shape[0]
