@@ -511,8 +511,9 @@ def main():
                     help="time the translation-invariant table as the main path (default: the dense 25.8 GB tensor)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}); "
-              "the multi-GPU row-sharded design is described in DESIGN.md and not launched.", file=sys.stderr)
+        print(f"bench.py measures one MI355X (asked for --gpus {args.gpus}): the per-GPU product of the "
+              "row-sharded tensor is mt_demag_apply_bf16x2_rows; runs over several GPUs are not launched.",
+              file=sys.stderr)
         sys.exit(2)
     # torch ships its own HIP runtime; initialise it before libmagtense_hip's
     # (both then share the device; the reverse order fails to initialise torch)
