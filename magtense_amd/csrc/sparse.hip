@@ -232,6 +232,10 @@ __global__ void __launch_bounds__(256) spmv_rows_kernel(CsrView A, int n, const 
     const int a = ca[c], b = cb[c];
     const int64_t base = A.coff[c];
     const int64_t q0 = base + A.rowptr[(size_t)c * (n + 1) + i], q1 = base + A.rowptr[(size_t)c * (n + 1) + i + 1];
+    // four iterations unrolled: their loads are issued together (measured 1.19 ms ->
+    // 0.95 ms per one-state product on the 32768-cell film); each lane still sums
+    // its entries in order
+#pragma unroll 4
     for (int64_t q = q0 + lane; q < q1; q += 64) {
       const int j = A.cols[q];
       const float v = A.vals[q];
