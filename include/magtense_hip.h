@@ -153,6 +153,18 @@ int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, in
 long long mt_split_tensor_row_offset(int n, int row0);
 int mt_demag_apply_bf16x2_rows(const void* split_rows, int n, int row0, int nrows, const float* m, float* h,
                                int nstates, float pref);
+/* Near-band correction of a split-BF16 product on a uniform nx x ny x nz grid
+ * of an Lx x Ly x Lz box: h += pref * (the FP32 product minus the three split
+ * products) over the sources within `radius` cells (1..4) in x and y and
+ * min(radius, nz - 1) layers in z of every receiver, m and h device [s][3n]
+ * FP32.  Applied after mt_demag_apply_bf16x2 / _toeplitz it brings the field
+ * of coherent magnetizations (uniform, smooth) to the FP32 product's level in
+ * the near field, where the split's rounding residuals are largest; the
+ * solver applies it with radius 2 on uniform grids (MT_NEAR_RADIUS overrides,
+ * 0 switches it off).  No reference counterpart (the reference's product is
+ * FP32, LandauLifshitzEquationSolver.f90:706-754). */
+int mt_demag_near_correction(int nx, int ny, int nz, double Lx, double Ly, double Lz, int radius, const float* m,
+                             float* h, int nstates, float pref);
 /* Same product from the translation-invariant table of mt_toeplitz_tensor. */
 int mt_demag_apply_bf16x2_toeplitz(const void* table, int nx, int ny, int nz, const float* m, float* h, int nstates,
                                    float pref);
@@ -167,8 +179,21 @@ int mt_demag_apply_bf16x2_toeplitz(const void* table, int nx, int ny, int nz, co
 int mt_demag_sparse_create(const float* K32, int n, int approx, double threshold, void** out);
 /* nnz per component (xx xy xz yy yz zz) and the FP32 threshold used. */
 int mt_demag_sparse_info(void* sp, long long* nnz6, float* threshold);
-/* h = pref * K_sparse m for nstates (<= 64 per call) states, [s][3n] device layout
- * (the product of updateDemagfield's sparse branch, LandauLifshitzEquationSolver.f90:581-624). */
+/* Same kept entries, held as a block-sparse split-BF16 tensor: the 32-row x
+ * 16-source tiles holding a kept entry, stored in the MFMA fragment order of
+ * the dense split tensor (entries at or below the threshold zero), with row
+ * tiles grouped four to a workgroup over a shared list of source chunks
+ * (grid_nx > 0, a multiple of 32: the cells form rows of grid_nx, and tiles of
+ * consecutive rows may share a group).  The product is the batched split-BF16
+ * MFMA GEMM over the stored tiles.  No reference counterpart (an additive
+ * precision mode of the sparse branch, LandauLifshitzEquationSolver.f90:581-624). */
+int mt_demag_sparse_create_bf16x2(const float* K32, int n, int approx, double threshold, int grid_nx, void** out);
+/* Block statistics of a tensor from mt_demag_sparse_create_bf16x2: list
+ * entries (each 4 x 12 KiB of fragments), non-empty tiles, fragment bytes, k-split. */
+int mt_demag_sparse_blocks(void* sp, long long* entries, long long* tiles_nonempty, long long* bytes, int* ksplit);
+/* h = pref * K_sparse m for nstates states, [s][3n] device layout (the
+ * product of updateDemagfield's sparse branch, LandauLifshitzEquationSolver.f90:581-624;
+ * FP32 CSR, or split-BF16 tiles for a tensor of mt_demag_sparse_create_bf16x2). */
 int mt_demag_apply_sparse(void* sp, const float* m, float* h, int nstates, float pref);
 void mt_demag_sparse_destroy(void* sp);
 
@@ -216,7 +241,7 @@ typedef struct {
   int keep_fp32_tensor;  /* keep an FP32 copy beside the split tensor */
   int ksplit;            /* 0: automatic k-split of the MFMA GEMM */
   int demag_approx;      /* MT_DEMAG_DENSE (0 or 1), MT_DEMAG_THRESHOLD (2), MT_DEMAG_THRESHOLD_FRACTION (4);
-                            the sparse modes need precision MT_PREC_FP32 */
+                            the sparse modes keep an FP32 CSR (MT_PREC_FP32) or split-BF16 tiles (MT_PREC_BF16X2) */
   double demag_threshold; /* dem_thres: |K| threshold (mode 2) or fraction of the positive entries (mode 4) */
   int toeplitz;          /* MT_PREC_BF16X2 tensor storage: 0 automatic (the translation-invariant fragment
                             table when nx % 32 == 0 and no K_in), 1 require it, -1 dense fragments */
@@ -254,6 +279,7 @@ typedef struct {
   double demag_ms;
   double vec_ms, llg_ms, vec_bytes, llg_bytes;
   long long vec_launches, llg_launches;
+  double near_ms;  /* of those demag milliseconds, the near-band correction's (uniform grids, bf16x2) */
 } mt_stats;
 
 typedef struct mt_solver mt_solver;

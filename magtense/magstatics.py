@@ -23,7 +23,7 @@ from typing import List, Optional, Tuple, Union
 
 import numpy as np
 
-from . import _lib
+from magtense_amd import _lib
 
 MU0 = 4 * np.pi * 1e-7
 PRISM = 2

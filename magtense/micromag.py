@@ -10,7 +10,10 @@ module ``magtensesource``.
 Additive extensions (no reference counterpart):
   * ``precision='fp32' | 'bf16x2'`` selects the demag product: the plain
     per-state FP32 mat-vec (reference semantics, the default) or the split-BF16
-    batched MFMA GEMM.  A boolean ``precision`` (the reference's unused
+    batched MFMA GEMM.  With ``demag_approx`` 'threshold' or
+    'threshold_fraction' the kept entries are an FP32 CSR product ('fp32') or
+    the split-BF16 GEMM over the 32 x 16 tiles that hold a kept entry
+    ('bf16x2').  A boolean ``precision`` (the reference's unused
     ``usePrecision`` flag) keeps the default.
   * ``Ms`` may be an array of per-cell values.
   * ``run_simulation_batch`` integrates several independent states (and
@@ -51,7 +54,7 @@ from typing import List, Optional, Union
 
 import numpy as np
 
-from . import _lib
+from magtense_amd import _lib
 
 MU0 = 4 * np.pi * 1e-7
 
@@ -250,8 +253,6 @@ class MicromagProblem:
         if self._dem_appr in (3, 5):
             raise NotImplementedError("the FFT-threshold demag approximations are not implemented "
                                       "(demag_approx None, 'threshold' and 'threshold_fraction' are)")
-        if self._dem_appr in (2, 4) and self._precision_code() != _lib.PREC_FP32:
-            raise ValueError("the threshold-sparse demag product runs in FP32: use precision='fp32'")
         if self._solver not in (1, 2):
             raise NotImplementedError("solver must be 'explicit' or 'dynamic'")
         if self.CV > 0:
@@ -363,6 +364,14 @@ class MicromagProblem:
         h = C.c_void_p()
         _lib.check(L.mt_solver_create(C.byref(q), C.byref(h)))
         try:
+            if getattr(self, "_timing", False):  # HIP-event kernel timing (benchmarks)
+                L.mt_solver_set_timing(h, 1)
+            if Kout is not None:
+                # written straight after assembly, before integrating, as the reference
+                # does (LandauLifshitzEquationSolver.f90:1149-1151); creation has
+                # synchronised the device-to-host copy of the tensor
+                save_demag_tensor(path, Kout)
+                Kout = None
             _lib.check(L.mt_solver_run(h))
             _lib.check(L.mt_solver_results(h, t_out.ctypes.data, y_out.ctypes.data,
                                            fields.ctypes.data if fields is not None else None,
@@ -375,10 +384,10 @@ class MicromagProblem:
                 self.last_demag_info = dict(nnz=[int(v) for v in nnz], threshold=float(thr.value))
         finally:
             L.mt_solver_destroy(h)
-        if Kout is not None:
-            save_demag_tensor(path, Kout)
         self.last_stats = [dict(f_calls=s.f_calls, steps=s.steps, bad_steps=s.bad_steps, flag3=s.flag3,
-                                flag4=s.flag4, h_last=s.h_last, evals=s.evals) for s in stats]
+                                flag4=s.flag4, h_last=s.h_last, evals=s.evals, demag_passes=s.demag_passes,
+                                demag_ms=s.demag_ms, llg_ms=s.llg_ms, vec_ms=s.vec_ms,
+                                llg_launches=s.llg_launches, vec_launches=s.vec_launches) for s in stats]
         return t_out, y_out, fields
 
     def _pts(self):

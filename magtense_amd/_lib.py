@@ -66,6 +66,7 @@ class MtStats(C.Structure):
         ("demag_ms", C.c_double),
         ("vec_ms", C.c_double), ("llg_ms", C.c_double), ("vec_bytes", C.c_double), ("llg_bytes", C.c_double),
         ("vec_launches", C.c_longlong), ("llg_launches", C.c_longlong),
+        ("near_ms", C.c_double),
     ]
 
 
@@ -86,6 +87,7 @@ EXPORTS = [
     "mt_solver_set_timing", "mt_solver_destroy", "mt_run_micromag", "mt_rk4_create", "mt_rk4_forward",
     "mt_rk4_backward", "mt_rk4_destroy", "mt_tiles_demag_tensor", "mt_tiles_h_field", "mt_demag_sparse_create",
     "mt_demag_sparse_info", "mt_demag_apply_sparse", "mt_demag_sparse_destroy", "mt_solver_demag_info",
+    "mt_demag_sparse_create_bf16x2", "mt_demag_sparse_blocks", "mt_demag_near_correction",
     "mt_toeplitz_tensor_bytes", "mt_toeplitz_tensor", "mt_demag_apply_bf16x2_toeplitz",
     "mt_split_tensor_row_offset", "mt_demag_apply_bf16x2_rows",
     "icl_initDemagMatrices", "icl_MVMult_GetH", "icl_destroy", "icl_initDemagMatrices_sparse", "icl_MVMult_GetH_sparse",
@@ -95,7 +97,13 @@ _lib = None
 
 
 class MagTenseHipError(RuntimeError):
-    pass
+    """A non-zero status of the C ABI; ``code`` is the status (MT_ERR_* in
+    include/magtense_hip.h: -1 argument, -2 HIP runtime, -3 solver, -4
+    device allocation, -5 call order, -6 unsupported input)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def build():
@@ -147,12 +155,15 @@ def load():
     lib.mt_toeplitz_tensor_bytes.argtypes = [C.c_int] * 3
     lib.mt_toeplitz_tensor_bytes.restype = C.c_longlong
     lib.mt_toeplitz_tensor.argtypes = [C.c_int] * 3 + [C.c_double] * 3 + [vp]
+    lib.mt_demag_near_correction.argtypes = [C.c_int] * 3 + [C.c_double] * 3 + [C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_demag_apply_bf16x2_toeplitz.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_split_tensor_row_offset.argtypes = [C.c_int, C.c_int]
     lib.mt_split_tensor_row_offset.restype = C.c_longlong
     lib.mt_demag_apply_bf16x2_rows.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_demag_sparse_create.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.POINTER(vp)]
     lib.mt_demag_sparse_info.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_float)]
+    lib.mt_demag_sparse_create_bf16x2.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
+    lib.mt_demag_sparse_blocks.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 3 + [C.POINTER(C.c_int)]
     lib.mt_demag_apply_sparse.argtypes = [vp, vp, vp, C.c_int, C.c_float]
     lib.mt_demag_sparse_destroy.argtypes = [vp]
     lib.mt_demag_sparse_destroy.restype = None
@@ -181,4 +192,4 @@ def lib():
 def check(rc):
     if rc != 0:
         msg = load().mt_last_error()
-        raise MagTenseHipError(f"libmagtense_hip error {rc}: {msg.decode() if msg else ''}")
+        raise MagTenseHipError(f"libmagtense_hip error {rc}: {msg.decode() if msg else ''}", rc)

@@ -1,8 +1,10 @@
 // C ABI: library utilities, the stand-alone kernel entry points and the
 // reference plug-in ABI (icl_*, MagTenseCudaBlasICLWrapper.cxx:13-59).
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -124,6 +126,8 @@ __global__ void fill_random_kernel(uint4* __restrict__ p, int64_t n16) {
   }
 }
 
+}  // namespace
+
 int device_ok() {
   int nd = 0;
   if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) {
@@ -132,8 +136,6 @@ int device_ok() {
   }
   return kOk;
 }
-
-}  // namespace
 
 void set_error(const std::string& msg) { g_error = msg; }
 const char* last_error() { return g_error.c_str(); }
@@ -263,30 +265,49 @@ int mt_toeplitz_tensor(int nx, int ny, int nz, double Lx, double Ly, double Lz, 
   return kOk;
 }
 
+extern "C++" {
+namespace mt {
 namespace {
+__global__ void zero16_kernel(uint4* __restrict__ p, int64_t n16) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n16; t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+}  // namespace
+
+Bf16x2Work bf16x2_work(int n, int nrows, int nx, int ny, int nz, int nstates, int ks) {
+  Bf16x2Work w;
+  w.ks = ks > 0 ? ks : (nx > 0 ? gemm_ksplit_table(nx, ny, nz) : gemm_ksplit_dense(n, nrows));
+  w.cols = nstates < kMaxCols ? nstates : kMaxCols;
+  w.packed_bytes = (size_t)packed_m_bytes(n);
+  w.part_floats = (size_t)w.ks * w.cols * 3 * (size_t)nrows;
+  return w;
+}
+
 // shared by the dense, row-block and translation-invariant batched products;
-// nrows < n only for a row block of the dense tensor (h holds 3 nrows per state)
-int apply_bf16x2(const void* K, int n, int nrows, int nx, int ny, int nz, const float* m, float* h, int nstates,
-                 float pref) {
+// nrows < n only for a row block of the dense tensor (h holds 3 nrows per state).
+// `ws` holds w.bytes() of device memory.
+int apply_bf16x2_in(const void* K, int n, int nrows, int nx, int ny, int nz, const float* m, float* h, int nstates,
+                    float pref, const Bf16x2Work& w, uint8_t* ws, const BlockSparseView* bsv) {
   int rc = kOk;
   const int64_t n3 = (int64_t)3 * n;
   const int64_t r3 = (int64_t)3 * nrows;
-  int ks = nx > 0 ? gemm_ksplit_table(nx, ny, nz) : gemm_ksplit_dense(n, nrows);
-  DevArr<uint8_t> P8;
-  DevArr<float> PF;
-  MT_HIP(P8.alloc((size_t)packed_m_bytes(n)));
-  int cols = nstates < kMaxCols ? nstates : kMaxCols;
-  MT_HIP(PF.alloc((size_t)ks * cols * r3));
-  uint8_t* packed = P8.p;
-  float* part = PF.p;
+  const int ks = w.ks, cols = w.cols;
+  uint8_t* packed = ws;
+  float* part = reinterpret_cast<float*>(ws + round_up((int64_t)w.packed_bytes, 256));
+  const int64_t n16 = (int64_t)(w.packed_bytes / 16);
   for (int s0 = 0; s0 < nstates && rc == kOk; s0 += kMaxCols) {
     int S = nstates - s0 < kMaxCols ? nstates - s0 : kMaxCols;
-    MT_HIP(hipMemsetAsync(packed, 0, (size_t)packed_m_bytes(n), nullptr));
+    hipLaunchKernelGGL(zero16_kernel, dim3((unsigned)std::min<int64_t>((n16 + 255) / 256, 4096)), dim3(256), 0,
+                       nullptr, reinterpret_cast<uint4*>(packed), n16);
+    MT_HIP(hipGetLastError());
     int64_t tot = (int64_t)S * n3;
     hipLaunchKernelGGL(pack_bf16_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr, m + s0 * n3, n,
                        S, packed);
     MT_HIP(hipGetLastError());
-    if (nx > 0)
+    if (bsv)
+      rc = launch_gemm_bf16x2_bsr((const uint8_t*)K, *bsv, packed, n, part, n3, (int64_t)cols * n3, S, ks, nullptr);
+    else if (nx > 0)
       rc = launch_gemm_bf16x2_toeplitz((const uint8_t*)K, nx, ny, nz, packed, part, n3, (int64_t)cols * n3, S, ks,
                                        nullptr);
     else if (nrows < n)
@@ -303,7 +324,18 @@ int apply_bf16x2(const void* K, int n, int nrows, int nx, int ny, int nz, const 
   MT_HIP(hipStreamSynchronize(nullptr));
   return rc;
 }
+
+namespace {
+int apply_bf16x2(const void* K, int n, int nrows, int nx, int ny, int nz, const float* m, float* h, int nstates,
+                 float pref) {
+  const Bf16x2Work w = bf16x2_work(n, nrows, nx, ny, nz, nstates);
+  DevArr<uint8_t> ws;
+  MT_HIP(ws.alloc(w.bytes()));
+  return apply_bf16x2_in(K, n, nrows, nx, ny, nz, m, h, nstates, pref, w, ws.p);
+}
 }  // namespace
+}  // namespace mt
+}  // extern "C++"
 
 int mt_demag_apply_bf16x2_toeplitz(const void* table, int nx, int ny, int nz, const float* m, float* h, int nstates,
                                    float pref) {
@@ -319,6 +351,28 @@ int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, in
   if (rc) return rc;
   MT_CHECK(n > 0 && nstates > 0 && split && m && h, "invalid demag arguments");
   return apply_bf16x2(split, n, n, 0, 0, 0, m, h, nstates, pref);
+}
+
+int mt_demag_near_correction(int nx, int ny, int nz, double Lx, double Ly, double Lz, int radius, const float* m,
+                             float* h, int nstates, float pref) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(nx >= 1 && ny >= 1 && nz >= 1 && m && h && nstates > 0, "invalid near-band correction arguments");
+  MT_CHECK(radius >= 1 && radius <= 4, "the near-band radius must be 1..4 cells");
+  const int Rz = near_band_rz(radius, nz);
+  DevArr<float> tab;
+  MT_HIP(tab.alloc((size_t)near_table_floats(radius, nz)));
+  rc = launch_near_table(nx, ny, nz, Lx, Ly, Lz, radius, Rz, -1.0f, tab.p, nullptr);
+  if (rc) return rc;
+  const int n = nx * ny * nz;
+  for (int s0 = 0; s0 < nstates && rc == kOk; s0 += 65535) {
+    const int S = nstates - s0 < 65535 ? nstates - s0 : 65535;
+    rc = launch_near_correction(tab.p, radius, nx, ny, nz, nullptr, m + (int64_t)s0 * 3 * n, (int64_t)3 * n, S,
+                                h + (int64_t)s0 * 3 * n, (int64_t)3 * n, pref, true, nullptr);
+  }
+  if (rc) return rc;
+  MT_HIP(hipStreamSynchronize(nullptr));
+  return kOk;
 }
 
 long long mt_split_tensor_row_offset(int n, int row0) {
@@ -454,6 +508,40 @@ int mt_demag_sparse_create(const float* K32, int n, int approx, double threshold
   return kOk;
 }
 
+int mt_demag_sparse_create_bf16x2(const float* K32, int n, int approx, double threshold, int grid_nx, void** out) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(out && K32 && n > 0 && grid_nx >= 0, "invalid sparse tensor arguments");
+  MT_CHECK(approx == MT_DEMAG_THRESHOLD || approx == MT_DEMAG_THRESHOLD_FRACTION,
+           "approx must be MT_DEMAG_THRESHOLD or MT_DEMAG_THRESHOLD_FRACTION");
+  *out = nullptr;
+  float thr = (float)threshold;
+  if (approx == MT_DEMAG_THRESHOLD_FRACTION) {
+    rc = threshold_for_fraction(K32, n, threshold, &thr, nullptr);
+    if (rc) return rc;
+  }
+  std::unique_ptr<SparseDemag> sp(new SparseDemag());
+  sp->blocks = new BlockSparse();
+  rc = sp->blocks->build(K32, n, thr, grid_nx, nullptr);
+  if (rc) return rc;
+  sp->n = n;
+  sp->threshold = thr;
+  for (int c = 0; c < 6; ++c) sp->nnz[c] = sp->blocks->nnz[c];
+  *out = sp.release();
+  return kOk;
+}
+
+int mt_demag_sparse_blocks(void* h, long long* entries, long long* tiles_nonempty, long long* bytes, int* ksplit) {
+  MT_CHECK(h, "null sparse tensor");
+  SparseDemag* sp = static_cast<SparseDemag*>(h);
+  MT_CHECK(sp->blocks, "the sparse tensor is held as CSR (mt_demag_sparse_create), not as blocks");
+  if (entries) *entries = sp->blocks->entries;
+  if (tiles_nonempty) *tiles_nonempty = sp->blocks->tiles_nonempty;
+  if (bytes) *bytes = sp->blocks->bytes();
+  if (ksplit) *ksplit = sp->blocks->ksplit;
+  return kOk;
+}
+
 int mt_demag_sparse_info(void* h, long long* nnz6, float* threshold) {
   MT_CHECK(h, "null sparse tensor");
   SparseDemag* sp = static_cast<SparseDemag*>(h);
@@ -468,6 +556,14 @@ int mt_demag_apply_sparse(void* h, const float* m, float* hout, int nstates, flo
   if (rc) return rc;
   MT_CHECK(h && m && hout && nstates > 0, "invalid sparse product arguments");
   SparseDemag* sp = static_cast<SparseDemag*>(h);
+  if (sp->blocks) {  // split-BF16 block-sparse tensor: the batched MFMA product over its tiles
+    const BlockSparse& b = *sp->blocks;
+    const Bf16x2Work w = bf16x2_work(b.n, b.n, 0, 0, 0, nstates, b.ksplit);
+    DevArr<uint8_t> ws;
+    MT_HIP(ws.alloc(w.bytes()));
+    const BlockSparseView v = b.view();
+    return apply_bf16x2_in(b.frags, b.n, b.n, 0, 0, 0, m, hout, nstates, pref, w, ws.p, &v);
+  }
   const int64_t n3 = (int64_t)3 * sp->n;
   for (int s0 = 0; s0 < nstates; s0 += kMaxCols) {
     int S = nstates - s0 < kMaxCols ? nstates - s0 : kMaxCols;
@@ -601,6 +697,7 @@ int mt_tiles_h_field(const mt_tiles* tiles, int n_pts, const double* pts, const 
 }
 
 // ---- reference plug-in ABI: one global instance, host arrays, PCIe per call ----
+extern "C++" {
 namespace {
 std::mutex g_icl_mutex;
 float* g_icl_K = nullptr;  // receiver-major FP32 planes
@@ -608,7 +705,28 @@ float* g_icl_io = nullptr; // 6n floats: m (3n) then h (3n)
 int g_icl_n = 0;
 SparseDemag* g_icl_sparse = nullptr;  // sparse plug-in tensor (components loaded one by one)
 float* g_icl_sp_io = nullptr;         // 6n floats: m (3n) then h (3n)
+
+// One plug-in product on host arrays: M -> device, product, H <- device.
+// The reference's void entry points have no status, so a failed copy or
+// product is recorded in mt_last_error() and H is left untouched.
+template <class F>
+int icl_apply(int N, float* io, const float* Mx, const float* My, const float* Mz, float* Hx, float* Hy, float* Hz,
+              F&& product) {
+  float* dm = io;
+  float* dh = io + 3 * (size_t)N;
+  const size_t b = (size_t)N * sizeof(float);
+  MT_HIP(hipMemcpy(dm, Mx, b, hipMemcpyHostToDevice));
+  MT_HIP(hipMemcpy(dm + N, My, b, hipMemcpyHostToDevice));
+  MT_HIP(hipMemcpy(dm + 2 * (size_t)N, Mz, b, hipMemcpyHostToDevice));
+  int rc = product(dm, dh);
+  if (rc != kOk) return rc;
+  MT_HIP(hipMemcpy(Hx, dh, b, hipMemcpyDeviceToHost));
+  MT_HIP(hipMemcpy(Hy, dh + N, b, hipMemcpyDeviceToHost));
+  MT_HIP(hipMemcpy(Hz, dh + 2 * (size_t)N, b, hipMemcpyDeviceToHost));
+  return kOk;
+}
 }  // namespace
+}  // extern "C++"
 
 void icl_initDemagMatrices(const float* Kxx, const float* Kxy, const float* Kxz, const float* Kyy, const float* Kyz,
                            const float* Kzz, int* n) {
@@ -648,16 +766,9 @@ void icl_MVMult_GetH(const float* Mx, const float* My, const float* Mz, float* H
     set_error("icl_MVMult_GetH: icl_initDemagMatrices not called for this size");
     return;
   }
-  int N = *n;
-  float* dm = g_icl_io;
-  float* dh = g_icl_io + 3 * (size_t)N;
-  (void)hipMemcpy(dm, Mx, N * sizeof(float), hipMemcpyHostToDevice);
-  (void)hipMemcpy(dm + N, My, N * sizeof(float), hipMemcpyHostToDevice);
-  (void)hipMemcpy(dm + 2 * (size_t)N, Mz, N * sizeof(float), hipMemcpyHostToDevice);
-  if (demag_apply_f32(g_icl_K, N, dm, dh, 1, *pref, nullptr) != kOk) return;
-  (void)hipMemcpy(Hx, dh, N * sizeof(float), hipMemcpyDeviceToHost);
-  (void)hipMemcpy(Hy, dh + N, N * sizeof(float), hipMemcpyDeviceToHost);
-  (void)hipMemcpy(Hz, dh + 2 * (size_t)N, N * sizeof(float), hipMemcpyDeviceToHost);
+  (void)icl_apply(*n, g_icl_io, Mx, My, Mz, Hx, Hy, Hz, [&](const float* dm, float* dh) {
+    return demag_apply_f32(g_icl_K, *n, dm, dh, 1, *pref, nullptr);
+  });
 }
 
 void icl_initDemagMatrices_sparse(int* n, int* nnz, int* mat_no, float* values, int* colInds, int* rowInds) {
@@ -682,16 +793,9 @@ void icl_MVMult_GetH_sparse(const float* Mx, const float* My, const float* Mz, f
     set_error("icl_MVMult_GetH_sparse: icl_initDemagMatrices_sparse not called for this size");
     return;
   }
-  const int N = *n;
-  float* dm = g_icl_sp_io;
-  float* dh = g_icl_sp_io + 3 * (size_t)N;
-  (void)hipMemcpy(dm, Mx, N * sizeof(float), hipMemcpyHostToDevice);
-  (void)hipMemcpy(dm + N, My, N * sizeof(float), hipMemcpyHostToDevice);
-  (void)hipMemcpy(dm + 2 * (size_t)N, Mz, N * sizeof(float), hipMemcpyHostToDevice);
-  if (g_icl_sparse->apply(dm, 3 * (int64_t)N, 1, *pref, dh, nullptr) != kOk) return;
-  (void)hipMemcpy(Hx, dh, N * sizeof(float), hipMemcpyDeviceToHost);
-  (void)hipMemcpy(Hy, dh + N, N * sizeof(float), hipMemcpyDeviceToHost);
-  (void)hipMemcpy(Hz, dh + 2 * (size_t)N, N * sizeof(float), hipMemcpyDeviceToHost);
+  (void)icl_apply(*n, g_icl_sp_io, Mx, My, Mz, Hx, Hy, Hz, [&](const float* dm, float* dh) {
+    return g_icl_sparse->apply(dm, 3 * (int64_t)(*n), 1, *pref, dh, nullptr);
+  });
 }
 
 void icl_destroy(void) {

@@ -25,6 +25,7 @@
 #include <cstdlib>
 
 #include "mt_common.h"
+#include "mt_kernels.h"
 
 namespace mt {
 
@@ -360,12 +361,18 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
 // uniform grid; each wave looks up its tile's fragment set from the cell
 // offset and the k-chunks are visited in offset order (as in the LDS kernel),
 // with ordinary loads so the table stays in L2.
-template <int D, bool TOEP>
+// BSR: Kt holds only the (row tile, k-chunk) tiles of a block-sparse tensor
+// (BlockSparseView bsv): workgroup g owns the row tiles bsv.tiles[4g .. 4g+3]
+// (-1: none) and walks the shared chunk list bsv.chunks[bsv.goff[g] ..
+// bsv.goff[g+1]); wave w's fragments of list entry l sit at fragment set
+// 4 goff[g] + w cnt_g + l, so each wave still streams one contiguous region.
+template <int D, bool TOEP, bool BSR = false>
 __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __restrict__ Kt,
                                                                  const uint4* __restrict__ Mp, int64_t np,
                                                                  int ksplit, float* __restrict__ out, int n,
                                                                  int64_t col_stride, int64_t split_stride,
-                                                                 int ncols, int gnx, int gny, int gnz) {
+                                                                 int ncols, int gnx, int gny, int gnz,
+                                                                 BlockSparseView bsv) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   __shared__ u32x4 bs[2][12][64];
   const int lane = threadIdx.x & 63;
@@ -390,14 +397,19 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
       group = (int)(blockIdx.x / ksplit);
     }
   }
-  if (group >= groups) return;  // whole workgroup, before any barrier
-  const int jb = (int)((int64_t)nJ * ks / ksplit);
-  const int nchunks = (int)((int64_t)nJ * (ks + 1) / ksplit) - jb;
-  const int64_t I = (int64_t)group * 4 + wave;
-  const bool tile_ok = I < n_row_tiles;
+  if (group >= (BSR ? bsv.ngroups : groups)) return;  // whole workgroup, before any barrier
+  // BSR: the group's chunk list (scalar loads, wave-uniform)
+  const int64_t lbase = BSR ? bsv.goff[group] : 0;
+  const int cnt = BSR ? (int)(bsv.goff[group + 1] - lbase) : nJ;
+  const int jb = (int)((int64_t)cnt * ks / ksplit);
+  const int nchunks = (int)((int64_t)cnt * (ks + 1) / ksplit) - jb;
+  const int64_t I = BSR ? (int64_t)bsv.tiles[4 * group + wave] : (int64_t)group * 4 + wave;
+  const bool tile_ok = BSR ? I >= 0 : I < n_row_tiles;
   // a wave past the last tile loads the group's first tile (kept uniform, never consumed)
-  const int64_t Il = tile_ok ? I : (int64_t)group * 4;
-  const u32x4* A = reinterpret_cast<const u32x4*>(Kt) + (TOEP ? 0 : Il * nJ * 12 * 64) + lane;
+  const int64_t Il = tile_ok ? I : (BSR ? (int64_t)bsv.tiles[4 * group] : (int64_t)group * 4);
+  const int64_t aoff = BSR ? (4 * lbase + (int64_t)(tile_ok ? wave : 0) * cnt) * 12 * 64
+                           : (TOEP ? 0 : Il * nJ * 12 * 64);
+  const u32x4* A = reinterpret_cast<const u32x4*>(Kt) + aoff + lane;
   const u32x4* B = reinterpret_cast<const u32x4*>(Mp) + threadIdx.x;
   const bool two_halves = ncols > 32;
   // translation-invariant lookup: this wave's tile (x0 tile, iy, iz) and the
@@ -437,6 +449,10 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
                             (2 * tx0 - x1c_ + cpr - 1);                                                   \
       a_ = A + tset_ * 12 * 64;                                                                           \
       _Pragma("unroll") for (int f = 0; f < 12; ++f) ar[slot][f] = a_[f * 64];                            \
+    } else if (BSR) {                                                                                     \
+      a_ = A + (int64_t)J_ * 12 * 64;                                                                     \
+      _Pragma("unroll") for (int f = 0; f < 12; ++f) ar[slot][f] = __builtin_nontemporal_load(a_ + f * 64); \
+      J_ = bsv.chunks[lbase + J_];                                                                         \
     } else {                                                                                              \
       a_ = A + (int64_t)J_ * 12 * 64;                                                                     \
       _Pragma("unroll") for (int f = 0; f < 12; ++f) ar[slot][f] = __builtin_nontemporal_load(a_ + f * 64); \
@@ -510,17 +526,18 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
   }
 }
 
-template <int D, bool TOEP = false>
+template <int D, bool TOEP = false, bool BSR = false>
 int launch_reg(const uint8_t* K, const uint8_t* Mpack, int n, int nrows, float* out, int64_t col_stride,
                int64_t split_stride, int ncols, int ksplit, hipStream_t stream, int gnx = 0, int gny = 1,
-               int gnz = 1) {
+               int gnz = 1, BlockSparseView bs = BlockSparseView{}) {
   const int64_t np = round_up(n, kTileM);
-  const int groups = (int)((round_up(nrows, kTileM) / kTileM + 3) / 4);
+  const int groups = BSR ? bs.ngroups : (int)((round_up(nrows, kTileM) / kTileM + 3) / 4);
   int64_t blocks = (int64_t)groups * ksplit;
   if (ksplit % 8 != 0 && 8 % ksplit == 0) blocks = round_up(blocks, 8);
-  hipLaunchKernelGGL((gemm_bf16x2_reg_kernel<D, TOEP>), dim3((unsigned)blocks), dim3(256), 0, stream,
+  if (blocks == 0) return kOk;
+  hipLaunchKernelGGL((gemm_bf16x2_reg_kernel<D, TOEP, BSR>), dim3((unsigned)blocks), dim3(256), 0, stream,
                      reinterpret_cast<const uint4*>(K), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out, nrows,
-                     col_stride, split_stride, ncols, gnx, gny, gnz);
+                     col_stride, split_stride, ncols, gnx, gny, gnz, bs);
   MT_HIP(hipGetLastError());
   return kOk;
 }
@@ -978,6 +995,144 @@ int gemm_ksplit_table(int nx, int ny, int nz) {
   while (groups * ks < 256 && rows % (ks * 2) == 0) ks *= 2;
   if (forced_ksplit() > 0 && rows % forced_ksplit() == 0) ks = forced_ksplit();
   return ks;
+}
+
+namespace {
+__device__ inline float bf16_round(float v) {
+  return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, __float2bfloat16(v)) << 16);
+}
+
+// Near-band correction of the split-BF16 product on a uniform grid.  The
+// batched GEMM forms khi mhi + khi mlo + klo mhi for every pair; with K32 =
+// khi + klo + dK and m32 = mhi + mlo + dm (both exact) the FP32 product adds
+//   khi dm + klo (mlo + dm) + dK m32,
+// which this kernel sums over the sources within R cells in x and y and Rz
+// layers in z of each receiver, where the tensor entries are largest.  A
+// workgroup owns 64 x 4 receivers of one layer and one state; per source
+// layer it stages (m32, dm, mlo + dm) of the 3 components for the (64 + 2R) x
+// (4 + 2R) neighbourhood in LDS (zero outside the grid).  The table entries
+// of an offset are the same for the whole workgroup.  F64: the state comes
+// from the LL descriptors (FP64 stage arguments, rounded to FP32 as the
+// product's packing rounds them), else from FP32 states m32 + s * m_stride.
+constexpr int kNearTX = 64, kNearTY = 4;
+
+template <bool F64>
+__global__ void __launch_bounds__(256) near_corr_kernel(const float* __restrict__ table, int R, int Rz, int nx,
+                                                        int ny, int nz, const LlgDesc* __restrict__ descs,
+                                                        const float* __restrict__ m32, int64_t m_stride,
+                                                        float* __restrict__ out, int64_t col_stride, float pref,
+                                                        int accumulate) {
+  extern __shared__ float halo[];  // [hy][hx][9]: (m32, dm, mlo + dm) for x, y, z
+  const int n = nx * ny * nz;
+  const int tilesx = (nx + kNearTX - 1) / kNearTX, tilesy = (ny + kNearTY - 1) / kNearTY;
+  int b = blockIdx.x;
+  const int tx0 = (b % tilesx) * kNearTX;
+  b /= tilesx;
+  const int ty0 = (b % tilesy) * kNearTY;
+  const int iz = b / tilesy;
+  const int s = blockIdx.y;
+  const int lx = threadIdx.x % kNearTX, ly = threadIdx.x / kNearTX;
+  const int ix = tx0 + lx, iy = ty0 + ly;
+  const int hx = kNearTX + 2 * R, hy = kNearTY + 2 * R, w = 2 * R + 1;
+  const double* m64 = F64 ? descs[s].m : nullptr;
+  const float* mf = F64 ? nullptr : m32 + (int64_t)s * m_stride;
+  const bool own = ix < nx && iy < ny;
+  // (a, b) -> component xx xy xz yy yz zz
+  const int cab[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+  for (int oz = -Rz; oz <= Rz; ++oz) {  // receiver - source layer
+    const int jz = iz - oz;
+    if (jz < 0 || jz >= nz) continue;  // uniform over the workgroup
+    __syncthreads();                   // the previous layer's reads are done
+    for (int h = threadIdx.x; h < hx * hy; h += blockDim.x) {
+      const int gx = tx0 - R + h % hx, gy = ty0 - R + h / hx;
+      float v[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) v[q] = 0.0f;
+      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+        const int64_t j = gx + (int64_t)gy * nx + (int64_t)jz * nx * ny;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float f = F64 ? (float)m64[(int64_t)c * n + j] : mf[(int64_t)c * n + j];
+          const float hi = bf16_round(f);
+          const float lo = bf16_round(f - hi);
+          const float dm = (f - hi) - lo;
+          v[3 * c] = f;
+          v[3 * c + 1] = dm;
+          v[3 * c + 2] = lo + dm;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 9; ++q) halo[h * 9 + q] = v[q];
+    }
+    __syncthreads();
+    if (own) {
+      for (int oy = -R; oy <= R; ++oy)
+        for (int ox = -R; ox <= R; ++ox) {
+          // source (ix - ox, iy - oy, jz)
+          const float* hv = halo + ((ly + R - oy) * hx + (lx + R - ox)) * 9;
+          const float* t = table + ((int64_t)(((oz + Rz) * w + (oy + R)) * w + (ox + R)) * 6) * 3;
+          float mv[9];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) mv[q] = hv[q];
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb) {
+              const float* tc = t + 3 * cab[a][bb];
+              acc[a] = fmaf(tc[0], mv[3 * bb + 1], acc[a]);
+              acc[a] = fmaf(tc[1], mv[3 * bb + 2], acc[a]);
+              acc[a] = fmaf(tc[2], mv[3 * bb], acc[a]);
+            }
+        }
+    }
+  }
+  if (!own) return;
+  const int64_t i = ix + (int64_t)iy * nx + (int64_t)iz * nx * ny;
+  float* o = out + (int64_t)s * col_stride + i;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) o[(int64_t)a * n] = accumulate ? o[(int64_t)a * n] + pref * acc[a] : pref * acc[a];
+}
+}  // namespace
+
+int near_band_rz(int R, int nz) { return R < nz - 1 ? R : nz - 1; }
+
+int64_t near_table_floats(int R, int nz) {
+  const int rz = near_band_rz(R, nz);
+  return (int64_t)(2 * R + 1) * (2 * R + 1) * (2 * rz + 1) * 18;
+}
+
+int launch_near_correction(const float* table, int R, int nx, int ny, int nz, const LlgDesc* descs, const float* m32,
+                           int64_t m_stride, int nstates, float* out, int64_t col_stride, float pref, bool accumulate,
+                           hipStream_t stream) {
+  if (R < 1 || R > 4 || nx < 1 || ny < 1 || nz < 1 || nstates < 1 || nstates > 65535) {
+    set_error("near-band correction: radius 1..4, a non-empty grid and 1..65535 states");
+    return kErrArg;
+  }
+  const int Rz = near_band_rz(R, nz);
+  const int tiles = ((nx + kNearTX - 1) / kNearTX) * ((ny + kNearTY - 1) / kNearTY) * nz;
+  const size_t lds = (size_t)(kNearTX + 2 * R) * (kNearTY + 2 * R) * 9 * sizeof(float);
+  if (descs)
+    hipLaunchKernelGGL(near_corr_kernel<true>, dim3((unsigned)tiles, (unsigned)nstates), dim3(256), lds, stream, table,
+                       R, Rz, nx, ny, nz, descs, m32, m_stride, out, col_stride, pref, accumulate ? 1 : 0);
+  else
+    hipLaunchKernelGGL(near_corr_kernel<false>, dim3((unsigned)tiles, (unsigned)nstates), dim3(256), lds, stream,
+                       table, R, Rz, nx, ny, nz, descs, m32, m_stride, out, col_stride, pref, accumulate ? 1 : 0);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_gemm_bf16x2_bsr(const uint8_t* frags, const BlockSparseView& bs, const uint8_t* Mpack, int n, float* out,
+                           int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+  if (bs.ngroups == 0) {  // no kept entry anywhere: the partial sums are zero
+    for (int k = 0; k < ksplit; ++k)
+      for (int c = 0; c < ncols; ++c)
+        MT_HIP(hipMemsetAsync(out + (int64_t)k * split_stride + (int64_t)c * col_stride, 0, 3 * (size_t)n * sizeof(float),
+                              stream));
+    return kOk;
+  }
+  return launch_reg<4, false, true>(frags, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream, 0, 1, 1,
+                                    bs);
 }
 
 int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, const uint8_t* Mpack, float* out,

@@ -23,13 +23,18 @@ enum Status {
   kErrUnsupported = -6,  // input outside the supported subset (e.g. a non-prism tile)
 };
 
+// A failed allocation (hipErrorOutOfMemory) returns kErrAlloc, any other
+// runtime failure kErrHip; the message names the call and its location.  The
+// runtime's last-error state is cleared, so a later hipGetLastError() check
+// (after a kernel launch) does not report this failure again.
 #define MT_HIP(expr)                                                                              \
   do {                                                                                            \
     hipError_t _e = (expr);                                                                       \
     if (_e != hipSuccess) {                                                                       \
+      (void)hipGetLastError();                                                                    \
       ::mt::set_error(std::string("HIP error '") + hipGetErrorString(_e) + "' at " + __FILE__ + \
                       ":" + std::to_string(__LINE__) + " (" #expr ")");                          \
-      return ::mt::kErrHip;                                                                       \
+      return _e == hipErrorOutOfMemory ? ::mt::kErrAlloc : ::mt::kErrHip;                         \
     }                                                                                             \
   } while (0)
 

@@ -41,6 +41,18 @@ int gemm_ksplit_for(int n, int tiles_per_group);  // k-split of the batched GEMM
 int gemm_ksplit_rows(int n, int nrows, int tiles_per_group);  // same for nrows receiver rows of n sources
 int gemm_ksplit_table(int nx, int ny, int nz);  // k-split of the kernel launch_gemm_bf16x2_toeplitz picks
 int gemm_ksplit_dense(int n, int nrows);  // k-split of the dense-tensor kernel launch_gemm_bf16x2{,_rows} picks
+// Block-sparse split-BF16 tensor (threshold modes with precision bf16x2): only
+// the 32-row x 16-source tiles holding a kept entry are stored, in the dense
+// fragment order.  Row tiles are grouped four to a workgroup; a group's tiles
+// share one sorted list of k-chunks (the union of their non-empty chunks);
+// wave w's fragments of list entry l are fragment set 4 goff[g] + w cnt_g + l
+// (12 fragments of 1 KiB), cnt_g = goff[g+1] - goff[g].
+struct BlockSparseView {
+  int ngroups = 0;
+  const int* tiles = nullptr;     // 4 per group: row tile index, -1 none (the first is always a tile)
+  const int64_t* goff = nullptr;  // ngroups + 1 list offsets
+  const int* chunks = nullptr;    // k-chunk of each list entry
+};
 // out[ks*split_stride + col*col_stride + c*n + i] = k-split partial of K m
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
@@ -53,9 +65,51 @@ int launch_gemm_bf16x2_rows(const uint8_t* Krows, const uint8_t* Mpack, int n, i
 // an nx x ny x nz uniform grid (n = nx ny nz, nx % 32 == 0).
 int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, const uint8_t* Mpack, float* out,
                                 int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
+// Same product over a block-sparse tensor (fragment sets `frags`, structure `bs`).
+int launch_gemm_bf16x2_bsr(const uint8_t* frags, const BlockSparseView& bs, const uint8_t* Mpack, int n, float* out,
+                           int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
+
+struct BlockSparse {
+  int n = 0, ngroups = 0, ksplit = 1;
+  float threshold = 0.0f;
+  int64_t entries = 0;            // list entries over all groups (each 4 x 12 KiB of fragments)
+  int64_t tiles_nonempty = 0;     // (row tile, chunk) tiles holding a kept entry
+  int64_t nnz[6] = {0, 0, 0, 0, 0, 0};  // kept entries per component
+  int* tiles = nullptr;
+  int64_t* goff = nullptr;
+  int* chunks = nullptr;
+  int* egroup = nullptr;          // group of each list entry (build only)
+  uint8_t* frags = nullptr;
+  // grid_nx > 0 with grid_nx % 32 == 0: uniform-grid rows of grid_nx cells, so
+  // row tiles of consecutive grid rows can share a group
+  int build(const float* K32, int n, float thr, int grid_nx, hipStream_t stream);
+  BlockSparseView view() const { return BlockSparseView{ngroups, tiles, goff, chunks}; }
+  int64_t bytes() const { return entries * 4 * 12 * 1024; }
+  void release();
+  ~BlockSparse() { release(); }
+};
 
 // Threshold-sparse demag tensor: six CSR matrices (xx xy xz yy yz zz), rows =
 // receivers, FP32 values, entries with |K| > threshold.
+// Workspace of one batched split-BF16 product (api.hip): the packed states
+// (zeroed before every 64-state group: padded sources must read as zero) and
+// the k-split partial sums.
+struct Bf16x2Work {
+  int ks = 1, cols = 1;
+  size_t packed_bytes = 0, part_floats = 0;
+  size_t bytes() const { return (size_t)round_up((int64_t)packed_bytes, 256) + part_floats * sizeof(float); }
+};
+// ks > 0 overrides the k-split of the dense / table kernels (block-sparse tensors carry their own)
+Bf16x2Work bf16x2_work(int n, int nrows, int nx, int ny, int nz, int nstates, int ks = 0);
+// The batched product h = pref * (K m) for nstates states in groups of 64 on
+// the null stream, with the caller's workspace `ws` of w.bytes(); synchronous.
+// nx > 0: K is the translation-invariant table of an nx x ny x nz grid; nrows <
+// n: K is a row block of the dense tensor; bsv: K holds the fragments of a
+// block-sparse tensor.
+int apply_bf16x2_in(const void* K, int n, int nrows, int nx, int ny, int nz, const float* m, float* h, int nstates,
+                    float pref, const Bf16x2Work& w, uint8_t* ws, const BlockSparseView* bsv = nullptr);
+int device_ok();
+
 struct SparseDemag {
   int n = 0;
   float threshold = 0.0f;
@@ -67,6 +121,7 @@ struct SparseDemag {
   float* mT = nullptr;        // transposed staging for the many-state product
   float* hT = nullptr;
   int tcols = 0;
+  BlockSparse* blocks = nullptr;  // split-BF16 block-sparse form (precision bf16x2) instead of the CSR
   std::vector<std::vector<int>> host_cols;    // components loaded from host CSR
   std::vector<std::vector<float>> host_vals;
   int build(const float* K32, int n, float thr, hipStream_t stream);
@@ -110,6 +165,21 @@ struct LlgDesc {
 };
 
 int launch_llg(const LlgGrid& g, const LlgDesc* d_descs, int nstates, hipStream_t stream);
+
+// Near-band correction of the split-BF16 product on a uniform nx x ny x nz
+// grid: the FP32 product's difference from the three split products, summed
+// over the sources within R cells in x and y and min(R, nz - 1) layers in z.
+// Table (prism_tensor.hip): per offset and component khi, klo, dK.
+int near_band_rz(int R, int nz);
+int64_t near_table_floats(int R, int nz);
+// thr: entries with |K32| <= thr are zero (the threshold-sparse modes); -1 keeps all
+int launch_near_table(int nx, int ny, int nz, double Lx, double Ly, double Lz, int R, int Rz, float thr, float* table,
+                      hipStream_t stream);
+// out[s*col_stride + a*n + i] (+)= pref * correction of state s; the state is
+// read from descs[s].m (FP64) when descs is set, else from m32 + s*m_stride.
+int launch_near_correction(const float* table, int R, int nx, int ny, int nz, const LlgDesc* descs, const float* m32,
+                           int64_t m_stride, int nstates, float* out, int64_t col_stride, float pref, bool accumulate,
+                           hipStream_t stream);
 
 // ---- (d) vector operations -------------------------------------------------------
 constexpr int kMaxTerms = 12;

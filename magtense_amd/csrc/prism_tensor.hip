@@ -699,6 +699,55 @@ int launch_toeplitz_assemble(int nx, int ny, int nz, double Lx, double Ly, doubl
   return kOk;
 }
 
+namespace {
+// Near-band table of a uniform grid (one thread per source offset): for the
+// offset (ox, oy, oz) = receiver - source cell index, |ox|, |oy| <= R, |oz| <=
+// Rz, the FP32 tensor value K32 of every component and its split as the
+// batched product uses it: khi = bf16(K32), klo = bf16(K32 - khi), dK = K32 -
+// khi - klo (exact in FP32).  Values with |K32| <= thr (a threshold-sparse
+// tensor's dropped entries) are zero.  The offset is formed from the centres of one
+// representative pair, as the translation-invariant table forms it.
+__global__ void near_table_kernel(int nx, int ny, int nz, double Lx, double Ly, double Lz, int R, int Rz, float thr,
+                                  float* __restrict__ out) {
+  const int w = 2 * R + 1;
+  const int no = w * w * (2 * Rz + 1);
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= no) return;
+  const int ox = o % w - R, oy = (o / w) % w - R, oz = o / (w * w) - Rz;
+  const double dx = nx > 1 ? Lx / nx : Lx, dy = ny > 1 ? Ly / ny : Ly, dz = nz > 1 ? Lz / nz : Lz;
+  auto centre = [](int m, double L, double d, int i) { return m > 1 ? -L / 2 + i * d + d / 2 : 0.0; };
+  const int ix = ox > 0 ? ox : 0, jx = ox > 0 ? 0 : -ox;
+  const int iy = oy > 0 ? oy : 0, jy = oy > 0 ? 0 : -oy;
+  const int iz = oz > 0 ? oz : 0, jz = oz > 0 ? 0 : -oz;
+  double k6[6];
+  prism_k6(dx, dy, dz, centre(nx, Lx, dx, ix) - centre(nx, Lx, dx, jx), centre(ny, Ly, dy, iy) - centre(ny, Ly, dy, jy),
+           centre(nz, Lz, dz, iz) - centre(nz, Lz, dz, jz), k6);
+  for (int c = 0; c < 6; ++c) {
+    float k32 = (float)k6[c];
+    if (!(fabsf(k32) > thr)) k32 = 0.0f;  // an entry the threshold drops stays dropped
+    const float khi = bf16_value(bf16_bits(k32));
+    const float klo = bf16_value(bf16_bits(k32 - khi));
+    float* t = out + ((int64_t)o * 6 + c) * 3;
+    t[0] = khi;
+    t[1] = klo;
+    t[2] = (k32 - khi) - klo;
+  }
+}
+}  // namespace
+
+int launch_near_table(int nx, int ny, int nz, double Lx, double Ly, double Lz, int R, int Rz, float thr, float* table,
+                      hipStream_t stream) {
+  if (R < 0 || Rz < 0) {
+    set_error("near-band radius must be >= 0");
+    return kErrArg;
+  }
+  const int no = (2 * R + 1) * (2 * R + 1) * (2 * Rz + 1);
+  hipLaunchKernelGGL(near_table_kernel, dim3((unsigned)((no + 63) / 64)), dim3(64), 0, stream, nx, ny, nz, Lx, Ly, Lz,
+                     R, Rz, thr, table);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
 int launch_split(const float* K, int n, uint8_t* split, hipStream_t stream) {
   int64_t np = round_up(n, kTileM);
   int64_t threads = np * (np / 8);

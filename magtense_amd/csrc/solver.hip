@@ -27,6 +27,7 @@
 #include <cmath>
 #include <complex>
 #include <coroutine>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -37,6 +38,18 @@
 #include "../../include/magtense_hip.h"
 
 namespace mt {
+
+// Round to the nearest BF16 (ties to even), as __float2bfloat16 does for
+// finite values, returned as the FP32 value it represents.
+static float bf16_round_host(float v) {
+  uint32_t u;
+  std::memcpy(&u, &v, 4);
+  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+
 
 namespace {
 
@@ -287,6 +300,8 @@ struct Solver {
   double* xv = nullptr;
   long long exch_nnz_host = 0;
   uint8_t* Ktab = nullptr;    // translation-invariant fragment table (bf16x2 on uniform grids)
+  float* near_tab = nullptr;  // near-band correction table (bf16x2 on uniform grids, dense tensor)
+  int near_r = 0;             // its radius (0: no correction)
   int demag_approx = 1;       // 1 dense, 2 threshold, 4 threshold fraction
   SparseDemag sparse;
   double *jfact = nullptr, *kfact = nullptr, *aniso = nullptr;
@@ -312,13 +327,14 @@ struct Solver {
   // statistics
   long long ticks = 0, eval_ticks = 0, demag_passes = 0;
   double gemm_ms = 0.0;
+  double near_ms = 0.0;  // of gemm_ms: the near-band correction
   // timing mode: per-kernel event times and algorithmic bytes of the LL
   // right-hand side and of the stepper's vector operations
   double vec_ms = 0.0, llg_ms = 0.0, vec_bytes = 0.0, llg_bytes = 0.0;
   long long vec_launches = 0, llg_launches = 0;
   bool timing = false;
   long long total_target = 0;   // > 0: stop once the states' accepted steps sum to this
-  hipEvent_t ev[7] = {};  // timing: vec-only, pre-vec, GEMM, LL kernel, post-vec boundaries
+  hipEvent_t ev[8] = {};  // timing: vec-only, pre-vec, GEMM, LL kernel, post-vec boundaries, near band
 
   ~Solver() { release(); }
 
@@ -326,7 +342,7 @@ struct Solver {
     auto f = [](void* p) {
       if (p) (void)hipFree(p);
     };
-    f(K32); f(Ksplit); f(Ktab); f(jfact); f(kfact); f(aniso); f(ms32); f(pool); f(yout); f(fout); f(packf); f(packb);
+    f(K32); f(Ksplit); f(Ktab); f(near_tab); f(jfact); f(kfact); f(aniso); f(ms32); f(pool); f(yout); f(fout); f(packf); f(packb);
     f(hbuf); f(partials); f(results); f(arena_d); f(xs); f(xe); f(xc); f(xv);
     sparse.release();
     if (h_results) (void)hipHostFree(h_results);
@@ -335,7 +351,7 @@ struct Solver {
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     xs = xe = xc = nullptr; xv = nullptr;
-    K32 = nullptr; Ksplit = nullptr; Ktab = nullptr; jfact = kfact = aniso = nullptr; ms32 = nullptr; pool = yout = fout = nullptr;
+    K32 = nullptr; Ksplit = nullptr; Ktab = nullptr; near_tab = nullptr; jfact = kfact = aniso = nullptr; ms32 = nullptr; pool = yout = fout = nullptr;
     packf = nullptr; packb = nullptr; hbuf = nullptr; partials = results = nullptr; arena_d = nullptr;
     h_results = nullptr; arena_h = nullptr; stream = nullptr;
     for (hipEvent_t& e : ev) e = nullptr;
@@ -1111,7 +1127,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       L.fields = s.op.fields;
       L.hpart = hbuf + (size_t)k * N3;
       L.split_stride = (int64_t)hcols * N3;
-      L.nsplit = precision == MT_PREC_BF16X2 ? ksplit : 1;
+      L.nsplit = precision == MT_PREC_BF16X2 ? ksplit + (near_r > 0 ? 1 : 0) : 1;
       hext_at(s, s.op.t, L.hext);
       L.alpha = alpha_at(s.op.t);
       L.gamma = gamma;
@@ -1137,8 +1153,16 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       if (Ktab)
         rc = launch_gemm_bf16x2_toeplitz(Ktab, nx, ny, nz, packb, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit,
                                          stream);
+      else if (sparse.blocks)
+        rc = launch_gemm_bf16x2_bsr(sparse.blocks->frags, sparse.blocks->view(), packb, n, hbuf, (int64_t)N3,
+                                    (int64_t)hcols * N3, S, ksplit, stream);
       else
         rc = launch_gemm_bf16x2(Ksplit, packb, n, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit, stream);
+      // the near band's FP32 correction goes into one more partial-sum slot
+      if (rc == kOk && near_r > 0 && timing) MT_HIP(hipEventRecord(ev[7], stream));
+      if (rc == kOk && near_r > 0)
+        rc = launch_near_correction(near_tab, near_r, nx, ny, nz, dl, nullptr, 0, S,
+                                    hbuf + (size_t)ksplit * hcols * N3, (int64_t)N3, 1.0f, false, stream);
     } else if (demag_approx != MT_DEMAG_DENSE) {
       rc = sparse.apply(packf, (int64_t)N3, S, 1.0f, hbuf, stream);
     } else {
@@ -1164,6 +1188,11 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       MT_HIP(hipEventElapsedTime(&ms[2], ev[4], ev[5]));
       MT_HIP(hipEventElapsedTime(&ms[3], ev[5], ev[6]));
       gemm_ms += ms[1];
+      if (near_r > 0 && precision == MT_PREC_BF16X2) {
+        float mn = 0;
+        MT_HIP(hipEventElapsedTime(&mn, ev[7], ev[4]));
+        near_ms += mn;
+      }
       llg_ms += ms[2];
       llg_launches += 1;
       llg_bytes += llg_launch_bytes(ld);
@@ -1289,9 +1318,14 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   const int approx = q->demag_approx == 0 ? MT_DEMAG_DENSE : q->demag_approx;
   MT_CHECK(approx == MT_DEMAG_DENSE || approx == MT_DEMAG_THRESHOLD || approx == MT_DEMAG_THRESHOLD_FRACTION,
            "demag_approx must be 1 (dense), 2 (threshold) or 4 (threshold fraction)");
-  MT_CHECK(approx == MT_DEMAG_DENSE || q->precision == MT_PREC_FP32,
-           "the threshold-sparse demag product runs in FP32 (precision MT_PREC_FP32)");
   MT_CHECK(q->Ms && q->m0, "Ms and m0 are required");
+  {  // the reference integrates NaNs silently; refuse them before any device work
+    const size_t n3c = (size_t)3 * q->nx * q->ny * q->nz;
+    const size_t nm = (q->m0_per_state ? (size_t)q->n_states : 1) * n3c;
+    for (size_t k = 0; k < nm; ++k) MT_CHECK(std::isfinite(q->m0[k]), "m0 holds a NaN or infinite value");
+    for (size_t k = 0; k < n3c / 3; ++k)
+      MT_CHECK(std::isfinite(q->Ms[k]) && q->Ms[k] != 0.0, "Ms must be finite and non-zero in every cell");
+  }
   std::unique_ptr<Solver> S(new Solver());
   Solver& P = *S;
   P.nx = q->nx; P.ny = q->ny; P.nz = q->nz;
@@ -1359,12 +1393,15 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   for (hipEvent_t& e : P.ev) MT_HIP(hipEventCreate(&e));
   // tensor
   const size_t nn = (size_t)n * n;
-  const bool need32 = P.precision == MT_PREC_FP32 || q->keep_fp32_tensor;
-  const bool can_toep = !unstructured && P.precision == MT_PREC_BF16X2 && !q->K_in && P.nx >= 32 && P.nx % 32 == 0;
+  // the threshold modes build their sparse form from the FP32 tensor (CSR in
+  // FP32, block-sparse split-BF16 tiles in bf16x2)
+  const bool need32 = P.precision == MT_PREC_FP32 || q->keep_fp32_tensor || approx != MT_DEMAG_DENSE;
+  const bool can_toep = !unstructured && P.precision == MT_PREC_BF16X2 && !q->K_in && P.nx >= 32 && P.nx % 32 == 0 &&
+                        approx == MT_DEMAG_DENSE;
   MT_CHECK(q->toeplitz != 1 || can_toep,
            "the translation-invariant tensor needs a uniform grid, precision bf16x2, nx % 32 == 0 and no K_in");
   const bool use_toep = can_toep && q->toeplitz >= 0;
-  const bool needsplit = P.precision == MT_PREC_BF16X2 && !use_toep;
+  const bool needsplit = P.precision == MT_PREC_BF16X2 && !use_toep && approx == MT_DEMAG_DENSE;
   if (use_toep) {
     MT_HIP(hipMalloc(&P.Ktab, (size_t)toeplitz_tensor_bytes(P.nx, P.ny, P.nz)));
     int rc = launch_toeplitz_assemble(P.nx, P.ny, P.nz, q->Lx, q->Ly, q->Lz, P.Ktab, P.stream);
@@ -1452,7 +1489,16 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       int rc = threshold_for_fraction(P.K32, n, q->demag_threshold, &thr, P.stream);
       if (rc) return rc;
     }
-    int rc = P.sparse.build(P.K32, n, thr, P.stream);
+    int rc = kOk;
+    if (P.precision == MT_PREC_BF16X2) {
+      P.sparse.blocks = new BlockSparse();
+      rc = P.sparse.blocks->build(P.K32, n, thr, unstructured ? 0 : P.nx, P.stream);
+      P.sparse.n = n;
+      P.sparse.threshold = thr;
+      for (int c = 0; c < 6; ++c) P.sparse.nnz[c] = P.sparse.blocks->nnz[c];
+    } else {
+      rc = P.sparse.build(P.K32, n, thr, P.stream);
+    }
     if (rc) return rc;
     if (!q->keep_fp32_tensor) {
       MT_HIP(hipStreamSynchronize(P.stream));
@@ -1518,10 +1564,53 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   }
   P.hcols = std::min(P.B, kMaxCols);
   if (P.precision == MT_PREC_BF16X2) {
-    P.ksplit = q->ksplit > 0 ? q->ksplit : (P.Ktab ? gemm_ksplit_table(P.nx, P.ny, P.nz) : gemm_ksplit_dense(n, n));
+    P.ksplit = q->ksplit > 0 ? q->ksplit
+                             : (P.Ktab ? gemm_ksplit_table(P.nx, P.ny, P.nz)
+                                       : (P.sparse.blocks ? P.sparse.blocks->ksplit : gemm_ksplit_dense(n, n)));
     MT_HIP(hipMalloc(&P.packb, (size_t)packed_m_bytes(n)));
     MT_HIP(hipMemsetAsync(P.packb, 0, (size_t)packed_m_bytes(n), P.stream));
-    MT_HIP(hipMalloc(&P.hbuf, (size_t)P.ksplit * P.hcols * n3 * sizeof(float)));
+    // near-band correction of the split product (uniform grids: dense tensor,
+    // table or block-sparse tiles): MT_NEAR_RADIUS cells (default 2, 0 switches it off)
+    if (!unstructured) {
+      const char* env = std::getenv("MT_NEAR_RADIUS");
+      P.near_r = env ? std::atoi(env) : 2;
+      if (P.near_r < 0 || P.near_r > 4) P.near_r = 2;
+    }
+    if (P.near_r > 0) {
+      const int R = P.near_r, Rz = near_band_rz(R, P.nz);
+      // threshold modes: the entries the threshold drops get no correction either
+      const float thr = approx == MT_DEMAG_DENSE ? -1.0f : P.sparse.threshold;
+      MT_HIP(hipMalloc(&P.near_tab, (size_t)near_table_floats(R, P.nz) * sizeof(float)));
+      if (q->K_in) {
+        // the caller's tensor: each offset's values from the first pair of cells that has it
+        std::vector<float> tab((size_t)near_table_floats(R, P.nz), 0.0f);
+        const int w = 2 * R + 1;
+        for (int o = 0; o < w * w * (2 * Rz + 1); ++o) {
+          const int ox = o % w - R, oy = (o / w) % w - R, oz = o / (w * w) - Rz;
+          if (std::abs(ox) >= P.nx || std::abs(oy) >= P.ny || std::abs(oz) >= P.nz) continue;
+          const size_t i = (size_t)std::max(ox, 0) + (size_t)std::max(oy, 0) * P.nx +
+                           (size_t)std::max(oz, 0) * P.nx * P.ny;
+          const size_t j = (size_t)std::max(-ox, 0) + (size_t)std::max(-oy, 0) * P.nx +
+                           (size_t)std::max(-oz, 0) * P.nx * P.ny;
+          for (int c = 0; c < 6; ++c) {
+            float k32 = q->K_in[(size_t)c * nn + i * n + j];
+            if (!(std::fabs(k32) > thr)) k32 = 0.0f;
+            const float khi = bf16_round_host(k32);
+            const float klo = bf16_round_host(k32 - khi);
+            float* t = &tab[((size_t)o * 6 + c) * 3];
+            t[0] = khi;
+            t[1] = klo;
+            t[2] = (k32 - khi) - klo;
+          }
+        }
+        MT_HIP(hipMemcpyAsync(P.near_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, P.stream));
+        MT_HIP(hipStreamSynchronize(P.stream));
+      } else {
+        int rc = launch_near_table(P.nx, P.ny, P.nz, q->Lx, q->Ly, q->Lz, R, Rz, thr, P.near_tab, P.stream);
+        if (rc) return rc;
+      }
+    }
+    MT_HIP(hipMalloc(&P.hbuf, (size_t)(P.ksplit + (P.near_r > 0 ? 1 : 0)) * P.hcols * n3 * sizeof(float)));
   } else {
     P.ksplit = 1;
     P.hcols = P.B;
@@ -1632,6 +1721,7 @@ int mt_solver_results(mt_solver* h, double* t_out, double* y_out, double* fields
       o.ticks = P.ticks;
       o.demag_passes = P.demag_passes;
       o.demag_ms = P.gemm_ms;
+      o.near_ms = P.near_ms;
       o.vec_ms = P.vec_ms;
       o.llg_ms = P.llg_ms;
       o.vec_bytes = P.vec_bytes;
@@ -1657,7 +1747,7 @@ int mt_solver_state(mt_solver* h, double* y) {
 int mt_solver_set_timing(mt_solver* h, int on) {
   mt::Solver& P = *reinterpret_cast<mt::Solver*>(h);
   P.timing = on != 0;
-  P.gemm_ms = P.vec_ms = P.llg_ms = P.vec_bytes = P.llg_bytes = 0.0;
+  P.gemm_ms = P.near_ms = P.vec_ms = P.llg_ms = P.vec_bytes = P.llg_bytes = 0.0;
   P.vec_launches = P.llg_launches = 0;
   return 0;
 }

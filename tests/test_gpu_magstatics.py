@@ -19,7 +19,7 @@ from test_oracle_tiles import comsol_case, comsol_check
 
 
 def _random_tiles(rng, nt, sym_every=0):
-    from magtense_amd import magstatics as ms
+    from magtense import magstatics as ms
 
     t = ms.Tiles(nt, size=rng.uniform(0.05, 0.4, (nt, 3)).tolist(), offset=rng.uniform(-1, 1, (nt, 3)).tolist(),
                  tile_type=2, M_rem=1.1 / (4e-7 * math.pi), easy_axis=rng.normal(size=(nt, 3)).tolist())
@@ -36,7 +36,7 @@ def _oracle_args(t):
 
 @pytest.mark.parametrize("nt,npt,sym", [(1, 300, 0), (37, 129, 0), (300, 260, 5)])
 def test_tiles_tensor_and_field(oracle, nt, npt, sym):
-    from magtense_amd import magstatics as ms
+    from magtense import magstatics as ms
 
     rng = np.random.default_rng(nt + npt)
     t = _random_tiles(rng, nt, sym)
@@ -58,7 +58,7 @@ def test_tiles_tensor_and_field(oracle, nt, npt, sym):
 
 
 def test_comsol_prism_lines_gpu(oracle):
-    from magtense_amd import magstatics as ms
+    from magtense import magstatics as ms
 
     def fn(case, pts):
         t = ms.Tiles(1, size=case["size"], offset=case["offset"], rot=case["rot"], tile_type=2,

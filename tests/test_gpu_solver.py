@@ -32,7 +32,7 @@ def _oracle_run(oracle, res, L, m0, hfield, t_end, nt, nt_h, **kw):
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x2"])
 def test_run_simulation_matches_oracle(oracle, precision):
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [16, 4, 1], [250e-9, 62.5e-9, 3e-9]
     m0 = _random_m(64, 1)
@@ -45,17 +45,32 @@ def test_run_simulation_matches_oracle(oracle, precision):
     err = np.abs(M[:, :, 0, :] - yr).max()
     print(precision, "max traj err", err, "gpu stats", p.last_stats[0], "oracle", st)
     assert err < 1e-4
-    # returned fields (negated, as the reference stores them)
-    assert np.abs(Hexc[:, :, 0, :] - Hr[0]).max() <= 1e-3 * np.abs(Hr[0]).max()
+    # Returned fields (negated, as the reference stores them) are linear in the
+    # stored m, so the trajectory difference bounds theirs: exchange by
+    # 2 Jfact ||A||_inf max|dm| (A the Neumann Laplacian), demag by
+    # Ms max_i sum_b sum_j |K_ab(i, j)| max|dm| plus the product's own rounding
+    # (FP32: 1e-6, split-BF16: 1e-5 of the largest field).
+    ms, a0 = 8e5, 1.3e-11
+    rowptr, _, val = oracle.exchange_csr(res, L)
+    a_inf = max(np.abs(val[rowptr[i]:rowptr[i + 1]]).sum() for i in range(64))
+    bar_exc = 2 * a0 / (MU0 * ms) * a_inf * err + 1e-12 * np.abs(Hr[0]).max()
+    K = oracle.tensor_uniform(res, L)
+    comp = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    k_inf = max(sum(np.abs(K[comp[a][b]]).sum(axis=1) for b in range(3)).max() for a in range(3))
+    bar_dem = ms * k_inf * err + (1e-5 if precision == "bf16x2" else 1e-6) * np.abs(Hr[2]).max()
+    d_exc = np.abs(Hexc[:, :, 0, :] - Hr[0]).max()
+    d_dem = np.abs(Hdem[:, :, 0, :] - Hr[2]).max()
+    print(f"fields: exchange {d_exc:.3e} (bar {bar_exc:.3e}), demag {d_dem:.3e} (bar {bar_dem:.3e})")
+    assert d_exc <= bar_exc
     assert np.allclose(Hext[:, :, 0, :], Hr[1])
-    assert np.abs(Hdem[:, :, 0, :] - Hr[2]).max() <= 2e-3 * np.abs(Hr[2]).max()
+    assert d_dem <= bar_dem
     # step sequence: same algorithm, ulp-level differences only
     if precision == "fp32":
         assert abs(p.last_stats[0]["steps"] - st["steps"]) <= max(2, 0.05 * st["steps"])
 
 
 def test_batch_equals_individual_runs(oracle):
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [8, 4, 1], [40e-9, 20e-9, 3e-9]
     n = 32
@@ -78,7 +93,7 @@ def test_batch_beyond_one_demag_group(oracle, precision):
     one-state run bit for bit, on both sides of the split; in FP32 it also
     follows the oracle run (split-BF16 against the oracle: the single-run tests
     above, from starts whose trajectories do not amplify the 4e-6 field error)."""
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [8, 4, 1], [40e-9, 20e-9, 3e-9]
     n, B = 32, 130
@@ -102,7 +117,7 @@ def test_batch_beyond_one_demag_group(oracle, precision):
 
 def test_explicit_solver_rows(oracle):
     """solver='explicit': each field row relaxes from the previous row's final state."""
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [8, 4, 1], [40e-9, 20e-9, 3e-9]
     m0 = _random_m(32, 4)
@@ -130,7 +145,7 @@ def test_convergence_early_exit_matches_oracle(oracle):
     with conv_tol; below it the state is stored once more and the run ends.
     A damping-only relaxation exits at the same output sample as the oracle,
     with the same trajectory up to it and zero rows after it."""
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [16, 4, 1], [250e-9, 62.5e-9, 3e-9]
     n = 64
@@ -168,7 +183,7 @@ def test_time_dependent_alpha_matches_oracle(oracle):
     """alpha = 0 selects the damping table alphat (t, alpha(t)) interpolated at
     every right-hand side (LandauLifshitzEquationSolver.f90:281-294, Q12): a
     damping ramp from 0 to 8.84e3 over 0.2 ns, GPU against the oracle."""
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [16, 4, 1], [250e-9, 62.5e-9, 3e-9]
     # a smooth start (from random unit vectors the exchange-driven dynamics

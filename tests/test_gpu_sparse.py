@@ -4,10 +4,14 @@
   same FP32 tensor (integer, exact); the fraction threshold equals the
   oracle's FP32 bisection exactly (same arithmetic, integer counts).
 * Product: 1e-6 relative (norm-wise) to the FP64 product over the kept
-  entries, for 1, 3, 5 and 64 states (both kernel shapes).
+  entries, for 1, 2, 3, 4, 5 and 64 states (every few-state kernel width
+  and the many-state kernel).
 * Reference plug-in ABI (icl_*_sparse, 1-based host CSR): same bar.
-* Solver with demag_approx 'threshold' / 'threshold_fraction': trajectories
-  within 1e-4 of the oracle run with the same threshold.
+* Block-sparse split-BF16 form (precision bf16x2): same kept entries, tile
+  count exact, product within 1e-5 of the FP64 product over the kept entries.
+* Solver with demag_approx 'threshold' / 'threshold_fraction' (FP32 CSR, and
+  split-BF16 tiles): trajectories within 1e-4 of the oracle run with the same
+  threshold.
 """
 
 import ctypes as C
@@ -67,7 +71,7 @@ def test_sparse_build_and_product(oracle, approx, value):
         expect = [int(np.count_nonzero(np.abs(K[c]) > t)) for c in range(6)]
         assert list(nnz) == expect
         print(f"approx={approx} value={value}: threshold {thr.value:.3e}, nnz {list(nnz)} of {n * n} each")
-        for S in (1, 3, 5, 64):
+        for S in (1, 2, 3, 4, 5, 64):
             rng = np.random.default_rng(S)
             m = rng.normal(size=(S, 3, n)).astype(np.float32)
             ref = -_masked_product(K, t, m.astype(np.float64))
@@ -113,16 +117,82 @@ def test_icl_sparse_plugin_abi(oracle):
     assert rel < 1e-6, rel
 
 
-@pytest.mark.parametrize("mode,value", [("threshold", 1e-3), ("threshold_fraction", 0.8)])
-def test_solver_threshold_matches_oracle(oracle, mode, value):
-    from magtense_amd.micromag import MicromagProblem
+@pytest.mark.parametrize("res,grid_nx,approx,value", [([64, 16, 2], 64, 4, 0.9), ([64, 16, 2], 0, 2, 2e-3),
+                                                     ([20, 7, 1], 0, 4, 0.5), ([32, 8, 1], 32, 2, 10.0)])
+def test_block_sparse_bf16x2_product(oracle, res, grid_nx, approx, value):
+    """The split-BF16 block-sparse form of the threshold tensor: the same kept
+    entries (per-component counts exact), fragments only for the tiles that
+    hold one, and the batched MFMA product within 1e-5 (norm-wise) of the FP64
+    product over the kept entries and of the FP32 CSR product, for 1 to 70
+    states; rows of a uniform grid grouped across grid rows (grid_nx) or
+    consecutively (grid_nx = 0, a ragged 140-cell grid); a threshold above
+    every entry keeps nothing and gives zero fields."""
+    import torch
+    from magtense_amd import _lib
+
+    Lb = _lib.lib()
+    L = [res[0] * 2e-9, res[1] * 2e-9, res[2] * 1.5e-9]
+    n, K32 = _tensor(res, L)
+    K = K32.cpu().numpy().reshape(6, n, n)
+    hb, hc = C.c_void_p(), C.c_void_p()
+    _lib.check(Lb.mt_demag_sparse_create_bf16x2(K32.data_ptr(), n, approx, value, grid_nx, C.byref(hb)))
+    _lib.check(Lb.mt_demag_sparse_create(K32.data_ptr(), n, approx, value, C.byref(hc)))
+    try:
+        nnz_b, nnz_c = (C.c_longlong * 6)(), (C.c_longlong * 6)()
+        thr_b, thr_c = C.c_float(), C.c_float()
+        _lib.check(Lb.mt_demag_sparse_info(hb, nnz_b, C.byref(thr_b)))
+        _lib.check(Lb.mt_demag_sparse_info(hc, nnz_c, C.byref(thr_c)))
+        assert thr_b.value == thr_c.value and list(nnz_b) == list(nnz_c)
+        t = np.float32(thr_b.value)
+        assert list(nnz_b) == [int(np.count_nonzero(np.abs(K[c]) > t)) for c in range(6)]
+        ent, tiles, nbytes = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        ks = C.c_int()
+        _lib.check(Lb.mt_demag_sparse_blocks(hb, C.byref(ent), C.byref(tiles), C.byref(nbytes), C.byref(ks)))
+        np32 = (n + 31) // 32 * 32
+        kept = np.zeros((np32 // 32, np32 // 16), dtype=bool)
+        for c in range(6):
+            mk = np.zeros((np32, np32), dtype=bool)
+            mk[:n, :n] = np.abs(K[c]) > t
+            kept |= mk.reshape(np32 // 32, 32, np32 // 16, 16).any(axis=(1, 3))
+        assert tiles.value == int(kept.sum())
+        assert nbytes.value == ent.value * 4 * 12 * 1024
+        print(f"res={res} grid_nx={grid_nx} approx={approx} value={value}: kept {sum(nnz_b)} entries "
+              f"({sum(nnz_b) / (6 * n * n):.3f} of dense), tiles {tiles.value} of {kept.size}, list entries "
+              f"{ent.value}, k-split {ks.value}")
+        for S in (1, 5, 64, 70):
+            rng = np.random.default_rng(S)
+            m = rng.normal(size=(S, 3, n)).astype(np.float32)
+            ref = -_masked_product(K, t, m.astype(np.float64))
+            dm = torch.as_tensor(m).cuda()
+            ob = torch.full((S * 3 * n,), float("nan"), dtype=torch.float32, device="cuda")
+            oc = torch.empty(S * 3 * n, dtype=torch.float32, device="cuda")
+            _lib.check(Lb.mt_demag_apply_sparse(hb, dm.data_ptr(), ob.data_ptr(), S, -1.0))
+            _lib.check(Lb.mt_demag_apply_sparse(hc, dm.data_ptr(), oc.data_ptr(), S, -1.0))
+            gb = ob.cpu().numpy().reshape(S, 3, n)
+            gc = oc.cpu().numpy().reshape(S, 3, n)
+            if sum(nnz_b) == 0:
+                assert np.all(gb == 0) and np.all(gc == 0)
+                continue
+            nr = np.linalg.norm(ref)
+            assert np.linalg.norm(gb - ref) / nr < 1e-5, (S, np.linalg.norm(gb - ref) / nr)
+            assert np.linalg.norm(gb - gc) / nr < 1e-5
+    finally:
+        Lb.mt_demag_sparse_destroy(hb)
+        Lb.mt_demag_sparse_destroy(hc)
+
+
+@pytest.mark.parametrize("mode,value,precision", [("threshold", 1e-3, "fp32"), ("threshold_fraction", 0.8, "fp32"),
+                                                  ("threshold_fraction", 0.8, "bf16x2")])
+def test_solver_threshold_matches_oracle(oracle, mode, value, precision):
+    from magtense.micromag import MicromagProblem
 
     res, L = [16, 4, 1], [250e-9, 62.5e-9, 3e-9]
     rng = np.random.default_rng(21)
     m0 = rng.normal(size=(64, 3))
     m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
     hfield = np.array([-24.6, 4.3, 0.0]) / 1000 / MU0
-    p = MicromagProblem(res, L, m0=m0, alpha=4.42e3, gamma=2.21e5, tol=1e-6, demag_approx=mode, dem_thres=value)
+    p = MicromagProblem(res, L, m0=m0, alpha=4.42e3, gamma=2.21e5, tol=1e-6, demag_approx=mode, dem_thres=value,
+                        precision=precision)
     t_out, M, *_ = p.run_simulation(1e-10, 11, lambda t: np.ones((len(t), 1)) * hfield, 20)
     t = np.linspace(0, 1e-10, 11)
     hext = np.zeros((20, 4))
@@ -133,6 +203,6 @@ def test_solver_threshold_matches_oracle(oracle, mode, value):
     _, y, _, st = q.run()
     yr = np.transpose(y[0], (0, 2, 1))
     err = np.abs(M[:, :, 0, :] - yr).max()
-    print(mode, value, "threshold info", p.last_demag_info, "max traj err", err)
+    print(mode, value, precision, "threshold info", p.last_demag_info, "max traj err", err)
     assert err < 1e-4
     assert sum(p.last_demag_info["nnz"]) < 6 * 64 * 64

@@ -27,7 +27,7 @@ from test_oracle_tetra import box_mesh  # noqa: E402
 
 
 def test_tetra_tiles_match_oracle(oracle):
-    from magtense_amd import magstatics as ms
+    from magtense import magstatics as ms
 
     rng = np.random.default_rng(21)
     nt = 4
@@ -80,8 +80,8 @@ def _face_laplacian(elems, w):
 
 
 def test_tetra_grid_matches_oracle(oracle, tmp_path):
-    from magtense_amd import micromag
-    from magtense_amd.micromag import MicromagProblem
+    from magtense import micromag
+    from magtense.micromag import MicromagProblem
 
     nodes, elems, cent, box, a = box_mesh()
     n = len(elems)

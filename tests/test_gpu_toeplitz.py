@@ -1,8 +1,10 @@
 """The translation-invariant split-BF16 table (uniform grids, nx % 32 == 0)
 against the dense fragment-tiled tensor and the FP64 product.
 
-Grids with nx % 128 == 0 take the entry-reuse kernel (k-split 1 or 2 here),
-the others the four-tile kernel.  The table holds the same FP32 values split
+Grids with nx % 128 == 0 take the entry-reuse kernel (k-split 1 or 2 here,
+and 16 on 128 x 16 x 1, whose k-split does not divide the eight-slot ring
+and takes the kernel's second work mapping), the others the four-tile
+kernel.  The table holds the same FP32 values split
 the same way; the products differ
 only by the order in which k-chunks are summed (the table kernel visits them by
 cell offset) and by ulp-level FP64 differences in the offsets (bar 1e-6
@@ -30,7 +32,8 @@ def _fp64_product(K, m):
     return out
 
 
-@pytest.mark.parametrize("res", [[32, 8, 1], [64, 4, 3], [32, 2, 2], [96, 3, 1], [128, 4, 2], [256, 3, 3], [128, 2, 3]])
+@pytest.mark.parametrize("res", [[32, 8, 1], [64, 4, 3], [32, 2, 2], [96, 3, 1], [128, 4, 2], [256, 3, 3], [128, 2, 3],
+                                 [128, 16, 1]])
 def test_table_matches_dense_split(oracle, res):
     import torch
     from magtense_amd import _lib
@@ -71,7 +74,7 @@ def test_table_matches_dense_split(oracle, res):
 
 
 def test_solver_table_equals_dense_run():
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [32, 4, 1], [250e-9, 31.25e-9, 3e-9]
     rng = np.random.default_rng(8)

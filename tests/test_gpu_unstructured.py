@@ -80,7 +80,7 @@ def test_uniform_film_as_unstructured_prisms(oracle):
     exchange-driven dynamics amplify last-bit differences through the step
     sequence)."""
     import scipy.sparse as sp
-    from magtense_amd.micromag import MicromagProblem
+    from magtense.micromag import MicromagProblem
 
     res, L = [32, 8, 1], [500e-9, 125e-9, 3e-9]
     n = 256
@@ -122,8 +122,8 @@ def test_uniform_film_as_unstructured_prisms(oracle):
 
 @pytest.mark.parametrize("nave", [(1, 1, 1), (2, 2, 1)])
 def test_graded_film_matches_oracle(oracle, tmp_path, nave):
-    from magtense_amd import micromag
-    from magtense_amd.micromag import MicromagProblem
+    from magtense import micromag
+    from magtense.micromag import MicromagProblem
 
     pts, abc, A = graded_film()
     n = pts.shape[0]

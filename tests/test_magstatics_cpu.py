@@ -1,4 +1,4 @@
-"""Host-side behaviour of magtense_amd.magstatics (no GPU): the Tiles setters,
+"""Host-side behaviour of magtense.magstatics (no GPU): the Tiles setters,
 easy-axis conventions, grid_config and refine_prism, as written in
 python/src/magtense/magstatics.py:9-741."""
 
@@ -7,7 +7,7 @@ import math
 import numpy as np
 import pytest
 
-from magtense_amd import magstatics as ms
+from magtense import magstatics as ms
 
 MU0 = 4 * math.pi * 1e-7
 
