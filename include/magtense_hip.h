@@ -165,6 +165,18 @@ int mt_demag_apply_bf16x2_rows(const void* split_rows, int n, int row0, int nrow
  * FP32, LandauLifshitzEquationSolver.f90:706-754). */
 int mt_demag_near_correction(int nx, int ny, int nz, double Lx, double Ly, double Lz, int radius, const float* m,
                              float* h, int nstates, float pref);
+/* Allocation-free form of mt_demag_apply_bf16x2 (same kernels, same result).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace (packed states and k-split partial sums,
+ * mt_demag_apply_bf16x2_noalloc_bytes(n, nstates) bytes) lives in a 64 MiB
+ * file-scope device array that every device holds for the life of the
+ * process (mt_noalloc_scratch_bytes()); larger problems (about n > 43000 cells)
+ * are refused with MT_ERR_ARG before any launch.  Calls that overlap on one
+ * device share the array and race. */
+long long mt_noalloc_scratch_bytes(void);
+long long mt_demag_apply_bf16x2_noalloc_bytes(int n, int nstates);
+int mt_demag_apply_bf16x2_noalloc(const void* split, int n, const float* m, float* h, int nstates, float pref);
 /* Same product from the translation-invariant table of mt_toeplitz_tensor. */
 int mt_demag_apply_bf16x2_toeplitz(const void* table, int nx, int ny, int nz, const float* m, float* h, int nstates,
                                    float pref);

@@ -88,6 +88,7 @@ EXPORTS = [
     "mt_rk4_backward", "mt_rk4_destroy", "mt_tiles_demag_tensor", "mt_tiles_h_field", "mt_demag_sparse_create",
     "mt_demag_sparse_info", "mt_demag_apply_sparse", "mt_demag_sparse_destroy", "mt_solver_demag_info",
     "mt_demag_sparse_create_bf16x2", "mt_demag_sparse_blocks", "mt_demag_near_correction",
+    "mt_noalloc_scratch_bytes", "mt_demag_apply_bf16x2_noalloc_bytes", "mt_demag_apply_bf16x2_noalloc",
     "mt_toeplitz_tensor_bytes", "mt_toeplitz_tensor", "mt_demag_apply_bf16x2_toeplitz",
     "mt_split_tensor_row_offset", "mt_demag_apply_bf16x2_rows",
     "icl_initDemagMatrices", "icl_MVMult_GetH", "icl_destroy", "icl_initDemagMatrices_sparse", "icl_MVMult_GetH_sparse",
@@ -155,6 +156,10 @@ def load():
     lib.mt_toeplitz_tensor_bytes.argtypes = [C.c_int] * 3
     lib.mt_toeplitz_tensor_bytes.restype = C.c_longlong
     lib.mt_toeplitz_tensor.argtypes = [C.c_int] * 3 + [C.c_double] * 3 + [vp]
+    lib.mt_noalloc_scratch_bytes.restype = C.c_longlong
+    lib.mt_demag_apply_bf16x2_noalloc_bytes.argtypes = [C.c_int, C.c_int]
+    lib.mt_demag_apply_bf16x2_noalloc_bytes.restype = C.c_longlong
+    lib.mt_demag_apply_bf16x2_noalloc.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_demag_near_correction.argtypes = [C.c_int] * 3 + [C.c_double] * 3 + [C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_demag_apply_bf16x2_toeplitz.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_split_tensor_row_offset.argtypes = [C.c_int, C.c_int]
