@@ -360,8 +360,8 @@ int mt_demag_near_correction(int nx, int ny, int nz, double Lx, double Ly, doubl
   MT_CHECK(nx >= 1 && ny >= 1 && nz >= 1 && m && h && nstates > 0, "invalid near-band correction arguments");
   MT_CHECK(radius >= 1 && radius <= 4, "the near-band radius must be 1..4 cells");
   const int Rz = near_band_rz(radius, nz);
-  DevArr<float> tab;
-  MT_HIP(tab.alloc((size_t)near_table_floats(radius, nz)));
+  DevArr<uint32_t> tab;
+  MT_HIP(tab.alloc((size_t)near_table_words(radius, nz)));
   rc = launch_near_table(nx, ny, nz, Lx, Ly, Lz, radius, Rz, -1.0f, tab.p, nullptr);
   if (rc) return rc;
   const int n = nx * ny * nz;

@@ -347,6 +347,9 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
     }
 }
 
+constexpr int kFlushChunks = 128;  // chunks summed in the MFMA accumulators before they are flushed
+constexpr int kTableChainChunks = 512;  // longest partial sum of the entry-reuse table kernel
+
 // Register-ring kernel for the dense tensor.  Each wave owns one 32-row tile
 // and all 64 states (six accumulators); its A fragments (12 per k-chunk,
 // 12 KiB) stream from HBM straight into a D-deep register ring with
@@ -429,13 +432,16 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
   // native vector element types keep both rings in registers
   u32x4 ar[D][12];
   u32x4 br[D][3];
-  floatx16 acc[2][3];
+  // acc sums at most kFlushChunks chunks before it is added into sum and
+  // restarted: two-level FP32 summation, so a long row's partial sum does not
+  // swallow the rounding of thousands of additions
+  floatx16 acc[2][3], sum[2][3];
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[h][a][r] = 0.0f;
+      for (int r = 0; r < 16; ++r) acc[h][a][r] = sum[h][a][r] = 0.0f;
 
 #define MT_REG_ISSUE(v, slot)                                                                            \
   do {                                                                                                    \
@@ -501,6 +507,16 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
         MT_REG_ISSUE(j0 + s + D - 1, (s + D - 1) % D);
         MT_REG_CONSUME(j0 + s, s);
       }
+      if ((j0 + D) % kFlushChunks == 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            sum[h][a] += acc[h][a];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[h][a][r] = 0.0f;
+          }
+      }
     }
     // fewer than D chunks left, all already issued into slots 0 .. rem - 1
     const int rem = nchunks - j0;
@@ -521,7 +537,7 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-        if (row < n) o[col * col_stride + (int64_t)a * n + row] = acc[h][a][r];
+        if (row < n) o[col * col_stride + (int64_t)a * n + row] = sum[h][a][r] + acc[h][a][r];
       }
   }
 }
@@ -989,40 +1005,49 @@ int gemm_ksplit_table(int nx, int ny, int nz) {
   const int n = nx * ny * nz;
   if (nx % 128 != 0 || gemm_variant() != 0) return gemm_ksplit_for(n, 2);
   // one resident workgroup per CU: the smallest k-split giving every CU a
-  // workgroup (1 at 256 x 64 x 2), within the divisors of the source rows
-  const int rows = ny * nz, groups = n / kTileM / 4;
+  // workgroup, within the divisors of the source rows; and no partial sum
+  // over more than kTableChainChunks chunks (the kernel's accumulators are
+  // single-level FP32: long chains of additions lose accuracy on coherent
+  // states; 4 at 256 x 64 x 2)
+  const int rows = ny * nz, groups = n / kTileM / 4, nJ = n / kTileK;
   int ks = 1;
-  while (groups * ks < 256 && rows % (ks * 2) == 0) ks *= 2;
+  while ((groups * ks < 256 || nJ / ks > kTableChainChunks) && rows % (ks * 2) == 0) ks *= 2;
   if (forced_ksplit() > 0 && rows % forced_ksplit() == 0) ks = forced_ksplit();
   return ks;
 }
 
 namespace {
-__device__ inline float bf16_round(float v) {
-  return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, __float2bfloat16(v)) << 16);
+// Near-band correction of the split-BF16 product on a uniform grid (the
+// terms and the table words are described with near_pack_offset).  A
+// workgroup owns 64 x 4 receivers of one layer and one state, one receiver per
+// thread.  Per source layer it stages the (64 + 2R) x (4 + 2R) neighbourhood
+// in LDS as BF16 pairs in three planes (8, 8 and 4 bytes per cell, so that
+// consecutive lanes read consecutive words): (bf(dm_x), mlo_x), (bf(dm_y),
+// mlo_y) | (bf(dm_z), mlo_z), (mhi_x, mhi_y) | (mhi_z, 0), zero outside the
+// grid.  Each offset then costs every receiver three LDS reads, one 16-word
+// scalar load of the table (the same for the whole workgroup) and 15 BF16
+// pair dot products with FP32 sums.  F64: the state comes from the LL
+// descriptors (FP64 stage arguments, rounded to FP32 as the product's packing
+// rounds them), else from FP32 states m32 + s * m_stride.
+constexpr int kNearTX = 64, kNearTY = 4;
+typedef __bf16 near_bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline float near_dot(uint32_t t, uint32_t v, float acc) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(near_bf16x2, t), __builtin_bit_cast(near_bf16x2, v), acc,
+                                         false);
 }
 
-// Near-band correction of the split-BF16 product on a uniform grid.  The
-// batched GEMM forms khi mhi + khi mlo + klo mhi for every pair; with K32 =
-// khi + klo + dK and m32 = mhi + mlo + dm (both exact) the FP32 product adds
-//   khi dm + klo (mlo + dm) + dK m32,
-// which this kernel sums over the sources within R cells in x and y and Rz
-// layers in z of each receiver, where the tensor entries are largest.  A
-// workgroup owns 64 x 4 receivers of one layer and one state; per source
-// layer it stages (m32, dm, mlo + dm) of the 3 components for the (64 + 2R) x
-// (4 + 2R) neighbourhood in LDS (zero outside the grid).  The table entries
-// of an offset are the same for the whole workgroup.  F64: the state comes
-// from the LL descriptors (FP64 stage arguments, rounded to FP32 as the
-// product's packing rounds them), else from FP32 states m32 + s * m_stride.
-constexpr int kNearTX = 64, kNearTY = 4;
-
 template <bool F64>
-__global__ void __launch_bounds__(256) near_corr_kernel(const float* __restrict__ table, int R, int Rz, int nx,
+__global__ void __launch_bounds__(256) near_corr_kernel(const uint32_t* __restrict__ table, int R, int Rz, int nx,
                                                         int ny, int nz, const LlgDesc* __restrict__ descs,
                                                         const float* __restrict__ m32, int64_t m_stride,
                                                         float* __restrict__ out, int64_t col_stride, float pref,
                                                         int accumulate) {
-  extern __shared__ float halo[];  // [hy][hx][9]: (m32, dm, mlo + dm) for x, y, z
+  extern __shared__ uint2 near_lds[];
+  const int hx = kNearTX + 2 * R, hy = kNearTY + 2 * R, hc = hx * hy, w = 2 * R + 1;
+  uint2* p0 = near_lds;                                // (W_x, W_y)
+  uint2* p1 = near_lds + hc;                           // (W_z, M_xy)
+  uint32_t* p2 = reinterpret_cast<uint32_t*>(near_lds + 2 * hc);  // M_z
   const int n = nx * ny * nz;
   const int tilesx = (nx + kNearTX - 1) / kNearTX, tilesy = (ny + kNearTY - 1) / kNearTY;
   int b = blockIdx.x;
@@ -1033,91 +1058,89 @@ __global__ void __launch_bounds__(256) near_corr_kernel(const float* __restrict_
   const int s = blockIdx.y;
   const int lx = threadIdx.x % kNearTX, ly = threadIdx.x / kNearTX;
   const int ix = tx0 + lx, iy = ty0 + ly;
-  const int hx = kNearTX + 2 * R, hy = kNearTY + 2 * R, w = 2 * R + 1;
   const double* m64 = F64 ? descs[s].m : nullptr;
   const float* mf = F64 ? nullptr : m32 + (int64_t)s * m_stride;
-  const bool own = ix < nx && iy < ny;
-  // (a, b) -> component xx xy xz yy yz zz
-  const int cab[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
   float acc[3] = {0.0f, 0.0f, 0.0f};
   for (int oz = -Rz; oz <= Rz; ++oz) {  // receiver - source layer
     const int jz = iz - oz;
     if (jz < 0 || jz >= nz) continue;  // uniform over the workgroup
     __syncthreads();                   // the previous layer's reads are done
-    for (int h = threadIdx.x; h < hx * hy; h += blockDim.x) {
+    for (int h = threadIdx.x; h < hc; h += blockDim.x) {
       const int gx = tx0 - R + h % hx, gy = ty0 - R + h / hx;
-      float v[9];
-#pragma unroll
-      for (int q = 0; q < 9; ++q) v[q] = 0.0f;
+      uint32_t wv[3] = {0u, 0u, 0u}, mh[3] = {0u, 0u, 0u};
       if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
         const int64_t j = gx + (int64_t)gy * nx + (int64_t)jz * nx * ny;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const float f = F64 ? (float)m64[(int64_t)c * n + j] : mf[(int64_t)c * n + j];
-          const float hi = bf16_round(f);
-          const float lo = bf16_round(f - hi);
-          const float dm = (f - hi) - lo;
-          v[3 * c] = f;
-          v[3 * c + 1] = dm;
-          v[3 * c + 2] = lo + dm;
+          const uint32_t hi = (uint32_t)__builtin_bit_cast(uint16_t, __float2bfloat16(f));
+          const float r = f - __uint_as_float(hi << 16);
+          const uint32_t lo = (uint32_t)__builtin_bit_cast(uint16_t, __float2bfloat16(r));
+          const float dm = r - __uint_as_float(lo << 16);
+          wv[c] = (uint32_t)__builtin_bit_cast(uint16_t, __float2bfloat16(dm)) | lo << 16;
+          mh[c] = hi;
         }
       }
-#pragma unroll
-      for (int q = 0; q < 9; ++q) halo[h * 9 + q] = v[q];
+      p0[h] = make_uint2(wv[0], wv[1]);
+      p1[h] = make_uint2(wv[2], mh[0] | mh[1] << 16);
+      p2[h] = mh[2];
     }
     __syncthreads();
-    if (own) {
-      for (int oy = -R; oy <= R; ++oy)
-        for (int ox = -R; ox <= R; ++ox) {
-          // source (ix - ox, iy - oy, jz)
-          const float* hv = halo + ((ly + R - oy) * hx + (lx + R - ox)) * 9;
-          const float* t = table + ((int64_t)(((oz + Rz) * w + (oy + R)) * w + (ox + R)) * 6) * 3;
-          float mv[9];
+    const uint32_t* tz = table + (int64_t)(oz + Rz) * w * w * kNearWords;
+    for (int oy = -R; oy <= R; ++oy) {
+      // source (ix - ox, iy - oy): halo cell (lx - ox + R, ly - oy + R)
+      const int hrow = (ly - oy + R) * hx + lx + R;
+      const uint32_t* ty = tz + (oy + R) * w * kNearWords;
+      for (int ox = -R; ox <= R; ++ox) {
+        const uint2 v0 = p0[hrow - ox], v1 = p1[hrow - ox];
+        const uint32_t v2 = p2[hrow - ox];
+        const uint32_t* t = ty + (ox + R) * kNearWords;
 #pragma unroll
-          for (int q = 0; q < 9; ++q) mv[q] = hv[q];
-#pragma unroll
-          for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int bb = 0; bb < 3; ++bb) {
-              const float* tc = t + 3 * cab[a][bb];
-              acc[a] = fmaf(tc[0], mv[3 * bb + 1], acc[a]);
-              acc[a] = fmaf(tc[1], mv[3 * bb + 2], acc[a]);
-              acc[a] = fmaf(tc[2], mv[3 * bb], acc[a]);
-            }
+        for (int a = 0; a < 3; ++a) {
+          float x = acc[a];
+          x = near_dot(t[5 * a], v0.x, x);
+          x = near_dot(t[5 * a + 1], v0.y, x);
+          x = near_dot(t[5 * a + 2], v1.x, x);
+          x = near_dot(t[5 * a + 3], v1.y, x);
+          acc[a] = near_dot(t[5 * a + 4], v2, x);
         }
+      }
     }
   }
-  if (!own) return;
+  if (ix >= nx || iy >= ny) return;
   const int64_t i = ix + (int64_t)iy * nx + (int64_t)iz * nx * ny;
   float* o = out + (int64_t)s * col_stride + i;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) o[(int64_t)a * n] = accumulate ? o[(int64_t)a * n] + pref * acc[a] : pref * acc[a];
+  for (int a = 0; a < 3; ++a)
+    o[(int64_t)a * n] = accumulate ? o[(int64_t)a * n] + pref * acc[a] : pref * acc[a];
 }
 }  // namespace
 
 int near_band_rz(int R, int nz) { return R < nz - 1 ? R : nz - 1; }
 
-int64_t near_table_floats(int R, int nz) {
+int64_t near_table_words(int R, int nz) {
   const int rz = near_band_rz(R, nz);
-  return (int64_t)(2 * R + 1) * (2 * R + 1) * (2 * rz + 1) * 18;
+  return (int64_t)(2 * R + 1) * (2 * R + 1) * (2 * rz + 1) * kNearWords;
 }
 
-int launch_near_correction(const float* table, int R, int nx, int ny, int nz, const LlgDesc* descs, const float* m32,
-                           int64_t m_stride, int nstates, float* out, int64_t col_stride, float pref, bool accumulate,
-                           hipStream_t stream) {
+int launch_near_correction(const uint32_t* table, int R, int nx, int ny, int nz, const LlgDesc* descs,
+                           const float* m32, int64_t m_stride, int nstates, float* out, int64_t col_stride, float pref,
+                           bool accumulate, hipStream_t stream) {
   if (R < 1 || R > 4 || nx < 1 || ny < 1 || nz < 1 || nstates < 1 || nstates > 65535) {
     set_error("near-band correction: radius 1..4, a non-empty grid and 1..65535 states");
     return kErrArg;
   }
   const int Rz = near_band_rz(R, nz);
   const int tiles = ((nx + kNearTX - 1) / kNearTX) * ((ny + kNearTY - 1) / kNearTY) * nz;
-  const size_t lds = (size_t)(kNearTX + 2 * R) * (kNearTY + 2 * R) * 9 * sizeof(float);
+  const size_t lds = (size_t)(kNearTX + 2 * R) * (kNearTY + 2 * R) * 20;  // 8 + 8 + 4 bytes per cell
+  const dim3 grid((unsigned)tiles, (unsigned)nstates);
+  const int acc = accumulate ? 1 : 0;
   if (descs)
-    hipLaunchKernelGGL(near_corr_kernel<true>, dim3((unsigned)tiles, (unsigned)nstates), dim3(256), lds, stream, table,
-                       R, Rz, nx, ny, nz, descs, m32, m_stride, out, col_stride, pref, accumulate ? 1 : 0);
+    hipLaunchKernelGGL(near_corr_kernel<true>, grid, dim3(256), lds, stream, table, R, Rz, nx, ny, nz, descs, m32,
+                       m_stride, out, col_stride, pref, acc);
   else
-    hipLaunchKernelGGL(near_corr_kernel<false>, dim3((unsigned)tiles, (unsigned)nstates), dim3(256), lds, stream,
-                       table, R, Rz, nx, ny, nz, descs, m32, m_stride, out, col_stride, pref, accumulate ? 1 : 0);
+    hipLaunchKernelGGL(near_corr_kernel<false>, grid, dim3(256), lds, stream, table, R, Rz, nx, ny, nz, descs, m32,
+                       m_stride, out, col_stride, pref, acc);
   MT_HIP(hipGetLastError());
   return kOk;
 }

@@ -388,17 +388,16 @@ __global__ void finalize_kernel(const VecDesc* __restrict__ descs, int nstates, 
   if (t >= nstates * 4) return;
   int s = t / 4, k = t % 4;
   int kind = descs[s].rkind;
-  if (kind == kRedNone || kind == kRedWeightsAvg) {
-    results[t] = 0.0;
-    return;
-  }
+  // a descriptor without a reduction leaves its slot alone: a result written
+  // by an earlier launch may still be waiting to be read
+  if (kind == kRedNone || kind == kRedWeightsAvg) return;
   bool is_sum = kind == kRedWdot;
   double x = partials[((int64_t)s * blocks_per_state) * 4 + k];
   for (int b = 1; b < blocks_per_state; ++b) {
     double o = partials[((int64_t)s * blocks_per_state + b) * 4 + k];
     x = is_sum ? x + o : fmax(x, o);
   }
-  results[t] = x;
+  results[descs[s].rslot * 4 + k] = x;
 }
 
 }  // namespace

@@ -169,17 +169,59 @@ int launch_llg(const LlgGrid& g, const LlgDesc* d_descs, int nstates, hipStream_
 // Near-band correction of the split-BF16 product on a uniform nx x ny x nz
 // grid: the FP32 product's difference from the three split products, summed
 // over the sources within R cells in x and y and min(R, nz - 1) layers in z.
-// Table (prism_tensor.hip): per offset and component khi, klo, dK.
+// With K32 = khi + klo + dK and m32 = mhi + mlo + dm (BF16 hi and lo, FP32
+// rest) that difference is khi dm + klo (mlo + dm) + dK m32.  Every term is
+// about 2^-17 of K32 m32, so BF16 operands carry it to about 2^-26 of the
+// product: the kernel forms khi bf(dm) + klo mlo + bf(dK) mhi with BF16 pair
+// dot products (FP32 sums).
+//
+// Table: per offset kNearWords words of BF16 pairs (first element in the low
+// half), for output component a = x, y, z at 5a: (khi, klo) of K_a,x, K_a,y,
+// K_a,z, then (dK_a,x, dK_a,y), (dK_a,z, 0); word 15 is zero.  Offsets o =
+// (ox + R) + (oy + R) w + (oz + Rz) w^2, w = 2R + 1, o = receiver - source.
+constexpr int kNearWords = 16;
 int near_band_rz(int R, int nz);
-int64_t near_table_floats(int R, int nz);
+int64_t near_table_words(int R, int nz);
+
+// round to the nearest BF16, ties to even (finite inputs)
+__host__ __device__ inline uint32_t bf16_rne_bits(float v) {
+  uint32_t u;
+  __builtin_memcpy(&u, &v, 4);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__host__ __device__ inline float bf16_bits_value(uint32_t b) {
+  const uint32_t u = b << 16;
+  float v;
+  __builtin_memcpy(&v, &u, 4);
+  return v;
+}
+// One offset's table words from its six FP32 tensor entries (xx xy xz yy yz zz).
+__host__ __device__ inline void near_pack_offset(const float k32[6], uint32_t* w) {
+  constexpr int cab[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  uint32_t hl[6], dk[6];
+  for (int c = 0; c < 6; ++c) {
+    const uint32_t hi = bf16_rne_bits(k32[c]);
+    const float khi = bf16_bits_value(hi);
+    const uint32_t lo = bf16_rne_bits(k32[c] - khi);
+    const float klo = bf16_bits_value(lo);
+    hl[c] = hi | lo << 16;
+    dk[c] = bf16_rne_bits((k32[c] - khi) - klo);
+  }
+  for (int a = 0; a < 3; ++a) {
+    for (int b = 0; b < 3; ++b) w[5 * a + b] = hl[cab[a][b]];
+    w[5 * a + 3] = dk[cab[a][0]] | dk[cab[a][1]] << 16;
+    w[5 * a + 4] = dk[cab[a][2]];
+  }
+  w[15] = 0;
+}
 // thr: entries with |K32| <= thr are zero (the threshold-sparse modes); -1 keeps all
-int launch_near_table(int nx, int ny, int nz, double Lx, double Ly, double Lz, int R, int Rz, float thr, float* table,
-                      hipStream_t stream);
+int launch_near_table(int nx, int ny, int nz, double Lx, double Ly, double Lz, int R, int Rz, float thr,
+                      uint32_t* table, hipStream_t stream);
 // out[s*col_stride + a*n + i] (+)= pref * correction of state s; the state is
 // read from descs[s].m (FP64) when descs is set, else from m32 + s*m_stride.
-int launch_near_correction(const float* table, int R, int nx, int ny, int nz, const LlgDesc* descs, const float* m32,
-                           int64_t m_stride, int nstates, float* out, int64_t col_stride, float pref, bool accumulate,
-                           hipStream_t stream);
+int launch_near_correction(const uint32_t* table, int R, int nx, int ny, int nz, const LlgDesc* descs,
+                           const float* m32, int64_t m_stride, int nstates, float* out, int64_t col_stride, float pref,
+                           bool accumulate, hipStream_t stream);
 
 // ---- (d) vector operations -------------------------------------------------------
 constexpr int kMaxTerms = 12;
@@ -218,6 +260,7 @@ struct VecDesc {
   double rcoef[8];
   double rscale;
   double thresh;
+  int rslot;                 // the reduction's four results go to results[4*rslot .. 4*rslot+3]
 };
 
 enum PackMode { kPackNone = 0, kPackF32 = 1, kPackBf16 = 2 };

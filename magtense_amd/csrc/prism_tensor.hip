@@ -22,6 +22,7 @@
 #include <hip/hip_bf16.h>
 
 #include "mt_common.h"
+#include "mt_kernels.h"
 
 #pragma clang fp contract(off)
 
@@ -702,13 +703,14 @@ int launch_toeplitz_assemble(int nx, int ny, int nz, double Lx, double Ly, doubl
 namespace {
 // Near-band table of a uniform grid (one thread per source offset): for the
 // offset (ox, oy, oz) = receiver - source cell index, |ox|, |oy| <= R, |oz| <=
-// Rz, the FP32 tensor value K32 of every component and its split as the
-// batched product uses it: khi = bf16(K32), klo = bf16(K32 - khi), dK = K32 -
-// khi - klo (exact in FP32).  Values with |K32| <= thr (a threshold-sparse
-// tensor's dropped entries) are zero.  The offset is formed from the centres of one
-// representative pair, as the translation-invariant table forms it.
+// Rz, the FP32 tensor value K32 of every component, packed by
+// near_pack_offset into the split as the batched product uses it (khi =
+// bf16(K32), klo = bf16(K32 - khi)) and the BF16 rounding of the rest dK.
+// Values with |K32| <= thr (a threshold-sparse tensor's dropped entries) are
+// zero.  The offset is formed from the centres of one representative pair, as
+// the translation-invariant table forms it.
 __global__ void near_table_kernel(int nx, int ny, int nz, double Lx, double Ly, double Lz, int R, int Rz, float thr,
-                                  float* __restrict__ out) {
+                                  uint32_t* __restrict__ out) {
   const int w = 2 * R + 1;
   const int no = w * w * (2 * Rz + 1);
   const int o = blockIdx.x * blockDim.x + threadIdx.x;
@@ -722,21 +724,17 @@ __global__ void near_table_kernel(int nx, int ny, int nz, double Lx, double Ly, 
   double k6[6];
   prism_k6(dx, dy, dz, centre(nx, Lx, dx, ix) - centre(nx, Lx, dx, jx), centre(ny, Ly, dy, iy) - centre(ny, Ly, dy, jy),
            centre(nz, Lz, dz, iz) - centre(nz, Lz, dz, jz), k6);
+  float k32[6];
   for (int c = 0; c < 6; ++c) {
-    float k32 = (float)k6[c];
-    if (!(fabsf(k32) > thr)) k32 = 0.0f;  // an entry the threshold drops stays dropped
-    const float khi = bf16_value(bf16_bits(k32));
-    const float klo = bf16_value(bf16_bits(k32 - khi));
-    float* t = out + ((int64_t)o * 6 + c) * 3;
-    t[0] = khi;
-    t[1] = klo;
-    t[2] = (k32 - khi) - klo;
+    k32[c] = (float)k6[c];
+    if (!(fabsf(k32[c]) > thr)) k32[c] = 0.0f;  // an entry the threshold drops stays dropped
   }
+  near_pack_offset(k32, out + (int64_t)o * kNearWords);
 }
 }  // namespace
 
-int launch_near_table(int nx, int ny, int nz, double Lx, double Ly, double Lz, int R, int Rz, float thr, float* table,
-                      hipStream_t stream) {
+int launch_near_table(int nx, int ny, int nz, double Lx, double Ly, double Lz, int R, int Rz, float thr,
+                      uint32_t* table, hipStream_t stream) {
   if (R < 0 || Rz < 0) {
     set_error("near-band radius must be >= 0");
     return kErrArg;

@@ -39,16 +39,6 @@
 
 namespace mt {
 
-// Round to the nearest BF16 (ties to even), as __float2bfloat16 does for
-// finite values, returned as the FP32 value it represents.
-static float bf16_round_host(float v) {
-  uint32_t u;
-  std::memcpy(&u, &v, 4);
-  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
-  float r;
-  std::memcpy(&r, &u, 4);
-  return r;
-}
 
 
 namespace {
@@ -106,6 +96,7 @@ struct Op {
   VecDesc vec{};      // kVec
   double res[4] = {0, 0, 0, 0};
   double post_res[4] = {0, 0, 0, 0};
+  bool defer_pre = false;  // kEval: the pre-reduction's result may wait for the next synchronising tick
 };
 
 struct Sim {
@@ -300,7 +291,8 @@ struct Solver {
   double* xv = nullptr;
   long long exch_nnz_host = 0;
   uint8_t* Ktab = nullptr;    // translation-invariant fragment table (bf16x2 on uniform grids)
-  float* near_tab = nullptr;  // near-band correction table (bf16x2 on uniform grids, dense tensor)
+  bool defer_early = false;   // read the early error estimate with stage 8's (cheap right-hand sides)
+  uint32_t* near_tab = nullptr;  // near-band correction table (bf16x2 on uniform grids, dense tensor)
   int near_r = 0;             // its radius (0: no correction)
   int demag_approx = 1;       // 1 dense, 2 threshold, 4 threshold fraction
   SparseDemag sparse;
@@ -488,7 +480,8 @@ struct Solver {
   }
 
   // ---- operations issued by the per-state coroutines ----------------------
-  OpAwait eval(Sim& s, double t, const VecDesc& pre, double* out, const VecDesc& post, double* fields = nullptr) {
+  OpAwait eval(Sim& s, double t, const VecDesc& pre, double* out, const VecDesc& post, double* fields = nullptr,
+               bool defer_pre = false) {
     s.op = Op();
     s.op.kind = Op::kEval;
     s.op.pre = pre;
@@ -496,6 +489,7 @@ struct Solver {
     s.op.t = t;
     s.op.out = out;
     s.op.fields = fields;
+    s.op.defer_pre = defer_pre;
     return OpAwait{&s};
   }
   OpAwait vec(Sim& s, const VecDesc& d) {
@@ -583,10 +577,18 @@ struct Solver {
           post.rscale = htry;
           post.ra = s.y; post.rb = s.scratch; post.rd = s.errest; post.rw = s.weights; post.thresh = thres;
         }
-        co_await eval(s, tstg, pre, s.stage[i - 1], post);
-        if (i == 7) {
+        // Stage 8 is formed from stages 1..7 whatever the early estimate says,
+        // so when a right-hand side is cheap next to a host round trip the
+        // estimate is read together with stage 8's: a failed estimate then
+        // rejects the step exactly as before and the extra stage-8
+        // evaluation is discarded uncounted.
+        const bool defer = i == 7 && defer_early;
+        co_await eval(s, tstg, pre, s.stage[i - 1], post, nullptr, defer);
+        if (defer) continue;
+        if (i == 7 || (i == 8 && defer_early)) {
           err = std::fabs(s.h) * s.op.res[0];  // stepb scales by |comm%h|
           if (err > s.tol) co_return 0;       // the stage-7 evaluation is not counted
+          if (i == 8) s.f_count += 1;         // stage 7
         }
         s.f_count += 1;
         if (i == 8) err = s.op.post_res[0];
@@ -1086,6 +1088,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     std::vector<VecDesc> d(vecs.size());
     for (size_t k = 0; k < vecs.size(); ++k) {
       d[k] = vecs[k]->op.vec;
+      d[k].rslot = vecs[k]->id;
       if (d[k].rkind != kRedNone && d[k].rkind != kRedWeightsAvg) vec_reduce = true;
     }
     if (timing) MT_HIP(hipEventRecord(ev[0], stream));
@@ -1106,7 +1109,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     }
   }
   // right-hand sides: groups of up to 64 columns per demag pass
-  bool pre_reduce_any = false, post_reduce_any = false;
+  bool pre_sync = false, post_reduce_any = false;
   for (size_t g0 = 0; g0 < evals.size(); g0 += kMaxCols) {
     size_t g1 = std::min(evals.size(), g0 + (size_t)kMaxCols);
     int S = (int)(g1 - g0);
@@ -1117,8 +1120,13 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       Sim& s = *evals[g0 + k];
       pre[k] = s.op.pre;
       pre[k].pack_col = k;
-      if (pre[k].rkind != kRedNone) pre_red = true;
+      pre[k].rslot = s.id;  // per-state result slots: a deferred result survives later ticks
+      if (pre[k].rkind != kRedNone) {
+        pre_red = true;
+        if (!s.op.defer_pre) pre_sync = true;
+      }
       post[k] = s.op.post;
+      post[k].rslot = s.id;
       if (post[k].rkind != kRedNone) post_red = true;
       LlgDesc& L = ld[k];
       std::memset(&L, 0, sizeof(L));
@@ -1145,8 +1153,8 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     const VecDesc* dpost = post_red ? stage_put(post) : nullptr;
     rc = stage_flush();
     if (rc) return rc;
-    rc = launch_vec(dpre, S, n, partials + (size_t)B * vec_blocks_per_state((int)N3) * 4, res_pre + g0 * 4, pk,
-                    pre_red, stream);
+    rc = launch_vec(dpre, S, n, partials + (size_t)B * vec_blocks_per_state((int)N3) * 4, res_pre, pk, pre_red,
+                    stream);
     if (rc) return rc;
     if (timing) MT_HIP(hipEventRecord(ev[3], stream));
     if (precision == MT_PREC_BF16X2) {
@@ -1175,8 +1183,8 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     if (rc) return rc;
     if (timing) MT_HIP(hipEventRecord(ev[5], stream));
     if (post_red) {
-      rc = launch_vec(dpost, S, n, partials + (size_t)2 * B * vec_blocks_per_state((int)N3) * 4, res_post + g0 * 4,
-                      nopack, true, stream);
+      rc = launch_vec(dpost, S, n, partials + (size_t)2 * B * vec_blocks_per_state((int)N3) * 4, res_post, nopack,
+                      true, stream);
       if (rc) return rc;
     }
     if (timing) {
@@ -1202,21 +1210,20 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       if (post_red)
         for (const VecDesc& v : post) vec_bytes += vec_desc_bytes(v, kPackNone);
     }
-    pre_reduce_any |= pre_red;
     post_reduce_any |= post_red;
   }
   if (!evals.empty()) eval_ticks += 1;
-  need_sync |= pre_reduce_any || post_reduce_any;
+  need_sync |= pre_sync || post_reduce_any;
   if (need_sync) {
     MT_HIP(hipMemcpyAsync(h_results, results, sizeof(double) * 3 * nres, hipMemcpyDeviceToHost, stream));
     MT_HIP(hipStreamSynchronize(stream));
     arena_off = 0;
-    for (size_t k = 0; k < vecs.size(); ++k)
-      for (int q = 0; q < 4; ++q) vecs[k]->op.res[q] = h_results[k * 4 + q];
-    for (size_t k = 0; k < evals.size(); ++k)
+    for (Sim* v : vecs)
+      for (int q = 0; q < 4; ++q) v->op.res[q] = h_results[v->id * 4 + q];
+    for (Sim* e : evals)
       for (int q = 0; q < 4; ++q) {
-        evals[k]->op.res[q] = h_results[nres + k * 4 + q];
-        evals[k]->op.post_res[q] = h_results[2 * nres + k * 4 + q];
+        e->op.res[q] = h_results[nres + e->id * 4 + q];
+        e->op.post_res[q] = h_results[2 * nres + e->id * 4 + q];
       }
   }
   for (Sim* s : vecs) s->op.kind = Op::kNone;
@@ -1563,6 +1570,13 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
     MT_HIP(hipMemsetAsync(P.fout, 0, ysz * 4 * sizeof(double), P.stream));
   }
   P.hcols = std::min(P.B, kMaxCols);
+  // one host round trip per step instead of two where a pass over the tensor
+  // is cheap next to it (under 2 GB of FP32 tensor, N < ~9100 cells): the
+  // early error estimate is then read with stage 8's; MT_DEFER_EARLY=0/1 overrides
+  {
+    const char* env = std::getenv("MT_DEFER_EARLY");
+    P.defer_early = env ? std::atoi(env) != 0 : (double)n * n * 24.0 < 2e9;
+  }
   if (P.precision == MT_PREC_BF16X2) {
     P.ksplit = q->ksplit > 0 ? q->ksplit
                              : (P.Ktab ? gemm_ksplit_table(P.nx, P.ny, P.nz)
@@ -1580,10 +1594,10 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       const int R = P.near_r, Rz = near_band_rz(R, P.nz);
       // threshold modes: the entries the threshold drops get no correction either
       const float thr = approx == MT_DEMAG_DENSE ? -1.0f : P.sparse.threshold;
-      MT_HIP(hipMalloc(&P.near_tab, (size_t)near_table_floats(R, P.nz) * sizeof(float)));
+      MT_HIP(hipMalloc(&P.near_tab, (size_t)near_table_words(R, P.nz) * sizeof(uint32_t)));
       if (q->K_in) {
         // the caller's tensor: each offset's values from the first pair of cells that has it
-        std::vector<float> tab((size_t)near_table_floats(R, P.nz), 0.0f);
+        std::vector<uint32_t> tab((size_t)near_table_words(R, P.nz), 0u);
         const int w = 2 * R + 1;
         for (int o = 0; o < w * w * (2 * Rz + 1); ++o) {
           const int ox = o % w - R, oy = (o / w) % w - R, oz = o / (w * w) - Rz;
@@ -1592,18 +1606,14 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
                            (size_t)std::max(oz, 0) * P.nx * P.ny;
           const size_t j = (size_t)std::max(-ox, 0) + (size_t)std::max(-oy, 0) * P.nx +
                            (size_t)std::max(-oz, 0) * P.nx * P.ny;
+          float k32[6];
           for (int c = 0; c < 6; ++c) {
-            float k32 = q->K_in[(size_t)c * nn + i * n + j];
-            if (!(std::fabs(k32) > thr)) k32 = 0.0f;
-            const float khi = bf16_round_host(k32);
-            const float klo = bf16_round_host(k32 - khi);
-            float* t = &tab[((size_t)o * 6 + c) * 3];
-            t[0] = khi;
-            t[1] = klo;
-            t[2] = (k32 - khi) - klo;
+            k32[c] = q->K_in[(size_t)c * nn + i * n + j];
+            if (!(std::fabs(k32[c]) > thr)) k32[c] = 0.0f;
           }
+          near_pack_offset(k32, &tab[(size_t)o * kNearWords]);
         }
-        MT_HIP(hipMemcpyAsync(P.near_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, P.stream));
+        MT_HIP(hipMemcpyAsync(P.near_tab, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice, P.stream));
         MT_HIP(hipStreamSynchronize(P.stream));
       } else {
         int rc = launch_near_table(P.nx, P.ny, P.nz, q->Lx, q->Ly, q->Lz, R, Rz, thr, P.near_tab, P.stream);

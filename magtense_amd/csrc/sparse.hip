@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -430,9 +431,13 @@ int BlockSparse::build(const float* K32, int n_, float thr, int grid_nx, hipStre
   } else {
     ngroups = 0;  // nothing kept: the product is zero
   }
-  // the smallest k-split that gives every CU a workgroup
+  // the smallest k-split that gives every CU a workgroup (MT_BSR_KSPLIT overrides, for measurement)
   ksplit = 1;
   while (ngroups > 0 && ngroups * ksplit < 256 && ksplit < 16) ksplit *= 2;
+  if (const char* e = std::getenv("MT_BSR_KSPLIT")) {
+    const int k = std::atoi(e);
+    if (k >= 1 && k <= 64) ksplit = k;
+  }
   MT_HIP(hipStreamSynchronize(st));
   return kOk;
 }
