@@ -201,11 +201,14 @@ int mt_demag_sparse_info(void* sp, long long* nnz6, float* threshold);
  * precision mode of the sparse branch, LandauLifshitzEquationSolver.f90:581-624). */
 int mt_demag_sparse_create_bf16x2(const float* K32, int n, int approx, double threshold, int grid_nx, void** out);
 /* Block statistics of a tensor from mt_demag_sparse_create_bf16x2: list
- * entries (each 4 x 12 KiB of fragments), non-empty tiles, fragment bytes, k-split. */
+ * entries (each 4 x 12 KiB of fragments), non-empty tiles, fragment bytes, and
+ * the partial-sum slots of its balanced workgroup schedule (ksplit). */
 int mt_demag_sparse_blocks(void* sp, long long* entries, long long* tiles_nonempty, long long* bytes, int* ksplit);
 /* h = pref * K_sparse m for nstates states, [s][3n] device layout (the
  * product of updateDemagfield's sparse branch, LandauLifshitzEquationSolver.f90:581-624;
- * FP32 CSR, or split-BF16 tiles for a tensor of mt_demag_sparse_create_bf16x2). */
+ * FP32 CSR, or split-BF16 tiles for a tensor of mt_demag_sparse_create_bf16x2,
+ * whose workspace the handle allocates on first use and keeps: calls on one
+ * handle must not overlap). */
 int mt_demag_apply_sparse(void* sp, const float* m, float* h, int nstates, float pref);
 void mt_demag_sparse_destroy(void* sp);
 

@@ -559,10 +559,17 @@ int mt_demag_apply_sparse(void* h, const float* m, float* hout, int nstates, flo
   if (sp->blocks) {  // split-BF16 block-sparse tensor: the batched MFMA product over its tiles
     const BlockSparse& b = *sp->blocks;
     const Bf16x2Work w = bf16x2_work(b.n, b.n, 0, 0, 0, nstates, b.ksplit);
-    DevArr<uint8_t> ws;
-    MT_HIP(ws.alloc(w.bytes()));
+    // the handle keeps its workspace (grown on demand): calls on one handle
+    // must not overlap
+    if (sp->ws_bytes < w.bytes()) {
+      if (sp->ws) MT_HIP(hipFree(sp->ws));
+      sp->ws = nullptr;
+      sp->ws_bytes = 0;
+      MT_HIP(hipMalloc(&sp->ws, w.bytes()));
+      sp->ws_bytes = w.bytes();
+    }
     const BlockSparseView v = b.view();
-    return apply_bf16x2_in(b.frags, b.n, b.n, 0, 0, 0, m, hout, nstates, pref, w, ws.p, &v);
+    return apply_bf16x2_in(b.frags, b.n, b.n, 0, 0, 0, m, hout, nstates, pref, w, sp->ws, &v);
   }
   const int64_t n3 = (int64_t)3 * sp->n;
   for (int s0 = 0; s0 < nstates; s0 += kMaxCols) {

@@ -365,28 +365,57 @@ constexpr int kTableChainChunks = 512;  // longest partial sum of the entry-reus
 // offset and the k-chunks are visited in offset order (as in the LDS kernel),
 // with ordinary loads so the table stays in L2.
 // BSR: Kt holds only the (row tile, k-chunk) tiles of a block-sparse tensor
-// (BlockSparseView bsv): workgroup g owns the row tiles bsv.tiles[4g .. 4g+3]
-// (-1: none) and walks the shared chunk list bsv.chunks[bsv.goff[g] ..
+// (BlockSparseView bsv): group g owns the row tiles bsv.tiles[4g .. 4g+3]
+// (-1: none) and the shared chunk list bsv.chunks[bsv.goff[g] ..
 // bsv.goff[g+1]); wave w's fragments of list entry l sit at fragment set
 // 4 goff[g] + w cnt_g + l, so each wave still streams one contiguous region.
+// Workgroups follow the balanced schedule: workgroup b runs the segments
+// bsv.segs[bsv.wseg[b] .. bsv.wseg[b+1]), each a run of one group's list
+// whose sums go to that segment's partial-sum slot.
 template <int D, bool TOEP, bool BSR = false>
 __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __restrict__ Kt,
                                                                  const uint4* __restrict__ Mp, int64_t np,
                                                                  int ksplit, float* __restrict__ out, int n,
                                                                  int64_t col_stride, int64_t split_stride,
                                                                  int ncols, int gnx, int gny, int gnz,
-                                                                 BlockSparseView bsv) {
+                                                                 BlockSparseView bsv,
+                                                                 const int* __restrict__ bchunks,
+                                                                 const int4* __restrict__ bsegs,
+                                                                 const int64_t* __restrict__ bgoff,
+                                                                 const int* __restrict__ btiles) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   __shared__ u32x4 bs[2][12][64];
+  extern __shared__ int sidx[];  // BSR: the current segment's source chunks (bsv.seg_max)
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nJ = (int)(np / kTileK);
   const int n_row_tiles = (int)((n + kTileM - 1) / kTileM);
   const int groups = (n_row_tiles + 3) / 4;
+  // BSR: workgroup w runs the segments of its balanced range, one after another
+  // (the BSR arrays come in as separate read-only arguments as well, so that
+  // the compiler reads them with scalar loads: a vector load of the chunk index
+  // would make every chunk wait for the whole ring of fragment loads)
+  const int seg0 = BSR ? bsv.wseg[blockIdx.x] : 0, seg1 = BSR ? bsv.wseg[blockIdx.x + 1] : 1;
+  for (int sgi = seg0; sgi < seg1; ++sgi) {
   // XCD-aware order (block b runs on XCD b % 8): the workgroups resident on
   // one XCD share k-splits, hence the state slab they read through its L2.
-  int ks, group;
-  {
+  int ks, group, jb, nchunks, cnt;
+  int64_t lbase = 0;
+  if (BSR) {
+    const int4 sg = bsegs[sgi];  // (group, slot, first entry, entries)
+    group = sg.x;
+    ks = sg.y;
+    lbase = bgoff[group];
+    cnt = (int)(bgoff[group + 1] - lbase);
+    jb = sg.z;
+    nchunks = sg.w;
+    // the segment's source chunks go to LDS: an index read from memory inside
+    // the ring (a vector load, or a scalar load sharing its counter with the
+    // LDS traffic) would make chunks wait behind the fragment loads in flight
+    __syncthreads();  // the previous segment's reads of the LDS buffers are done
+    for (int i = threadIdx.x; i < nchunks; i += blockDim.x) sidx[i] = bchunks[lbase + jb + i];
+    __syncthreads();
+  } else {
     const int x = (int)(blockIdx.x % 8), q = (int)(blockIdx.x / 8);
     if (ksplit % 8 == 0) {
       const int per = ksplit / 8;
@@ -399,17 +428,15 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
       ks = (int)(blockIdx.x % ksplit);
       group = (int)(blockIdx.x / ksplit);
     }
+    if (group >= groups) return;  // whole workgroup, before any barrier
+    cnt = nJ;
+    jb = (int)((int64_t)cnt * ks / ksplit);
+    nchunks = (int)((int64_t)cnt * (ks + 1) / ksplit) - jb;
   }
-  if (group >= (BSR ? bsv.ngroups : groups)) return;  // whole workgroup, before any barrier
-  // BSR: the group's chunk list (scalar loads, wave-uniform)
-  const int64_t lbase = BSR ? bsv.goff[group] : 0;
-  const int cnt = BSR ? (int)(bsv.goff[group + 1] - lbase) : nJ;
-  const int jb = (int)((int64_t)cnt * ks / ksplit);
-  const int nchunks = (int)((int64_t)cnt * (ks + 1) / ksplit) - jb;
-  const int64_t I = BSR ? (int64_t)bsv.tiles[4 * group + wave] : (int64_t)group * 4 + wave;
+  const int64_t I = BSR ? (int64_t)btiles[4 * group + wave] : (int64_t)group * 4 + wave;
   const bool tile_ok = BSR ? I >= 0 : I < n_row_tiles;
   // a wave past the last tile loads the group's first tile (kept uniform, never consumed)
-  const int64_t Il = tile_ok ? I : (BSR ? (int64_t)bsv.tiles[4 * group] : (int64_t)group * 4);
+  const int64_t Il = tile_ok ? I : (BSR ? (int64_t)btiles[4 * group] : (int64_t)group * 4);
   const int64_t aoff = BSR ? (4 * lbase + (int64_t)(tile_ok ? wave : 0) * cnt) * 12 * 64
                            : (TOEP ? 0 : Il * nJ * 12 * 64);
   const u32x4* A = reinterpret_cast<const u32x4*>(Kt) + aoff + lane;
@@ -458,7 +485,7 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
     } else if (BSR) {                                                                                     \
       a_ = A + (int64_t)J_ * 12 * 64;                                                                     \
       _Pragma("unroll") for (int f = 0; f < 12; ++f) ar[slot][f] = __builtin_nontemporal_load(a_ + f * 64); \
-      J_ = bsv.chunks[lbase + J_];                                                                         \
+      J_ = sidx[J_ - jb];                                                                                 \
     } else {                                                                                              \
       a_ = A + (int64_t)J_ * 12 * 64;                                                                     \
       _Pragma("unroll") for (int f = 0; f < 12; ++f) ar[slot][f] = __builtin_nontemporal_load(a_ + f * 64); \
@@ -526,20 +553,22 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
   }
 #undef MT_REG_CONSUME
 #undef MT_REG_ISSUE
-  if (!tile_ok) return;
-  float* o = out + (int64_t)ks * split_stride;
+  if (tile_ok) {
+    float* o = out + (int64_t)ks * split_stride;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int col = 32 * h + (lane & 31);
-    if (col >= ncols) break;
+    for (int h = 0; h < 2; ++h) {
+      const int col = 32 * h + (lane & 31);
+      if (col >= ncols) break;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+      for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-        if (row < n) o[col * col_stride + (int64_t)a * n + row] = sum[h][a][r] + acc[h][a][r];
-      }
+        for (int r = 0; r < 16; ++r) {
+          int64_t row = I * kTileM + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+          if (row < n) o[col * col_stride + (int64_t)a * n + row] = sum[h][a][r] + acc[h][a][r];
+        }
+    }
   }
+  }  // segments
 }
 
 template <int D, bool TOEP = false, bool BSR = false>
@@ -550,10 +579,12 @@ int launch_reg(const uint8_t* K, const uint8_t* Mpack, int n, int nrows, float* 
   const int groups = BSR ? bs.ngroups : (int)((round_up(nrows, kTileM) / kTileM + 3) / 4);
   int64_t blocks = (int64_t)groups * ksplit;
   if (ksplit % 8 != 0 && 8 % ksplit == 0) blocks = round_up(blocks, 8);
+  if (BSR) blocks = bs.nwg;  // the balanced schedule's ranges
   if (blocks == 0) return kOk;
-  hipLaunchKernelGGL((gemm_bf16x2_reg_kernel<D, TOEP, BSR>), dim3((unsigned)blocks), dim3(256), 0, stream,
+  const size_t lds = BSR ? (size_t)bs.seg_max * sizeof(int) : 0;
+  hipLaunchKernelGGL((gemm_bf16x2_reg_kernel<D, TOEP, BSR>), dim3((unsigned)blocks), dim3(256), lds, stream,
                      reinterpret_cast<const uint4*>(K), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out, nrows,
-                     col_stride, split_stride, ncols, gnx, gny, gnz, bs);
+                     col_stride, split_stride, ncols, gnx, gny, gnz, bs, bs.chunks, bs.segs, bs.goff, bs.tiles);
   MT_HIP(hipGetLastError());
   return kOk;
 }

@@ -52,6 +52,11 @@ struct BlockSparseView {
   const int* tiles = nullptr;     // 4 per group: row tile index, -1 none (the first is always a tile)
   const int64_t* goff = nullptr;  // ngroups + 1 list offsets
   const int* chunks = nullptr;    // k-chunk of each list entry
+  // balanced schedule: workgroup w runs segments segs[wseg[w] .. wseg[w+1]),
+  // each (group, partial-sum slot, first list entry, entries) of one group's list
+  int nwg = 0, seg_max = 0;  // seg_max: the longest segment (its chunk indices are staged in LDS)
+  const int* wseg = nullptr;
+  const int4* segs = nullptr;
 };
 // out[ks*split_stride + col*col_stride + c*n + i] = k-split partial of K m
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
@@ -70,7 +75,11 @@ int launch_gemm_bf16x2_bsr(const uint8_t* frags, const BlockSparseView& bs, cons
                            int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
 
 struct BlockSparse {
-  int n = 0, ngroups = 0, ksplit = 1;
+  int n = 0, ngroups = 0;
+  int ksplit = 1;                 // partial-sum slots the balanced schedule writes
+  int nwg = 0;                    // workgroups of the schedule
+  int seg_max = 0;                // longest segment (list entries)
+  int max_list = 0;               // longest group list (entries)
   float threshold = 0.0f;
   int64_t entries = 0;            // list entries over all groups (each 4 x 12 KiB of fragments)
   int64_t tiles_nonempty = 0;     // (row tile, chunk) tiles holding a kept entry
@@ -79,11 +88,14 @@ struct BlockSparse {
   int64_t* goff = nullptr;
   int* chunks = nullptr;
   int* egroup = nullptr;          // group of each list entry (build only)
+  int* wseg = nullptr;
+  int4* segs = nullptr;
   uint8_t* frags = nullptr;
   // grid_nx > 0 with grid_nx % 32 == 0: uniform-grid rows of grid_nx cells, so
   // row tiles of consecutive grid rows can share a group
   int build(const float* K32, int n, float thr, int grid_nx, hipStream_t stream);
-  BlockSparseView view() const { return BlockSparseView{ngroups, tiles, goff, chunks}; }
+  int schedule(const std::vector<int64_t>& goff_host, int cus);
+  BlockSparseView view() const { return BlockSparseView{ngroups, tiles, goff, chunks, nwg, seg_max, wseg, segs}; }
   int64_t bytes() const { return entries * 4 * 12 * 1024; }
   void release();
   ~BlockSparse() { release(); }
@@ -122,6 +134,8 @@ struct SparseDemag {
   float* hT = nullptr;
   int tcols = 0;
   BlockSparse* blocks = nullptr;  // split-BF16 block-sparse form (precision bf16x2) instead of the CSR
+  uint8_t* ws = nullptr;          // the block-sparse product's workspace, kept between calls
+  size_t ws_bytes = 0;
   std::vector<std::vector<int>> host_cols;    // components loaded from host CSR
   std::vector<std::vector<float>> host_vals;
   int build(const float* K32, int n, float thr, hipStream_t stream);

@@ -321,8 +321,10 @@ void BlockSparse::release() {
   if (chunks) (void)hipFree(chunks);
   if (egroup) (void)hipFree(egroup);
   if (frags) (void)hipFree(frags);
-  tiles = nullptr; goff = nullptr; chunks = nullptr; egroup = nullptr; frags = nullptr;
-  n = 0; ngroups = 0; ksplit = 1; entries = 0; tiles_nonempty = 0;
+  if (wseg) (void)hipFree(wseg);
+  if (segs) (void)hipFree(segs);
+  tiles = nullptr; goff = nullptr; chunks = nullptr; egroup = nullptr; frags = nullptr; wseg = nullptr; segs = nullptr;
+  n = 0; ngroups = 0; ksplit = 1; nwg = 0; seg_max = 0; max_list = 0; entries = 0; tiles_nonempty = 0;
   for (int c = 0; c < 6; ++c) nnz[c] = 0;
 }
 
@@ -431,20 +433,72 @@ int BlockSparse::build(const float* K32, int n_, float thr, int grid_nx, hipStre
   } else {
     ngroups = 0;  // nothing kept: the product is zero
   }
-  // the smallest k-split that gives every CU a workgroup (MT_BSR_KSPLIT overrides, for measurement)
-  ksplit = 1;
-  while (ngroups > 0 && ngroups * ksplit < 256 && ksplit < 16) ksplit *= 2;
-  if (const char* e = std::getenv("MT_BSR_KSPLIT")) {
-    const int k = std::atoi(e);
-    if (k >= 1 && k <= 64) ksplit = k;
+  if (ngroups > 0) {
+    int cus = 0, dev = 0;
+    MT_HIP(hipGetDevice(&dev));
+    MT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int rc = schedule(go, cus > 0 ? cus : 256);
+    if (rc) return rc;
   }
   MT_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+// Balanced schedule over the group lists (list lengths vary between groups
+// by a factor of two and more on a thin film; with one resident workgroup per
+// CU the longest list would set the time).  The concatenated lists are cut
+// into nwg ranges of about entries / cus entries (at least kMinRange), one per
+// workgroup; a range covers the end of one group's list, whole lists and the
+// start of the next, as segments.  The pieces of one group go to partial-sum
+// slots 0, 1, ... in range order; ksplit = the most pieces of any group, and a
+// group with fewer pieces gets segments of no entries for its remaining slots
+// (they write zeros), in the workgroup that holds its last piece.
+int BlockSparse::schedule(const std::vector<int64_t>& go, int cus) {
+  // ranges of at least kMinRange entries, at most kMaxRange (their chunk
+  // indices are staged in LDS beside the kernel's 24 KiB)
+  constexpr int64_t kMinRange = 32, kMaxRange = 16384;
+  const int64_t T = std::min(std::max<int64_t>((entries + cus - 1) / cus, kMinRange), kMaxRange);
+  nwg = (int)((entries + T - 1) / T);
+  std::vector<std::vector<int4>> per(nwg);
+  std::vector<int> pieces(ngroups, 0), last_w(ngroups, 0);
+  max_list = 0;
+  for (int g = 0; g < ngroups; ++g) {
+    max_list = std::max<int>(max_list, (int)(go[g + 1] - go[g]));
+    for (int64_t e = go[g]; e < go[g + 1];) {
+      const int w = (int)(e / T);
+      const int64_t e1 = std::min(go[g + 1], (int64_t)(w + 1) * T);
+      per[w].push_back(make_int4(g, pieces[g]++, (int)(e - go[g]), (int)(e1 - e)));
+      last_w[g] = w;
+      e = e1;
+    }
+    if (go[g + 1] == go[g]) last_w[g] = std::min<int>((int)(go[g] / T), nwg - 1);
+  }
+  ksplit = 1;
+  for (int g = 0; g < ngroups; ++g) ksplit = std::max(ksplit, pieces[g]);
+  seg_max = 0;
+  for (const auto& pw : per)
+    for (const int4& q : pw) seg_max = std::max(seg_max, q.w);
+  for (int g = 0; g < ngroups; ++g)
+    for (int k = pieces[g]; k < ksplit; ++k) per[last_w[g]].push_back(make_int4(g, k, 0, 0));
+  std::vector<int> ws(nwg + 1, 0);
+  std::vector<int4> sg;
+  for (int w = 0; w < nwg; ++w) {
+    sg.insert(sg.end(), per[w].begin(), per[w].end());
+    ws[w + 1] = (int)sg.size();
+  }
+  MT_HIP(hipMalloc(&wseg, ws.size() * sizeof(int)));
+  MT_HIP(hipMalloc(&segs, sg.size() * sizeof(int4)));
+  MT_HIP(hipMemcpy(wseg, ws.data(), ws.size() * sizeof(int), hipMemcpyHostToDevice));
+  MT_HIP(hipMemcpy(segs, sg.data(), sg.size() * sizeof(int4), hipMemcpyHostToDevice));
   return kOk;
 }
 
 void SparseDemag::release() {
   delete blocks;
   blocks = nullptr;
+  if (ws) (void)hipFree(ws);
+  ws = nullptr;
+  ws_bytes = 0;
   if (rowptr) (void)hipFree(rowptr);
   if (cols) (void)hipFree(cols);
   if (vals) (void)hipFree(vals);
