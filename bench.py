@@ -616,8 +616,9 @@ def _line(wall, st, timed_st, n, cells_bytes):
         out["gpu_kernel_frac_of_step"] = kern / out["ms_per_step"]
         if cells_bytes:
             gbs = cells_bytes / (out["demag_ms_per_launch"] * 1e-3) / 1e9
-            out["roofline"] = {"bound": "hbm", "kernel": "matvec_f32_kernel", "achieved": gbs, "peak": HBM_PEAK_GBS,
-                               "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "alg_bytes_per_launch": cells_bytes}
+            out["roofline"] = {"bound": "hbm", "kernel": "demag_llg_f32_kernel (FP32 demag rows + LL right-hand side)",
+                               "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                               "alg_bytes_per_launch": cells_bytes}
     return out
 
 
@@ -654,8 +655,10 @@ def small_configs(no_cpu=False):
     _, st_t, _ = _gpu_run(res, L, m_rel, h_dyn, 1e-9, 200, 2000, timing=True, gamma=2.21e5, **kw)
     c1 = {"workload": "32x8x1 film (15.6x15.6x3 nm), N = 256: relaxation 5 ns (gamma 0) + 1 ns dynamics, FP32 demag",
           "relax": _line(w_rel, st_rel, None, 256, 0), "dynamics": _line(w_dyn, st_dyn, st_t, 256, 0),
-          "note": "the 1.6 MB tensor stays in L2: the step is bound by launch latency and the host round trips "
-                  "of the RKSUITE controller, not by HBM"}
+          "note": "the 1.6 MB tensor stays in L2: the step is bound by kernel latency (two launches per right-hand "
+                  "side: stage combination, then FP32 demag rows fused with the LL right-hand side) and the one host "
+                  "round trip per step of the RKSUITE controller, not by HBM; kernel_ms_per_step.demag includes "
+                  "the LL right-hand side"}
     if not no_cpu:
         ht = np.linspace(0, 5e-9, 200)
         hext = np.zeros((200, 4))
@@ -842,7 +845,8 @@ def main():
                           passes_per_batch_step=plain_passes / max(bs2, 1e-9),
                           ms_per_pass=1e3 * wall2 / max(plain_passes, 1),
                           note="one pass = the 64 per-state FP32 mat-vecs of one right-hand side "
-                               "(matvec_f32_kernel, 6 N^2 4 B per state)")
+                               "(demag_llg_f32_kernel: the FP32 rows, 6 N^2 4 B per state, fused with the LL "
+                               "right-hand side)")
         log(f"fp32 plain: {args.plain_steps} steps in {wall2:.3f} s = {plain:.4f} steps/s, {plain_passes} passes")
     cfgs = None if args.no_configs else small_configs(no_cpu=args.no_cpu)
     log(f"plain-method configurations: {json.dumps(cfgs)}")

@@ -24,6 +24,7 @@
 
 #include <cstdlib>
 
+#include "matvec_rows.h"
 #include "mt_common.h"
 #include "mt_kernels.h"
 
@@ -32,87 +33,22 @@ namespace mt {
 namespace {
 
 // ---------------------------------------------------------------------------
-// Plain FP32 per-state mat-vec: block = RB receiver rows of one state.
+// Plain FP32 per-state mat-vec: block = RB receiver rows of one state (the
+// row sums of matvec_rows.h).
 // ---------------------------------------------------------------------------
-__device__ inline float4 ldnt4(const float* p) {
-  typedef float f32x4 __attribute__((ext_vector_type(4)));
-  const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
-  return make_float4(v.x, v.y, v.z, v.w);
-}
-
 template <int RB, bool VEC4>
 __global__ void __launch_bounds__(256) matvec_f32_kernel(const float* __restrict__ K, int n,
                                                          const float* __restrict__ m, int64_t m_stride,
                                                          float* __restrict__ out, int64_t out_stride) {
   const int s = blockIdx.y;
   const int i0 = blockIdx.x * RB;
-  const float* mx = m + s * m_stride;
-  const float* my = mx + n;
-  const float* mz = my + n;
-  const size_t nn = (size_t)n * n;
-  float acc[RB][3];
-#pragma unroll
-  for (int r = 0; r < RB; ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.0f;
-  if (VEC4) {
-    for (int j = threadIdx.x * 4; j < n; j += 256 * 4) {
-      float4 vx = *reinterpret_cast<const float4*>(mx + j);
-      float4 vy = *reinterpret_cast<const float4*>(my + j);
-      float4 vz = *reinterpret_cast<const float4*>(mz + j);
-#pragma unroll
-      for (int r = 0; r < RB; ++r) {
-        int i = i0 + r;
-        if (i >= n) break;
-        // the tensor is streamed once per state: non-temporal loads, so that it
-        // does not displace the state vectors from L2
-        const float* row = K + (size_t)i * n + j;
-        float4 kxx = ldnt4(row);
-        float4 kxy = ldnt4(row + nn);
-        float4 kxz = ldnt4(row + 2 * nn);
-        float4 kyy = ldnt4(row + 3 * nn);
-        float4 kyz = ldnt4(row + 4 * nn);
-        float4 kzz = ldnt4(row + 5 * nn);
-        acc[r][0] += kxx.x * vx.x + kxy.x * vy.x + kxz.x * vz.x + kxx.y * vx.y + kxy.y * vy.y + kxz.y * vz.y +
-                     kxx.z * vx.z + kxy.z * vy.z + kxz.z * vz.z + kxx.w * vx.w + kxy.w * vy.w + kxz.w * vz.w;
-        acc[r][1] += kxy.x * vx.x + kyy.x * vy.x + kyz.x * vz.x + kxy.y * vx.y + kyy.y * vy.y + kyz.y * vz.y +
-                     kxy.z * vx.z + kyy.z * vy.z + kyz.z * vz.z + kxy.w * vx.w + kyy.w * vy.w + kyz.w * vz.w;
-        acc[r][2] += kxz.x * vx.x + kyz.x * vy.x + kzz.x * vz.x + kxz.y * vx.y + kyz.y * vy.y + kzz.y * vz.y +
-                     kxz.z * vx.z + kyz.z * vy.z + kzz.z * vz.z + kxz.w * vx.w + kyz.w * vy.w + kzz.w * vz.w;
-      }
-    }
-  } else {
-    for (int j = threadIdx.x; j < n; j += 256) {
-      float vx = mx[j], vy = my[j], vz = mz[j];
-#pragma unroll
-      for (int r = 0; r < RB; ++r) {
-        int i = i0 + r;
-        if (i >= n) break;
-        const float* row = K + (size_t)i * n + j;
-        float kxx = row[0], kxy = row[nn], kxz = row[2 * nn], kyy = row[3 * nn], kyz = row[4 * nn],
-              kzz = row[5 * nn];
-        acc[r][0] += kxx * vx + kxy * vy + kxz * vz;
-        acc[r][1] += kxy * vx + kyy * vy + kyz * vz;
-        acc[r][2] += kxz * vx + kyz * vy + kzz * vz;
-      }
-    }
-  }
-  // block reduction of RB*3 sums: wave shuffle, then across the four waves
   __shared__ float red[4][RB * 3];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int r = 0; r < RB; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      float v = acc[r][c];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-      if (lane == 0) red[wave][r * 3 + c] = v;
-    }
-  __syncthreads();
+  __shared__ float sums[RB * 3];
+  matvec_rows_f32<RB, VEC4>(K, n, m + s * m_stride, i0, red, sums);
   if (threadIdx.x < RB * 3) {
     int r = threadIdx.x / 3, c = threadIdx.x % 3;
     int i = i0 + r;
-    float v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-    if (i < n) out[s * out_stride + (int64_t)c * n + i] = v;
+    if (i < n) out[s * out_stride + (int64_t)c * n + i] = sums[threadIdx.x];
   }
 }
 
@@ -348,7 +284,7 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
 }
 
 constexpr int kFlushChunks = 128;  // chunks summed in the MFMA accumulators before they are flushed
-constexpr int kTableChainChunks = 512;  // longest partial sum of the entry-reuse table kernel
+constexpr int kTableChainChunks = 256;  // longest partial sum of the entry-reuse table kernel
 
 // Register-ring kernel for the dense tensor.  Each wave owns one 32-row tile
 // and all 64 states (six accumulators); its A fragments (12 per k-chunk,
@@ -395,7 +331,12 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
   // (the BSR arrays come in as separate read-only arguments as well, so that
   // the compiler reads them with scalar loads: a vector load of the chunk index
   // would make every chunk wait for the whole ring of fragment loads)
-  const int seg0 = BSR ? bsv.wseg[blockIdx.x] : 0, seg1 = BSR ? bsv.wseg[blockIdx.x + 1] : 1;
+  // XCD x (blocks b with b % 8 == x) runs the x-th eighth of the ranges
+  // (gridDim.x % 8 == 0), consecutive groups of neighbouring rows (measured
+  // within 1 % of the plain order: the workgroups of an XCD still sit at
+  // different sources of their lists at any one time)
+  const int wgi = BSR ? (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3)) : 0;
+  const int seg0 = BSR ? bsv.wseg[wgi] : 0, seg1 = BSR ? bsv.wseg[wgi + 1] : 1;
   for (int sgi = seg0; sgi < seg1; ++sgi) {
   // XCD-aware order (block b runs on XCD b % 8): the workgroups resident on
   // one XCD share k-splits, hence the state slab they read through its L2.
@@ -1039,7 +980,9 @@ int gemm_ksplit_table(int nx, int ny, int nz) {
   // workgroup, within the divisors of the source rows; and no partial sum
   // over more than kTableChainChunks chunks (the kernel's accumulators are
   // single-level FP32: long chains of additions lose accuracy on coherent
-  // states; 4 at 256 x 64 x 2)
+  // states; 8 at 256 x 64 x 2, where chains of 512 chunks left 1.07e-5 of the
+  // FP64 field on a smooth in-plane state and chains of 256 leave 7.3e-6 at
+  // the same speed)
   const int rows = ny * nz, groups = n / kTileM / 4, nJ = n / kTileK;
   int ks = 1;
   while ((groups * ks < 256 || nJ / ks > kTableChainChunks) && rows % (ks * 2) == 0) ks *= 2;

@@ -17,6 +17,7 @@
 
 #include <hip/hip_bf16.h>
 
+#include "matvec_rows.h"
 #include "mt_common.h"
 #include "mt_kernels.h"
 
@@ -30,20 +31,12 @@ __device__ inline uint16_t bf16_bits(float v) {
 }
 __device__ inline float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// One thread per (cell, state).  The grid is 1-D with a multiple of 8 blocks;
-// block b runs on XCD b % 8, and XCD x takes the contiguous x-th eighth of the
-// (state, cell block) range, so the exchange stencil's neighbour rows and
-// planes are read through the L2 that already holds them.
-__global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc* __restrict__ descs,
-                                                  int cell_blocks, int nstates) {
-  const int total = cell_blocks * nstates;
-  const int per_xcd = (int)(gridDim.x / 8);
-  const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-  if (w >= total) return;
-  const LlgDesc& D = descs[w / cell_blocks];
+// LL right-hand side of cell i for the state of descriptor D.  hsum(c)
+// returns the FP32 demag sum S_c = sum of the partials of (K m)_c at i (used
+// unless D.hpart64 holds FP64 sums).
+template <typename HS>
+__device__ inline void llg_cell(const LlgGrid& g, const LlgDesc& D, int i, HS hsum) {
   const int n = g.n;
-  const int i = (w % cell_blocks) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   const double* m = D.m;
   const int ix = i % g.nx;
   const int iy = (i / g.nx) % g.ny;
@@ -98,8 +91,7 @@ __global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc
     for (int c = 0; c < 3; ++c) hm[c] = (-D.hpart64[(int64_t)c * n + i]) * g.ms64[i];
   } else {
     for (int c = 0; c < 3; ++c) {
-      float s = 0.0f;
-      for (int k = 0; k < D.nsplit; ++k) s += D.hpart[(int64_t)k * D.split_stride + (int64_t)c * n + i];
+      const float s = hsum(c);
       float h = (-s) * g.ms32[i];
       hm[c] = (double)h;
     }
@@ -136,6 +128,52 @@ __global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc
       F[(size_t)(9 + c) * n + i] = hk[c];
     }
   }
+}
+
+// One thread per (cell, state).  The grid is 1-D with a multiple of 8 blocks;
+// block b runs on XCD b % 8, and XCD x takes the contiguous x-th eighth of the
+// (state, cell block) range, so the exchange stencil's neighbour rows and
+// planes are read through the L2 that already holds them.  INL: descriptors
+// by value (inl) instead of from descs.
+template <bool INL>
+__global__ void __launch_bounds__(256) llg_kernel(const LlgGrid g, const LlgDesc* __restrict__ descs,
+                                                  LlgDescsInline inl, int cell_blocks, int nstates) {
+  const int total = cell_blocks * nstates;
+  const int per_xcd = (int)(gridDim.x / 8);
+  const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (w >= total) return;
+  const LlgDesc& D = INL ? inl.d[w / cell_blocks] : descs[w / cell_blocks];
+  const int i = (w % cell_blocks) * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  llg_cell(g, D, i, [&](int c) {
+    float s = 0.0f;
+    for (int k = 0; k < D.nsplit; ++k) s += D.hpart[(int64_t)k * D.split_stride + (int64_t)c * g.n + i];
+    return s;
+  });
+}
+
+// The FP32 dense path's demag rows and LL right-hand side in one launch:
+// block = kFusedRows receiver rows of state blockIdx.y (the sums of
+// matvec_rows_f32), then one thread per row evaluates llg_cell with them.
+constexpr int kFusedRows = 4;
+template <bool INL, bool VEC4>
+__global__ void __launch_bounds__(256) demag_llg_f32_kernel(const float* __restrict__ K, const LlgGrid g,
+                                                            const LlgDesc* __restrict__ descs, LlgDescsInline inl,
+                                                            const float* __restrict__ m32, int64_t m_stride) {
+  const int s = blockIdx.y;
+  const int i0 = blockIdx.x * kFusedRows;
+  __shared__ float red[4][kFusedRows * 3];
+  __shared__ float sums[kFusedRows * 3];
+  matvec_rows_f32<kFusedRows, VEC4>(K, g.n, m32 + s * m_stride, i0, red, sums);
+  const int r = threadIdx.x;
+  if (r >= kFusedRows || i0 + r >= g.n) return;
+  const LlgDesc& D = INL ? inl.d[s] : descs[s];
+  // one partial-sum slot: S = 0 + row sum, as the LL kernel forms it
+  llg_cell(g, D, i0 + r, [&](int c) {
+    float v = 0.0f;
+    v += sums[r * 3 + c];
+    return v;
+  });
 }
 
 __device__ inline double nan_to_inf(double v) { return isnan(v) ? INFINITY : v; }
@@ -179,10 +217,13 @@ __device__ inline void accumulate(double (&acc)[kVecE], int cnt, PF ptr, CF coef
   }
 }
 
-__global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ descs, int n, int n3,
-                                                  int blocks_per_state, double* __restrict__ partials,
-                                                  PackSpec pack) {
-  const VecDesc& D = descs[blockIdx.y];
+// INL: descriptors by value (inl) instead of from descs.  With one block
+// per state the block's reduction is the result (written to results).
+template <bool INL>
+__global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ descs, VecDescsInline inl, int n,
+                                                  int n3, int blocks_per_state, double* __restrict__ partials,
+                                                  double* __restrict__ results, PackSpec pack) {
+  const VecDesc& D = INL ? inl.d[blockIdx.y] : descs[blockIdx.y];
   const int t0 = blockIdx.x * 2048;
   const int e0 = t0 + threadIdx.x;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
@@ -377,17 +418,21 @@ __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ de
     int k = threadIdx.x;
     double x = red[0][k];
     for (int w = 1; w < 4; ++w) x = is_sum ? x + red[w][k] : fmax(x, red[w][k]);
-    partials[((int64_t)blockIdx.y * blocks_per_state + blockIdx.x) * 4 + k] = x;
+    if (blocks_per_state == 1 && results) results[D.rslot * 4 + k] = x;
+    else partials[((int64_t)blockIdx.y * blocks_per_state + blockIdx.x) * 4 + k] = x;
   }
 }
 
 // One thread per (state, value): fixed-order combine of the block partials.
-__global__ void finalize_kernel(const VecDesc* __restrict__ descs, int nstates, int blocks_per_state,
-                                const double* __restrict__ partials, double* __restrict__ results) {
+template <bool INL>
+__global__ void finalize_kernel(const VecDesc* __restrict__ descs, VecDescsInline inl, int nstates,
+                                int blocks_per_state, const double* __restrict__ partials,
+                                double* __restrict__ results) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nstates * 4) return;
   int s = t / 4, k = t % 4;
-  int kind = descs[s].rkind;
+  const VecDesc& D = INL ? inl.d[s] : descs[s];
+  int kind = D.rkind;
   // a descriptor without a reduction leaves its slot alone: a result written
   // by an earlier launch may still be waiting to be read
   if (kind == kRedNone || kind == kRedWeightsAvg) return;
@@ -397,12 +442,18 @@ __global__ void finalize_kernel(const VecDesc* __restrict__ descs, int nstates, 
     double o = partials[((int64_t)s * blocks_per_state + b) * 4 + k];
     x = is_sum ? x + o : fmax(x, o);
   }
-  results[descs[s].rslot * 4 + k] = x;
+  results[D.rslot * 4 + k] = x;
 }
 
 }  // namespace
 
-int launch_llg(const LlgGrid& g, const LlgDesc* d_descs, int nstates, hipStream_t stream) {
+// kernel arguments are limited to 4 KiB
+static_assert(sizeof(VecDescsInline) + 256 <= 4096, "vector kernel arguments exceed 4 KiB");
+static_assert(sizeof(LlgDescsInline) + sizeof(LlgGrid) + 256 <= 4096, "LL kernel arguments exceed 4 KiB");
+
+namespace {
+template <bool INL>
+int launch_llg_t(const LlgGrid& g, const LlgDesc* descs, const LlgDescsInline& inl, int nstates, hipStream_t stream) {
   const int cell_blocks = (g.n + 255) / 256;
   const int64_t total = (int64_t)cell_blocks * nstates;
   const int64_t blocks = (total + 7) / 8 * 8;
@@ -410,7 +461,71 @@ int launch_llg(const LlgGrid& g, const LlgDesc* d_descs, int nstates, hipStream_
     set_error("LL kernel: too many cells x states for one launch");
     return kErrArg;
   }
-  hipLaunchKernelGGL(llg_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, g, d_descs, cell_blocks, nstates);
+  hipLaunchKernelGGL(llg_kernel<INL>, dim3((unsigned)blocks), dim3(256), 0, stream, g, descs, inl, cell_blocks,
+                     nstates);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+template <bool INL>
+int launch_vec_t(const VecDesc* descs, const VecDescsInline& inl, int nstates, int n, double* partials,
+                 double* results, const PackSpec& pack, bool reduce, hipStream_t stream) {
+  int n3 = 3 * n;
+  int bps = vec_blocks_per_state(n3);
+  dim3 grid((unsigned)bps, (unsigned)nstates);
+  hipLaunchKernelGGL(vec_kernel<INL>, grid, dim3(256), 0, stream, descs, inl, n, n3, bps, partials, results, pack);
+  MT_HIP(hipGetLastError());
+  if (reduce && bps > 1) {
+    int t = nstates * 4;
+    hipLaunchKernelGGL(finalize_kernel<INL>, dim3((unsigned)((t + 127) / 128)), dim3(128), 0, stream, descs, inl,
+                       nstates, bps, partials, results);
+    MT_HIP(hipGetLastError());
+  }
+  return kOk;
+}
+}  // namespace
+
+int launch_llg(const LlgGrid& g, const LlgDesc* d_descs, int nstates, hipStream_t stream) {
+  return launch_llg_t<false>(g, d_descs, LlgDescsInline{}, nstates, stream);
+}
+
+int launch_llg_inline(const LlgGrid& g, const LlgDesc* h_descs, int nstates, hipStream_t stream) {
+  if (nstates < 1 || nstates > kInlineStates) {
+    set_error("LL kernel: descriptors by value hold at most kInlineStates states");
+    return kErrArg;
+  }
+  LlgDescsInline inl{};
+  for (int s = 0; s < nstates; ++s) inl.d[s] = h_descs[s];
+  return launch_llg_t<true>(g, nullptr, inl, nstates, stream);
+}
+
+int launch_demag_llg_f32(const float* K, const LlgGrid& g, const LlgDesc* d_descs, const LlgDesc* h_descs,
+                         const float* m32, int64_t m_stride, int nstates, hipStream_t stream) {
+  LlgDescsInline inl{};
+  if (!d_descs) {
+    if (nstates < 1 || nstates > kInlineStates) {
+      set_error("demag + LL kernel: descriptors by value hold at most kInlineStates states");
+      return kErrArg;
+    }
+    for (int s = 0; s < nstates; ++s) inl.d[s] = h_descs[s];
+  }
+  dim3 grid((unsigned)((g.n + kFusedRows - 1) / kFusedRows), (unsigned)nstates);
+  const bool vec4 = (g.n % 4) == 0 && (m_stride % 4) == 0;
+  if (d_descs) {
+    if (vec4)
+      hipLaunchKernelGGL((demag_llg_f32_kernel<false, true>), grid, dim3(256), 0, stream, K, g, d_descs, inl, m32,
+                         m_stride);
+    else
+      hipLaunchKernelGGL((demag_llg_f32_kernel<false, false>), grid, dim3(256), 0, stream, K, g, d_descs, inl, m32,
+                         m_stride);
+  } else {
+    if (vec4)
+      hipLaunchKernelGGL((demag_llg_f32_kernel<true, true>), grid, dim3(256), 0, stream, K, g, nullptr, inl, m32,
+                         m_stride);
+    else
+      hipLaunchKernelGGL((demag_llg_f32_kernel<true, false>), grid, dim3(256), 0, stream, K, g, nullptr, inl, m32,
+                         m_stride);
+  }
   MT_HIP(hipGetLastError());
   return kOk;
 }
@@ -419,18 +534,18 @@ int vec_blocks_per_state(int n3) { return (n3 + 256 * 8 - 1) / (256 * 8); }
 
 int launch_vec(const VecDesc* d_descs, int nstates, int n, double* partials, double* results, const PackSpec& pack,
                bool reduce, hipStream_t stream) {
-  int n3 = 3 * n;
-  int bps = vec_blocks_per_state(n3);
-  dim3 grid((unsigned)bps, (unsigned)nstates);
-  hipLaunchKernelGGL(vec_kernel, grid, dim3(256), 0, stream, d_descs, n, n3, bps, partials, pack);
-  MT_HIP(hipGetLastError());
-  if (reduce) {
-    int t = nstates * 4;
-    hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)((t + 127) / 128)), dim3(128), 0, stream, d_descs, nstates, bps,
-                       partials, results);
-    MT_HIP(hipGetLastError());
+  return launch_vec_t<false>(d_descs, VecDescsInline{}, nstates, n, partials, results, pack, reduce, stream);
+}
+
+int launch_vec_inline(const VecDesc* h_descs, int nstates, int n, double* partials, double* results,
+                      const PackSpec& pack, bool reduce, hipStream_t stream) {
+  if (nstates < 1 || nstates > kInlineStates) {
+    set_error("vector kernel: descriptors by value hold at most kInlineStates states");
+    return kErrArg;
   }
-  return kOk;
+  VecDescsInline inl{};
+  for (int s = 0; s < nstates; ++s) inl.d[s] = h_descs[s];
+  return launch_vec_t<true>(nullptr, inl, nstates, n, partials, results, pack, reduce, stream);
 }
 
 }  // namespace mt

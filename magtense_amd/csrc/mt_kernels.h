@@ -180,6 +180,24 @@ struct LlgDesc {
 
 int launch_llg(const LlgGrid& g, const LlgDesc* d_descs, int nstates, hipStream_t stream);
 
+// Small batches (up to kInlineStates states) pass their descriptors by value
+// in the launch arguments: no host -> device copy, and so no copy kernel, in
+// front of the launch (a tick of a small problem is a few kernels of a few
+// microseconds each).
+constexpr int kInlineStates = 4;
+struct LlgDescsInline {
+  LlgDesc d[kInlineStates];
+};
+int launch_llg_inline(const LlgGrid& g, const LlgDesc* h_descs, int nstates, hipStream_t stream);
+// Plain FP32 demag rows and the LL right-hand side in one launch (the FP32
+// dense path): the rows of matvec_f32 for each state, then the LL kernel's
+// arithmetic for the block's cells with those sums (bit-identical to
+// launch_matvec_f32 + launch_llg with one partial-sum slot).  Descriptors from
+// d_descs, or by value from h_descs when d_descs is null (nstates <=
+// kInlineStates); m32 + s * m_stride is state s's FP32 demag input.
+int launch_demag_llg_f32(const float* K, const LlgGrid& g, const LlgDesc* d_descs, const LlgDesc* h_descs,
+                         const float* m32, int64_t m_stride, int nstates, hipStream_t stream);
+
 // Near-band correction of the split-BF16 product on a uniform nx x ny x nz
 // grid: the FP32 product's difference from the three split products, summed
 // over the sources within R cells in x and y and min(R, nz - 1) layers in z.
@@ -287,7 +305,16 @@ struct PackSpec {
 };
 
 int vec_blocks_per_state(int n3);
+// With one block per state (3n <= 2048) the vector kernel writes the
+// reductions' results itself (no combine launch).
 int launch_vec(const VecDesc* d_descs, int nstates, int n, double* partials, double* results, const PackSpec& pack,
                bool reduce, hipStream_t stream);
+
+struct VecDescsInline {
+  VecDesc d[kInlineStates];
+};
+// launch_vec with the descriptors by value (nstates <= kInlineStates)
+int launch_vec_inline(const VecDesc* h_descs, int nstates, int n, double* partials, double* results,
+                      const PackSpec& pack, bool reduce, hipStream_t stream);
 
 }  // namespace mt

@@ -1092,10 +1092,15 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       if (d[k].rkind != kRedNone && d[k].rkind != kRedWeightsAvg) vec_reduce = true;
     }
     if (timing) MT_HIP(hipEventRecord(ev[0], stream));
-    const VecDesc* dd = nullptr;
-    int rc = stage_descs(d, &dd);
-    if (rc) return rc;
-    rc = launch_vec(dd, (int)d.size(), n, partials, res_vec, nopack, vec_reduce, stream);
+    int rc;
+    if ((int)d.size() <= kInlineStates) {
+      rc = launch_vec_inline(d.data(), (int)d.size(), n, partials, res_vec, nopack, vec_reduce, stream);
+    } else {
+      const VecDesc* dd = nullptr;
+      rc = stage_descs(d, &dd);
+      if (rc) return rc;
+      rc = launch_vec(dd, (int)d.size(), n, partials, res_vec, nopack, vec_reduce, stream);
+    }
     if (rc) return rc;
     need_sync |= vec_reduce;
     if (timing) {
@@ -1145,16 +1150,28 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     if (precision == MT_PREC_BF16X2) pk = PackSpec{kPackBf16, nullptr, 0, packb};
     else pk = PackSpec{kPackF32, packf, (int64_t)N3, nullptr};
     if (timing) MT_HIP(hipEventRecord(ev[2], stream));
-    // the group's three descriptor arrays go over in one copy
-    int rc = stage_reserve(3 * 256 + (size_t)S * (2 * sizeof(VecDesc) + sizeof(LlgDesc)));
-    if (rc) return rc;
-    const VecDesc* dpre = stage_put(pre);
-    const LlgDesc* dl = stage_put(ld);
-    const VecDesc* dpost = post_red ? stage_put(post) : nullptr;
-    rc = stage_flush();
-    if (rc) return rc;
-    rc = launch_vec(dpre, S, n, partials + (size_t)B * vec_blocks_per_state((int)N3) * 4, res_pre, pk, pre_red,
-                    stream);
+    // Small groups pass their descriptors by value; otherwise the group's
+    // three descriptor arrays go over in one copy (the near-band correction
+    // reads the LL descriptors from device memory)
+    const bool inl = S <= kInlineStates && !(precision == MT_PREC_BF16X2 && near_r > 0);
+    // the FP32 dense path runs its demag rows and the LL right-hand side in one kernel
+    const bool fused = precision != MT_PREC_BF16X2 && demag_approx == MT_DEMAG_DENSE;
+    const VecDesc* dpre = nullptr;
+    const LlgDesc* dl = nullptr;
+    const VecDesc* dpost = nullptr;
+    int rc;
+    if (!inl) {
+      rc = stage_reserve(3 * 256 + (size_t)S * (2 * sizeof(VecDesc) + sizeof(LlgDesc)));
+      if (rc) return rc;
+      dpre = stage_put(pre);
+      dl = stage_put(ld);
+      dpost = post_red ? stage_put(post) : nullptr;
+      rc = stage_flush();
+      if (rc) return rc;
+    }
+    double* pre_parts = partials + (size_t)B * vec_blocks_per_state((int)N3) * 4;
+    rc = inl ? launch_vec_inline(pre.data(), S, n, pre_parts, res_pre, pk, pre_red, stream)
+             : launch_vec(dpre, S, n, pre_parts, res_pre, pk, pre_red, stream);
     if (rc) return rc;
     if (timing) MT_HIP(hipEventRecord(ev[3], stream));
     if (precision == MT_PREC_BF16X2) {
@@ -1174,17 +1191,20 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     } else if (demag_approx != MT_DEMAG_DENSE) {
       rc = sparse.apply(packf, (int64_t)N3, S, 1.0f, hbuf, stream);
     } else {
-      rc = launch_matvec_f32(K32, n, packf, (int64_t)N3, hbuf, (int64_t)N3, S, stream);
+      rc = launch_demag_llg_f32(K32, grid, dl, ld.data(), packf, (int64_t)N3, S, stream);
     }
     if (rc) return rc;
     if (timing) MT_HIP(hipEventRecord(ev[4], stream));
     demag_passes += 1;
-    rc = launch_llg(grid, dl, S, stream);
-    if (rc) return rc;
+    if (!fused) {
+      rc = inl ? launch_llg_inline(grid, ld.data(), S, stream) : launch_llg(grid, dl, S, stream);
+      if (rc) return rc;
+    }
     if (timing) MT_HIP(hipEventRecord(ev[5], stream));
     if (post_red) {
-      rc = launch_vec(dpost, S, n, partials + (size_t)2 * B * vec_blocks_per_state((int)N3) * 4, res_post, nopack,
-                      true, stream);
+      double* post_parts = partials + (size_t)2 * B * vec_blocks_per_state((int)N3) * 4;
+      rc = inl ? launch_vec_inline(post.data(), S, n, post_parts, res_post, nopack, true, stream)
+               : launch_vec(dpost, S, n, post_parts, res_post, nopack, true, stream);
       if (rc) return rc;
     }
     if (timing) {
@@ -1201,9 +1221,10 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
         MT_HIP(hipEventElapsedTime(&mn, ev[7], ev[4]));
         near_ms += mn;
       }
-      llg_ms += ms[2];
-      llg_launches += 1;
-      llg_bytes += llg_launch_bytes(ld);
+      // fused: the LL right-hand side ran inside the demag kernel (its time is in gemm_ms)
+      llg_ms += fused ? 0.0f : ms[2];
+      llg_launches += fused ? 0 : 1;
+      llg_bytes += fused ? 0.0 : llg_launch_bytes(ld);
       vec_ms += ms[0] + (post_red ? ms[3] : 0.0f);
       vec_launches += post_red ? 2 : 1;
       for (const VecDesc& v : pre) vec_bytes += vec_desc_bytes(v, pk.mode);
@@ -1578,9 +1599,10 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
     P.defer_early = env ? std::atoi(env) != 0 : (double)n * n * 24.0 < 2e9;
   }
   if (P.precision == MT_PREC_BF16X2) {
-    P.ksplit = q->ksplit > 0 ? q->ksplit
-                             : (P.Ktab ? gemm_ksplit_table(P.nx, P.ny, P.nz)
-                                       : (P.sparse.blocks ? P.sparse.blocks->ksplit : gemm_ksplit_dense(n, n)));
+    // a block-sparse tensor's schedule fixes its partial-sum slots
+    P.ksplit = P.sparse.blocks ? P.sparse.blocks->ksplit
+                               : (q->ksplit > 0 ? q->ksplit
+                                                : (P.Ktab ? gemm_ksplit_table(P.nx, P.ny, P.nz) : gemm_ksplit_dense(n, n)));
     MT_HIP(hipMalloc(&P.packb, (size_t)packed_m_bytes(n)));
     MT_HIP(hipMemsetAsync(P.packb, 0, (size_t)packed_m_bytes(n), P.stream));
     // near-band correction of the split product (uniform grids: dense tensor,

@@ -458,7 +458,10 @@ int BlockSparse::schedule(const std::vector<int64_t>& go, int cus) {
   // indices are staged in LDS beside the kernel's 24 KiB)
   constexpr int64_t kMinRange = 32, kMaxRange = 16384;
   const int64_t T = std::min(std::max<int64_t>((entries + cus - 1) / cus, kMinRange), kMaxRange);
-  nwg = (int)((entries + T - 1) / T);
+  const int nwg_used = (int)((entries + T - 1) / T);
+  // a multiple of 8 (ranges of no segments at the end): the kernel gives XCD x
+  // the x-th eighth of the ranges
+  nwg = (int)round_up(nwg_used, 8);
   std::vector<std::vector<int4>> per(nwg);
   std::vector<int> pieces(ngroups, 0), last_w(ngroups, 0);
   max_list = 0;
@@ -471,7 +474,7 @@ int BlockSparse::schedule(const std::vector<int64_t>& go, int cus) {
       last_w[g] = w;
       e = e1;
     }
-    if (go[g + 1] == go[g]) last_w[g] = std::min<int>((int)(go[g] / T), nwg - 1);
+    if (go[g + 1] == go[g]) last_w[g] = std::min<int>((int)(go[g] / T), nwg_used - 1);
   }
   ksplit = 1;
   for (int g = 0; g < ngroups; ++g) ksplit = std::max(ksplit, pieces[g]);
