@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 MU0 = 4e-7 * np.pi
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0  # dense MFMA, MI355X
+FP32_PEAK_TFLOPS = 157.3  # FP32 matrix, MI355X
 
 
 def traffic_from_profile(kind):
@@ -405,6 +406,10 @@ def next_rows(W, reps=5):
     t0 = time.perf_counter()
     _lib.check(L.mt_demag_sparse_create_bf16x2(K32.data_ptr(), n, 4, 0.9, res[0], C.byref(hb)))
     t_build_b = time.perf_counter() - t0
+    hf = C.c_void_p()
+    t0 = time.perf_counter()
+    _lib.check(L.mt_demag_sparse_create_f32_tiles(K32.data_ptr(), n, 4, 0.9, res[0], C.byref(hf)))
+    t_build_f = time.perf_counter() - t0
     del K32
     torch.cuda.empty_cache()
     try:
@@ -442,9 +447,23 @@ def next_rows(W, reps=5):
         torch.cuda.synchronize()
         ms_apply_b = (time.perf_counter() - t0) / reps * 1e3
         rel_b = float(torch.linalg.norm(dhb - dh) / torch.linalg.norm(dh))
+        # the same kept entries as FP32 tiles (FP32 MFMAs): the FP32 product for many states
+        entf = C.c_longlong()
+        ksf = C.c_int()
+        _lib.check(L.mt_demag_sparse_blocks(hf, C.byref(entf), None, None, C.byref(ksf)))
+        dhf = torch.empty_like(dh)
+        _lib.check(L.mt_demag_apply_sparse(hf, dm.data_ptr(), dhf.data_ptr(), S, -1.0))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.check(L.mt_demag_apply_sparse(hf, dm.data_ptr(), dhf.data_ptr(), S, -1.0))
+        torch.cuda.synchronize()
+        ms_apply_f = (time.perf_counter() - t0) / reps * 1e3
+        rel_f = float(torch.linalg.norm(dhf - dh) / torch.linalg.norm(dh))
     finally:
         L.mt_demag_sparse_destroy(h)
         L.mt_demag_sparse_destroy(hb)
+        L.mt_demag_sparse_destroy(hf)
     stored = float(sum(nnz))
     alg = stored * 8.0 + 2.0 * S * 3 * n * 4  # value + column per stored entry, states in, fields out
     out["sparse_demag"] = {"workload": f"{res[0]}x{res[1]}x{res[2]} film, N = {n}, {S} states, threshold fraction 0.9",
@@ -467,6 +486,18 @@ def next_rows(W, reps=5):
                      "frac": alg_b / (ms_apply_b * 1e-3) / 1e9 / HBM_PEAK_GBS, "alg_bytes_per_launch": alg_b},
         "note": "wall time per synchronous mt_demag_apply_sparse call (state packing, GEMM over the stored tiles, "
                 "partial-sum combine, workspace allocation)"}
+    # FP32 work over the stored tiles: 9 block products x 2 flop x 32 rows x 16 sources per tile and state
+    flops_f = 18.0 * 4 * entf.value * 32 * 16 * S
+    tf_f = flops_f / (ms_apply_f * 1e-3) / 1e12
+    out["sparse_demag_f32_tiles"] = {
+        "workload": out["sparse_demag"]["workload"] + ", kept entries as FP32 32x16 tiles (FP32 MFMAs)",
+        "build_s": t_build_f, "list_entries": int(entf.value), "ksplit": int(ksf.value), "ms_per_product": ms_apply_f,
+        "speedup_vs_csr": ms_apply / ms_apply_f, "rel_diff_vs_fp32_csr": rel_f,
+        "roofline": {"bound": "mfma", "kernel": "gemm_bf16x2_reg_kernel<4, false, true, true> (FP32 fragments)",
+                     "achieved": tf_f, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf_f / FP32_PEAK_TFLOPS,
+                     "alg_flops_per_launch": flops_f},
+        "note": "wall time per synchronous mt_demag_apply_sparse call; FP32 products with FP32 accumulation "
+                "(the reference's arithmetic class), one pass over the stored tiles for all states"}
     # magnetostatics: prism lattice field
     c, abc = lattice_prisms()
     nt = c.shape[0]

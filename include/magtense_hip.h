@@ -200,15 +200,23 @@ int mt_demag_sparse_info(void* sp, long long* nnz6, float* threshold);
  * MFMA GEMM over the stored tiles.  No reference counterpart (an additive
  * precision mode of the sparse branch, LandauLifshitzEquationSolver.f90:581-624). */
 int mt_demag_sparse_create_bf16x2(const float* K32, int n, int approx, double threshold, int grid_nx, void** out);
-/* Block statistics of a tensor from mt_demag_sparse_create_bf16x2: list
+/* The same block-sparse tiles with FP32 values (the reference's FP32
+ * arithmetic): the product runs v_mfma_f32_32x32x2f32 over the stored tiles
+ * with FP32 accumulation, for many states at once (the solver takes this form
+ * for the threshold modes with precision fp32 and eight or more states).  No
+ * reference counterpart (the reference multiplies one state at a time by the
+ * CSR matrix, MagTenseCudaBlas.cu:106-297, LandauLifshitzEquationSolver.f90:581-624). */
+int mt_demag_sparse_create_f32_tiles(const float* K32, int n, int approx, double threshold, int grid_nx, void** out);
+/* Block statistics of a tensor from mt_demag_sparse_create_bf16x2 or
+ * mt_demag_sparse_create_f32_tiles: list
  * entries (each 4 x 12 KiB of fragments), non-empty tiles, fragment bytes, and
  * the partial-sum slots of its balanced workgroup schedule (ksplit). */
 int mt_demag_sparse_blocks(void* sp, long long* entries, long long* tiles_nonempty, long long* bytes, int* ksplit);
 /* h = pref * K_sparse m for nstates states, [s][3n] device layout (the
  * product of updateDemagfield's sparse branch, LandauLifshitzEquationSolver.f90:581-624;
- * FP32 CSR, or split-BF16 tiles for a tensor of mt_demag_sparse_create_bf16x2,
- * whose workspace the handle allocates on first use and keeps: calls on one
- * handle must not overlap). */
+ * FP32 CSR, or the tiles of mt_demag_sparse_create_bf16x2 /
+ * mt_demag_sparse_create_f32_tiles, whose workspace the handle allocates on
+ * first use and keeps: calls on one handle must not overlap). */
 int mt_demag_apply_sparse(void* sp, const float* m, float* h, int nstates, float pref);
 void mt_demag_sparse_destroy(void* sp);
 
@@ -316,6 +324,11 @@ int mt_solver_set_timing(mt_solver* s, int on);
 /* Sparse demag of a solver created with demag_approx 2 or 4: nnz per component
  * and the FP32 threshold used (an argument error when the solver is dense). */
 int mt_solver_demag_info(mt_solver* s, long long* nnz6, float* threshold);
+/* The form the solver holds its demag tensor in: 0 dense (FP32 or split-BF16,
+ * or the translation-invariant table), 1 threshold CSR, 2 threshold tiles in
+ * split-BF16, 3 threshold tiles in FP32 (precision fp32 with eight or more
+ * states).  No reference counterpart (the reference's sparse path is CSR). */
+int mt_solver_demag_form(mt_solver* s, int* form);
 void mt_solver_destroy(mt_solver* s);
 int mt_run_micromag(const mt_micromag_problem* problem, double* t_out, double* y_out, double* fields,
                     mt_stats* stats);

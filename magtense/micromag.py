@@ -381,7 +381,10 @@ class MicromagProblem:
                 nnz = (C.c_longlong * 6)()
                 thr = C.c_float()
                 _lib.check(L.mt_solver_demag_info(h, nnz, C.byref(thr)))
-                self.last_demag_info = dict(nnz=[int(v) for v in nnz], threshold=float(thr.value))
+                form = C.c_int()
+                _lib.check(L.mt_solver_demag_form(h, C.byref(form)))
+                self.last_demag_info = dict(nnz=[int(v) for v in nnz], threshold=float(thr.value),
+                                            form={1: "csr", 2: "bf16x2 tiles", 3: "fp32 tiles"}[form.value])
         finally:
             L.mt_solver_destroy(h)
         self.last_stats = [dict(f_calls=s.f_calls, steps=s.steps, bad_steps=s.bad_steps, flag3=s.flag3,

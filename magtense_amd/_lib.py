@@ -86,8 +86,8 @@ EXPORTS = [
     "mt_solver_create", "mt_solver_advance", "mt_solver_run", "mt_solver_advance_total", "mt_solver_steps_total", "mt_solver_results", "mt_solver_state",
     "mt_solver_set_timing", "mt_solver_destroy", "mt_run_micromag", "mt_rk4_create", "mt_rk4_forward",
     "mt_rk4_backward", "mt_rk4_destroy", "mt_tiles_demag_tensor", "mt_tiles_h_field", "mt_demag_sparse_create",
-    "mt_demag_sparse_info", "mt_demag_apply_sparse", "mt_demag_sparse_destroy", "mt_solver_demag_info",
-    "mt_demag_sparse_create_bf16x2", "mt_demag_sparse_blocks", "mt_demag_near_correction",
+    "mt_demag_sparse_info", "mt_demag_apply_sparse", "mt_demag_sparse_destroy", "mt_solver_demag_info", "mt_solver_demag_form",
+    "mt_demag_sparse_create_bf16x2", "mt_demag_sparse_create_f32_tiles", "mt_demag_sparse_blocks", "mt_demag_near_correction",
     "mt_noalloc_scratch_bytes", "mt_demag_apply_bf16x2_noalloc_bytes", "mt_demag_apply_bf16x2_noalloc",
     "mt_toeplitz_tensor_bytes", "mt_toeplitz_tensor", "mt_demag_apply_bf16x2_toeplitz",
     "mt_split_tensor_row_offset", "mt_demag_apply_bf16x2_rows",
@@ -168,11 +168,13 @@ def load():
     lib.mt_demag_sparse_create.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.POINTER(vp)]
     lib.mt_demag_sparse_info.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_float)]
     lib.mt_demag_sparse_create_bf16x2.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
+    lib.mt_demag_sparse_create_f32_tiles.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
     lib.mt_demag_sparse_blocks.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 3 + [C.POINTER(C.c_int)]
     lib.mt_demag_apply_sparse.argtypes = [vp, vp, vp, C.c_int, C.c_float]
     lib.mt_demag_sparse_destroy.argtypes = [vp]
     lib.mt_demag_sparse_destroy.restype = None
     lib.mt_solver_demag_info.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_float)]
+    lib.mt_solver_demag_form.argtypes = [vp, C.POINTER(C.c_int)]
     lib.icl_initDemagMatrices_sparse.argtypes = [C.POINTER(C.c_int)] * 3 + [vp, vp, vp]
     lib.icl_initDemagMatrices_sparse.restype = None
     lib.icl_MVMult_GetH_sparse.argtypes = [vp] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_float)]

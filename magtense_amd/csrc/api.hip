@@ -46,6 +46,17 @@ __global__ void pack_bf16_kernel(const float* __restrict__ m, int n, int nstates
   p[kFragBytes / 2] = lo;
 }
 
+// FP32 states [s][3n] -> FP32 B-fragments of the block-sparse FP32 product
+__global__ void pack_f32_frag_kernel(const float* __restrict__ m, int n, int nstates, uint8_t* __restrict__ packed) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)nstates * 3 * n;
+  if (t >= total) return;
+  int s = (int)(t / (3 * (int64_t)n));
+  int e = (int)(t % (3 * (int64_t)n));
+  int c = e / n, j = e % n;
+  *reinterpret_cast<float*>(packed + f32_frag_offset(s, c, j)) = m[t];
+}
+
 // h[t] = pref * sum_k part[k*stride + t]
 __global__ void combine_partials_kernel(const float* __restrict__ part, int ksplit, int64_t stride, int64_t total,
                                         float pref, float* __restrict__ h) {
@@ -288,7 +299,7 @@ Bf16x2Work bf16x2_work(int n, int nrows, int nx, int ny, int nz, int nstates, in
 // nrows < n only for a row block of the dense tensor (h holds 3 nrows per state).
 // `ws` holds w.bytes() of device memory.
 int apply_bf16x2_in(const void* K, int n, int nrows, int nx, int ny, int nz, const float* m, float* h, int nstates,
-                    float pref, const Bf16x2Work& w, uint8_t* ws, const BlockSparseView* bsv) {
+                    float pref, const Bf16x2Work& w, uint8_t* ws, const BlockSparseView* bsv, bool f32) {
   int rc = kOk;
   const int64_t n3 = (int64_t)3 * n;
   const int64_t r3 = (int64_t)3 * nrows;
@@ -302,11 +313,15 @@ int apply_bf16x2_in(const void* K, int n, int nrows, int nx, int ny, int nz, con
                        nullptr, reinterpret_cast<uint4*>(packed), n16);
     MT_HIP(hipGetLastError());
     int64_t tot = (int64_t)S * n3;
-    hipLaunchKernelGGL(pack_bf16_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr, m + s0 * n3, n,
-                       S, packed);
+    if (f32)
+      hipLaunchKernelGGL(pack_f32_frag_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr,
+                         m + s0 * n3, n, S, packed);
+    else
+      hipLaunchKernelGGL(pack_bf16_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr, m + s0 * n3,
+                         n, S, packed);
     MT_HIP(hipGetLastError());
     if (bsv)
-      rc = launch_gemm_bf16x2_bsr((const uint8_t*)K, *bsv, packed, n, part, n3, (int64_t)cols * n3, S, ks, nullptr);
+      rc = launch_gemm_bsr((const uint8_t*)K, *bsv, f32, packed, n, part, n3, (int64_t)cols * n3, S, ks, nullptr);
     else if (nx > 0)
       rc = launch_gemm_bf16x2_toeplitz((const uint8_t*)K, nx, ny, nz, packed, part, n3, (int64_t)cols * n3, S, ks,
                                        nullptr);
@@ -508,7 +523,10 @@ int mt_demag_sparse_create(const float* K32, int n, int approx, double threshold
   return kOk;
 }
 
-int mt_demag_sparse_create_bf16x2(const float* K32, int n, int approx, double threshold, int grid_nx, void** out) {
+}  // extern "C"
+
+namespace {
+int sparse_create_tiles(const float* K32, int n, int approx, double threshold, int grid_nx, bool f32, void** out) {
   int rc = device_ok();
   if (rc) return rc;
   MT_CHECK(out && K32 && n > 0 && grid_nx >= 0, "invalid sparse tensor arguments");
@@ -522,13 +540,24 @@ int mt_demag_sparse_create_bf16x2(const float* K32, int n, int approx, double th
   }
   std::unique_ptr<SparseDemag> sp(new SparseDemag());
   sp->blocks = new BlockSparse();
-  rc = sp->blocks->build(K32, n, thr, grid_nx, nullptr);
+  rc = sp->blocks->build(K32, n, thr, grid_nx, nullptr, f32);
   if (rc) return rc;
   sp->n = n;
   sp->threshold = thr;
   for (int c = 0; c < 6; ++c) sp->nnz[c] = sp->blocks->nnz[c];
   *out = sp.release();
   return kOk;
+}
+}  // namespace
+
+extern "C" {
+
+int mt_demag_sparse_create_bf16x2(const float* K32, int n, int approx, double threshold, int grid_nx, void** out) {
+  return sparse_create_tiles(K32, n, approx, threshold, grid_nx, false, out);
+}
+
+int mt_demag_sparse_create_f32_tiles(const float* K32, int n, int approx, double threshold, int grid_nx, void** out) {
+  return sparse_create_tiles(K32, n, approx, threshold, grid_nx, true, out);
 }
 
 int mt_demag_sparse_blocks(void* h, long long* entries, long long* tiles_nonempty, long long* bytes, int* ksplit) {
@@ -569,7 +598,7 @@ int mt_demag_apply_sparse(void* h, const float* m, float* hout, int nstates, flo
       sp->ws_bytes = w.bytes();
     }
     const BlockSparseView v = b.view();
-    return apply_bf16x2_in(b.frags, b.n, b.n, 0, 0, 0, m, hout, nstates, pref, w, sp->ws, &v);
+    return apply_bf16x2_in(b.frags, b.n, b.n, 0, 0, 0, m, hout, nstates, pref, w, sp->ws, &v, b.f32);
   }
   const int64_t n3 = (int64_t)3 * sp->n;
   for (int s0 = 0; s0 < nstates; s0 += kMaxCols) {

@@ -308,7 +308,13 @@ constexpr int kTableChainChunks = 256;  // longest partial sum of the entry-reus
 // Workgroups follow the balanced schedule: workgroup b runs the segments
 // bsv.segs[bsv.wseg[b] .. bsv.wseg[b+1]), each a run of one group's list
 // whose sums go to that segment's partial-sum slot.
-template <int D, bool TOEP, bool BSR = false>
+// F32: the fragments hold FP32 values instead of BF16 hi/lo pairs, in the
+// same 12 KiB sets: component c's two 1 KiB fragments give lane l the eight
+// k-steps s = 0..7 of v_mfma_f32_32x32x2f32 (s = 4p + q: fragment 2c + p,
+// float q), A = K_c(row l % 32, k = 2s + l / 32) and, for the states, B =
+// m_beta(k = 2s + l / 32, column l % 32 of the half) — eight FP32 MFMAs per
+// block instead of three split products.
+template <int D, bool TOEP, bool BSR = false, bool F32 = false>
 __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __restrict__ Kt,
                                                                  const uint4* __restrict__ Mp, int64_t np,
                                                                  int ksplit, float* __restrict__ out, int n,
@@ -320,6 +326,7 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
                                                                  const int64_t* __restrict__ bgoff,
                                                                  const int* __restrict__ btiles) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
   __shared__ u32x4 bs[2][12][64];
   extern __shared__ int sidx[];  // BSR: the current segment's source chunks (bsv.seg_max)
   const int lane = threadIdx.x & 63;
@@ -442,22 +449,44 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
     _Pragma("unroll") for (int q = 0; q < 3; ++q) dst_[q * 256] = br[slot][q];                          \
     __syncthreads();                                                                                     \
     if (tile_ok) {                                                                                       \
-      bf16x8 bh[3][2], bl[3][2];                                                                         \
-      _Pragma("unroll") for (int beta = 0; beta < 3; ++beta)                                             \
-      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                                    \
-        bh[beta][h] = __builtin_bit_cast(bf16x8, bs[(j) & 1][(beta * 2 + h) * 2 + 0][lane]);             \
-        bl[beta][h] = __builtin_bit_cast(bf16x8, bs[(j) & 1][(beta * 2 + h) * 2 + 1][lane]);             \
-      }                                                                                                  \
-      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                                    \
-        if (h == 1 && !two_halves) break;                                                                \
-        _Pragma("unroll") for (int a = 0; a < 3; ++a)                                                    \
-        _Pragma("unroll") for (int beta = 0; beta < 3; ++beta) {                                         \
-          const int c = (a == 0 ? beta : (a == 1 ? (beta == 0 ? 1 : 2 + beta) : (beta == 0 ? 2 : 3 + beta))); \
-          const bf16x8 khi = __builtin_bit_cast(bf16x8, ar[slot][2 * c]);                                \
-          const bf16x8 klo = __builtin_bit_cast(bf16x8, ar[slot][2 * c + 1]);                            \
-          acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(khi, bh[beta][h], acc[h][a], 0, 0, 0);     \
-          acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(khi, bl[beta][h], acc[h][a], 0, 0, 0);     \
-          acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(klo, bh[beta][h], acc[h][a], 0, 0, 0);     \
+      if (F32) {                                                                                         \
+        f32x4 b0[3][2], b1[3][2];                                                                        \
+        _Pragma("unroll") for (int beta = 0; beta < 3; ++beta)                                           \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                                  \
+          b0[beta][h] = __builtin_bit_cast(f32x4, bs[(j) & 1][(beta * 2 + h) * 2 + 0][lane]);            \
+          b1[beta][h] = __builtin_bit_cast(f32x4, bs[(j) & 1][(beta * 2 + h) * 2 + 1][lane]);            \
+        }                                                                                                \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                                  \
+          if (h == 1 && !two_halves) break;                                                              \
+          _Pragma("unroll") for (int a = 0; a < 3; ++a)                                                  \
+          _Pragma("unroll") for (int beta = 0; beta < 3; ++beta) {                                       \
+            const int c = (a == 0 ? beta : (a == 1 ? (beta == 0 ? 1 : 2 + beta) : (beta == 0 ? 2 : 3 + beta))); \
+            const f32x4 k0 = __builtin_bit_cast(f32x4, ar[slot][2 * c]);                                 \
+            const f32x4 k1 = __builtin_bit_cast(f32x4, ar[slot][2 * c + 1]);                             \
+            _Pragma("unroll") for (int q = 0; q < 4; ++q)                                                \
+              acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(k0[q], b0[beta][h][q], acc[h][a], 0, 0, 0); \
+            _Pragma("unroll") for (int q = 0; q < 4; ++q)                                                \
+              acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x2f32(k1[q], b1[beta][h][q], acc[h][a], 0, 0, 0); \
+          }                                                                                              \
+        }                                                                                                \
+      } else {                                                                                           \
+        bf16x8 bh[3][2], bl[3][2];                                                                       \
+        _Pragma("unroll") for (int beta = 0; beta < 3; ++beta)                                           \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                                  \
+          bh[beta][h] = __builtin_bit_cast(bf16x8, bs[(j) & 1][(beta * 2 + h) * 2 + 0][lane]);           \
+          bl[beta][h] = __builtin_bit_cast(bf16x8, bs[(j) & 1][(beta * 2 + h) * 2 + 1][lane]);           \
+        }                                                                                                \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                                  \
+          if (h == 1 && !two_halves) break;                                                              \
+          _Pragma("unroll") for (int a = 0; a < 3; ++a)                                                  \
+          _Pragma("unroll") for (int beta = 0; beta < 3; ++beta) {                                       \
+            const int c = (a == 0 ? beta : (a == 1 ? (beta == 0 ? 1 : 2 + beta) : (beta == 0 ? 2 : 3 + beta))); \
+            const bf16x8 khi = __builtin_bit_cast(bf16x8, ar[slot][2 * c]);                              \
+            const bf16x8 klo = __builtin_bit_cast(bf16x8, ar[slot][2 * c + 1]);                          \
+            acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(khi, bh[beta][h], acc[h][a], 0, 0, 0);   \
+            acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(khi, bl[beta][h], acc[h][a], 0, 0, 0);   \
+            acc[h][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(klo, bh[beta][h], acc[h][a], 0, 0, 0);   \
+          }                                                                                              \
         }                                                                                                \
       }                                                                                                  \
     }                                                                                                    \
@@ -512,7 +541,7 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
   }  // segments
 }
 
-template <int D, bool TOEP = false, bool BSR = false>
+template <int D, bool TOEP = false, bool BSR = false, bool F32 = false>
 int launch_reg(const uint8_t* K, const uint8_t* Mpack, int n, int nrows, float* out, int64_t col_stride,
                int64_t split_stride, int ncols, int ksplit, hipStream_t stream, int gnx = 0, int gny = 1,
                int gnz = 1, BlockSparseView bs = BlockSparseView{}) {
@@ -523,7 +552,7 @@ int launch_reg(const uint8_t* K, const uint8_t* Mpack, int n, int nrows, float* 
   if (BSR) blocks = bs.nwg;  // the balanced schedule's ranges
   if (blocks == 0) return kOk;
   const size_t lds = BSR ? (size_t)bs.seg_max * sizeof(int) : 0;
-  hipLaunchKernelGGL((gemm_bf16x2_reg_kernel<D, TOEP, BSR>), dim3((unsigned)blocks), dim3(256), lds, stream,
+  hipLaunchKernelGGL((gemm_bf16x2_reg_kernel<D, TOEP, BSR, F32>), dim3((unsigned)blocks), dim3(256), lds, stream,
                      reinterpret_cast<const uint4*>(K), reinterpret_cast<const uint4*>(Mpack), np, ksplit, out, nrows,
                      col_stride, split_stride, ncols, gnx, gny, gnz, bs, bs.chunks, bs.segs, bs.goff, bs.tiles);
   MT_HIP(hipGetLastError());
@@ -1119,8 +1148,8 @@ int launch_near_correction(const uint32_t* table, int R, int nx, int ny, int nz,
   return kOk;
 }
 
-int launch_gemm_bf16x2_bsr(const uint8_t* frags, const BlockSparseView& bs, const uint8_t* Mpack, int n, float* out,
-                           int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+int launch_gemm_bsr(const uint8_t* frags, const BlockSparseView& bs, bool f32, const uint8_t* Mpack, int n, float* out,
+                    int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
   if (bs.ngroups == 0) {  // no kept entry anywhere: the partial sums are zero
     for (int k = 0; k < ksplit; ++k)
       for (int c = 0; c < ncols; ++c)
@@ -1128,6 +1157,9 @@ int launch_gemm_bf16x2_bsr(const uint8_t* frags, const BlockSparseView& bs, cons
                               stream));
     return kOk;
   }
+  if (f32)
+    return launch_reg<4, false, true, true>(frags, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream, 0,
+                                            1, 1, bs);
   return launch_reg<4, false, true>(frags, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream, 0, 1, 1,
                                     bs);
 }

@@ -256,6 +256,13 @@ __global__ void __launch_bounds__(256) vec_kernel(const VecDesc* __restrict__ de
 #pragma unroll
       for (int q = 0; q < kVecE; ++q)
         if (ok[q]) pack.f32[(int64_t)D.pack_col * pack.col_stride + e0 + 256 * q] = (float)v[q];
+    } else if (pack.mode == kPackF32Frag) {
+#pragma unroll
+      for (int q = 0; q < kVecE; ++q)
+        if (ok[q]) {
+          const int e = e0 + 256 * q, c = e / n, j = e - c * n;
+          *reinterpret_cast<float*>(pack.bf16 + f32_frag_offset(D.pack_col, c, j)) = (float)v[q];
+        }
     } else {
 #pragma unroll
       for (int q = 0; q < kVecE; ++q) tile[threadIdx.x + 256 * q] = (float)v[q];

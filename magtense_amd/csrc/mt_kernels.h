@@ -70,9 +70,11 @@ int launch_gemm_bf16x2_rows(const uint8_t* Krows, const uint8_t* Mpack, int n, i
 // an nx x ny x nz uniform grid (n = nx ny nz, nx % 32 == 0).
 int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, const uint8_t* Mpack, float* out,
                                 int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
-// Same product over a block-sparse tensor (fragment sets `frags`, structure `bs`).
-int launch_gemm_bf16x2_bsr(const uint8_t* frags, const BlockSparseView& bs, const uint8_t* Mpack, int n, float* out,
-                           int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
+// Same product over a block-sparse tensor (fragment sets `frags`, structure
+// `bs`): split-BF16 fragments and states, or with f32 FP32 fragments and
+// states (the FP32 layouts of pack_f32_frag) on FP32 MFMAs.
+int launch_gemm_bsr(const uint8_t* frags, const BlockSparseView& bs, bool f32, const uint8_t* Mpack, int n, float* out,
+                    int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
 
 struct BlockSparse {
   int n = 0, ngroups = 0;
@@ -81,6 +83,7 @@ struct BlockSparse {
   int seg_max = 0;                // longest segment (list entries)
   int max_list = 0;               // longest group list (entries)
   float threshold = 0.0f;
+  bool f32 = false;               // FP32 fragments (FP32 MFMAs) instead of split-BF16 pairs
   int64_t entries = 0;            // list entries over all groups (each 4 x 12 KiB of fragments)
   int64_t tiles_nonempty = 0;     // (row tile, chunk) tiles holding a kept entry
   int64_t nnz[6] = {0, 0, 0, 0, 0, 0};  // kept entries per component
@@ -93,7 +96,7 @@ struct BlockSparse {
   uint8_t* frags = nullptr;
   // grid_nx > 0 with grid_nx % 32 == 0: uniform-grid rows of grid_nx cells, so
   // row tiles of consecutive grid rows can share a group
-  int build(const float* K32, int n, float thr, int grid_nx, hipStream_t stream);
+  int build(const float* K32, int n, float thr, int grid_nx, hipStream_t stream, bool fp32 = false);
   int schedule(const std::vector<int64_t>& goff_host, int cus);
   BlockSparseView view() const { return BlockSparseView{ngroups, tiles, goff, chunks, nwg, seg_max, wseg, segs}; }
   int64_t bytes() const { return entries * 4 * 12 * 1024; }
@@ -117,9 +120,10 @@ Bf16x2Work bf16x2_work(int n, int nrows, int nx, int ny, int nz, int nstates, in
 // the null stream, with the caller's workspace `ws` of w.bytes(); synchronous.
 // nx > 0: K is the translation-invariant table of an nx x ny x nz grid; nrows <
 // n: K is a row block of the dense tensor; bsv: K holds the fragments of a
-// block-sparse tensor.
+// block-sparse tensor (f32: its FP32 form, states packed as FP32 fragments).
 int apply_bf16x2_in(const void* K, int n, int nrows, int nx, int ny, int nz, const float* m, float* h, int nstates,
-                    float pref, const Bf16x2Work& w, uint8_t* ws, const BlockSparseView* bsv = nullptr);
+                    float pref, const Bf16x2Work& w, uint8_t* ws, const BlockSparseView* bsv = nullptr,
+                    bool f32 = false);
 int device_ok();
 
 struct SparseDemag {
@@ -295,13 +299,24 @@ struct VecDesc {
   int rslot;                 // the reduction's four results go to results[4*rslot .. 4*rslot+3]
 };
 
-enum PackMode { kPackNone = 0, kPackF32 = 1, kPackBf16 = 2 };
+enum PackMode { kPackNone = 0, kPackF32 = 1, kPackBf16 = 2, kPackF32Frag = 3 };
+
+// Byte offset of state column col's component c at source j in the FP32
+// B-fragment layout of the block-sparse FP32 product (same 12 KiB per 16-source
+// chunk as the split-BF16 states): fragment ((J*3 + c)*2 + col/32)*2 + s/4,
+// lane (col % 32) + 32 (k % 2), float s % 4, with k = j % 16, s = k / 2.
+__host__ __device__ inline int64_t f32_frag_offset(int col, int c, int j) {
+  const int J = j / kTileK, k = j % kTileK, st = k / 2;
+  const int lane = (col & 31) + 32 * (k & 1);
+  const int64_t frag = (((int64_t)J * 3 + c) * 2 + col / 32) * 2 + st / 4;
+  return frag * kFragBytes + (int64_t)lane * 16 + (st % 4) * 4;
+}
 
 struct PackSpec {
   int mode;
   float* f32;                // [col][3n]
   int64_t col_stride;
-  uint8_t* bf16;             // fragment layout
+  uint8_t* bf16;             // fragment layout (split-BF16, or FP32 with kPackF32Frag)
 };
 
 int vec_blocks_per_state(int n3);

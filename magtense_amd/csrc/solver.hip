@@ -1140,7 +1140,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       L.fields = s.op.fields;
       L.hpart = hbuf + (size_t)k * N3;
       L.split_stride = (int64_t)hcols * N3;
-      L.nsplit = precision == MT_PREC_BF16X2 ? ksplit + (near_r > 0 ? 1 : 0) : 1;
+      L.nsplit = precision == MT_PREC_BF16X2 ? ksplit + (near_r > 0 ? 1 : 0) : (sparse.blocks ? ksplit : 1);
       hext_at(s, s.op.t, L.hext);
       L.alpha = alpha_at(s.op.t);
       L.gamma = gamma;
@@ -1148,6 +1148,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     }
     PackSpec pk;
     if (precision == MT_PREC_BF16X2) pk = PackSpec{kPackBf16, nullptr, 0, packb};
+    else if (sparse.blocks) pk = PackSpec{kPackF32Frag, nullptr, 0, packb};
     else pk = PackSpec{kPackF32, packf, (int64_t)N3, nullptr};
     if (timing) MT_HIP(hipEventRecord(ev[2], stream));
     // Small groups pass their descriptors by value; otherwise the group's
@@ -1179,7 +1180,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
         rc = launch_gemm_bf16x2_toeplitz(Ktab, nx, ny, nz, packb, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit,
                                          stream);
       else if (sparse.blocks)
-        rc = launch_gemm_bf16x2_bsr(sparse.blocks->frags, sparse.blocks->view(), packb, n, hbuf, (int64_t)N3,
+        rc = launch_gemm_bsr(sparse.blocks->frags, sparse.blocks->view(), false, packb, n, hbuf, (int64_t)N3,
                                     (int64_t)hcols * N3, S, ksplit, stream);
       else
         rc = launch_gemm_bf16x2(Ksplit, packb, n, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit, stream);
@@ -1188,6 +1189,9 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       if (rc == kOk && near_r > 0)
         rc = launch_near_correction(near_tab, near_r, nx, ny, nz, dl, nullptr, 0, S,
                                     hbuf + (size_t)ksplit * hcols * N3, (int64_t)N3, 1.0f, false, stream);
+    } else if (sparse.blocks) {
+      rc = launch_gemm_bsr(sparse.blocks->frags, sparse.blocks->view(), true, packb, n, hbuf, (int64_t)N3,
+                           (int64_t)hcols * N3, S, ksplit, stream);
     } else if (demag_approx != MT_DEMAG_DENSE) {
       rc = sparse.apply(packf, (int64_t)N3, S, 1.0f, hbuf, stream);
     } else {
@@ -1518,9 +1522,15 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       if (rc) return rc;
     }
     int rc = kOk;
-    if (P.precision == MT_PREC_BF16X2) {
+    // FP32 with many states: the kept entries as FP32 tiles on FP32 MFMAs
+    // (one pass over the tiles for all states instead of the CSR gathers);
+    // MT_F32_TILE_STATES sets the smallest batch that takes them (default 8)
+    const char* env_ft = std::getenv("MT_F32_TILE_STATES");
+    const int f32_tile_states = env_ft ? std::atoi(env_ft) : 8;
+    const bool f32_tiles = P.precision != MT_PREC_BF16X2 && f32_tile_states > 0 && P.B >= f32_tile_states;
+    if (P.precision == MT_PREC_BF16X2 || f32_tiles) {
       P.sparse.blocks = new BlockSparse();
-      rc = P.sparse.blocks->build(P.K32, n, thr, unstructured ? 0 : P.nx, P.stream);
+      rc = P.sparse.blocks->build(P.K32, n, thr, unstructured ? 0 : P.nx, P.stream, f32_tiles);
       P.sparse.n = n;
       P.sparse.threshold = thr;
       for (int c = 0; c < 6; ++c) P.sparse.nnz[c] = P.sparse.blocks->nnz[c];
@@ -1643,6 +1653,12 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       }
     }
     MT_HIP(hipMalloc(&P.hbuf, (size_t)(P.ksplit + (P.near_r > 0 ? 1 : 0)) * P.hcols * n3 * sizeof(float)));
+  } else if (P.sparse.blocks) {
+    // FP32 tiles: FP32 state fragments, the schedule's partial-sum slots
+    P.ksplit = P.sparse.blocks->ksplit;
+    MT_HIP(hipMalloc(&P.packb, (size_t)packed_m_bytes(n)));
+    MT_HIP(hipMemsetAsync(P.packb, 0, (size_t)packed_m_bytes(n), P.stream));
+    MT_HIP(hipMalloc(&P.hbuf, (size_t)P.ksplit * P.hcols * n3 * sizeof(float)));
   } else {
     P.ksplit = 1;
     P.hcols = P.B;
@@ -1791,6 +1807,15 @@ int mt_solver_demag_info(mt_solver* h, long long* nnz6, float* threshold) {
   for (int c = 0; c < 6; ++c)
     if (nnz6) nnz6[c] = P.sparse.nnz[c];
   if (threshold) *threshold = P.sparse.threshold;
+  return mt::kOk;
+}
+
+int mt_solver_demag_form(mt_solver* h, int* form) {
+  MT_CHECK(h && form, "null solver or output");
+  mt::Solver& P = *reinterpret_cast<mt::Solver*>(h);
+  if (P.demag_approx == MT_DEMAG_DENSE) *form = 0;
+  else if (!P.sparse.blocks) *form = 1;
+  else *form = P.sparse.blocks->f32 ? 3 : 2;
   return mt::kOk;
 }
 

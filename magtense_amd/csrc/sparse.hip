@@ -311,6 +311,36 @@ __global__ void __launch_bounds__(256) bsr_fill_kernel(const float* __restrict__
   }
 }
 
+// The same list entries as FP32 fragments for v_mfma_f32_32x32x2f32: lane l
+// of component c's fragments 2c + p holds K_c(row 32I + l % 32, k = 16J +
+// 2s + l / 32) for the steps s = 4p .. 4p + 3 (entries at or below the
+// threshold zero).
+__global__ void __launch_bounds__(256) bsr_fill_f32_kernel(const float* __restrict__ K, int n, float thr,
+                                                           BlockSparseView bs, const int* __restrict__ egroup,
+                                                           uint8_t* __restrict__ frags) {
+  const int64_t e = blockIdx.x;
+  const int g = egroup[e];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int I = bs.tiles[4 * g + w];
+  if (I < 0) return;
+  const int64_t cnt = bs.goff[g + 1] - bs.goff[g];
+  const int64_t set = 4 * bs.goff[g] + w * cnt + (e - bs.goff[g]);
+  const int64_t i = (int64_t)I * kTileM + (lane % 32);
+  const int64_t j0 = (int64_t)bs.chunks[e] * kTileK + (lane / 32);
+  const size_t plane = (size_t)n * n;
+  for (int c = 0; c < 6; ++c)
+    for (int p = 0; p < 2; ++p) {
+      float v[4];
+      for (int q = 0; q < 4; ++q) {
+        const int64_t j = j0 + 2 * (4 * p + q);
+        float x = (i < n && j < n) ? K[c * plane + (size_t)i * n + j] : 0.0f;
+        v[q] = fabsf(x) > thr ? x : 0.0f;
+      }
+      *reinterpret_cast<float4*>(frags + ((set * 12 + c * 2 + p) * kFragBytes) + (int64_t)lane * 16) =
+          make_float4(v[0], v[1], v[2], v[3]);
+    }
+}
+
 }  // namespace
 
 // ---- host side -------------------------------------------------------------------
@@ -333,8 +363,9 @@ void BlockSparse::release() {
 // the same x in four consecutive grid rows, whichever needs fewer list
 // entries: neighbouring rows keep nearly the same sources), and writes the
 // fragments of every list entry.
-int BlockSparse::build(const float* K32, int n_, float thr, int grid_nx, hipStream_t st) {
+int BlockSparse::build(const float* K32, int n_, float thr, int grid_nx, hipStream_t st, bool fp32) {
   release();
+  f32 = fp32;
   n = n_;
   threshold = thr;
   const int64_t np = round_up(n, kTileM);
@@ -428,7 +459,12 @@ int BlockSparse::build(const float* K32, int n_, float thr, int grid_nx, hipStre
     MT_HIP(hipMemcpyAsync(chunks, ch.data(), ch.size() * sizeof(int), hipMemcpyHostToDevice, st));
     MT_HIP(hipMemcpyAsync(egroup, eg.data(), eg.size() * sizeof(int), hipMemcpyHostToDevice, st));
     MT_HIP(hipMalloc(&frags, (size_t)bytes()));
-    hipLaunchKernelGGL(bsr_fill_kernel, dim3((unsigned)entries), dim3(256), 0, st, K32, n, thr, view(), egroup, frags);
+    if (f32)
+      hipLaunchKernelGGL(bsr_fill_f32_kernel, dim3((unsigned)entries), dim3(256), 0, st, K32, n, thr, view(), egroup,
+                         frags);
+    else
+      hipLaunchKernelGGL(bsr_fill_kernel, dim3((unsigned)entries), dim3(256), 0, st, K32, n, thr, view(), egroup,
+                         frags);
     MT_HIP(hipGetLastError());
   } else {
     ngroups = 0;  // nothing kept: the product is zero

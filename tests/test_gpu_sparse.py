@@ -7,11 +7,12 @@
   entries, for 1, 2, 3, 4, 5 and 64 states (every few-state kernel width
   and the many-state kernel).
 * Reference plug-in ABI (icl_*_sparse, 1-based host CSR): same bar.
-* Block-sparse split-BF16 form (precision bf16x2): same kept entries, tile
-  count exact, product within 1e-5 of the FP64 product over the kept entries.
-* Solver with demag_approx 'threshold' / 'threshold_fraction' (FP32 CSR, and
-  split-BF16 tiles): trajectories within 1e-4 of the oracle run with the same
-  threshold.
+* Block-sparse forms: same kept entries, tile count exact; split-BF16 tiles
+  (precision bf16x2) within 1e-5 and FP32 tiles (FP32 MFMAs) within 1e-6 of
+  the FP64 product over the kept entries and of the FP32 CSR product.
+* Solver with demag_approx 'threshold' / 'threshold_fraction' (FP32 CSR,
+  split-BF16 tiles, and FP32 tiles for batches of eight or more states):
+  trajectories within 1e-4 of the oracle run with the same threshold.
 """
 
 import ctypes as C
@@ -117,16 +118,17 @@ def test_icl_sparse_plugin_abi(oracle):
     assert rel < 1e-6, rel
 
 
+@pytest.mark.parametrize("kind", ["bf16x2", "f32"])
 @pytest.mark.parametrize("res,grid_nx,approx,value", [([64, 16, 2], 64, 4, 0.9), ([64, 16, 2], 0, 2, 2e-3),
                                                      ([20, 7, 1], 0, 4, 0.5), ([32, 8, 1], 32, 2, 10.0)])
-def test_block_sparse_bf16x2_product(oracle, res, grid_nx, approx, value):
-    """The split-BF16 block-sparse form of the threshold tensor: the same kept
-    entries (per-component counts exact), fragments only for the tiles that
-    hold one, and the batched MFMA product within 1e-5 (norm-wise) of the FP64
-    product over the kept entries and of the FP32 CSR product, for 1 to 70
-    states; rows of a uniform grid grouped across grid rows (grid_nx) or
-    consecutively (grid_nx = 0, a ragged 140-cell grid); a threshold above
-    every entry keeps nothing and gives zero fields."""
+def test_block_sparse_product(oracle, res, grid_nx, approx, value, kind):
+    """The block-sparse forms of the threshold tensor: the same kept entries
+    (per-component counts exact), fragments only for the tiles that hold one,
+    and the batched MFMA product — split-BF16 within 1e-5, FP32 (FP32 MFMAs)
+    within 1e-6 (norm-wise) — of the FP64 product over the kept entries and of
+    the FP32 CSR product, for 1 to 70 states; rows of a uniform grid grouped
+    across grid rows (grid_nx) or consecutively (grid_nx = 0, a ragged 140-cell
+    grid); a threshold above every entry keeps nothing and gives zero fields."""
     import torch
     from magtense_amd import _lib
 
@@ -135,7 +137,9 @@ def test_block_sparse_bf16x2_product(oracle, res, grid_nx, approx, value):
     n, K32 = _tensor(res, L)
     K = K32.cpu().numpy().reshape(6, n, n)
     hb, hc = C.c_void_p(), C.c_void_p()
-    _lib.check(Lb.mt_demag_sparse_create_bf16x2(K32.data_ptr(), n, approx, value, grid_nx, C.byref(hb)))
+    create = Lb.mt_demag_sparse_create_bf16x2 if kind == "bf16x2" else Lb.mt_demag_sparse_create_f32_tiles
+    bar = 1e-5 if kind == "bf16x2" else 1e-6
+    _lib.check(create(K32.data_ptr(), n, approx, value, grid_nx, C.byref(hb)))
     _lib.check(Lb.mt_demag_sparse_create(K32.data_ptr(), n, approx, value, C.byref(hc)))
     try:
         nnz_b, nnz_c = (C.c_longlong * 6)(), (C.c_longlong * 6)()
@@ -156,7 +160,7 @@ def test_block_sparse_bf16x2_product(oracle, res, grid_nx, approx, value):
             kept |= mk.reshape(np32 // 32, 32, np32 // 16, 16).any(axis=(1, 3))
         assert tiles.value == int(kept.sum())
         assert nbytes.value == ent.value * 4 * 12 * 1024
-        print(f"res={res} grid_nx={grid_nx} approx={approx} value={value}: kept {sum(nnz_b)} entries "
+        print(f"{kind} res={res} grid_nx={grid_nx} approx={approx} value={value}: kept {sum(nnz_b)} entries "
               f"({sum(nnz_b) / (6 * n * n):.3f} of dense), tiles {tiles.value} of {kept.size}, list entries "
               f"{ent.value}, k-split {ks.value}")
         for S in (1, 5, 64, 70):
@@ -174,8 +178,10 @@ def test_block_sparse_bf16x2_product(oracle, res, grid_nx, approx, value):
                 assert np.all(gb == 0) and np.all(gc == 0)
                 continue
             nr = np.linalg.norm(ref)
-            assert np.linalg.norm(gb - ref) / nr < 1e-5, (S, np.linalg.norm(gb - ref) / nr)
-            assert np.linalg.norm(gb - gc) / nr < 1e-5
+            print(f"  S={S}: tiles vs FP64 {np.linalg.norm(gb - ref) / nr:.2e}, CSR vs FP64 "
+                  f"{np.linalg.norm(gc - ref) / nr:.2e}")
+            assert np.linalg.norm(gb - ref) / nr < bar, (S, np.linalg.norm(gb - ref) / nr)
+            assert np.linalg.norm(gb - gc) / nr < bar
     finally:
         Lb.mt_demag_sparse_destroy(hb)
         Lb.mt_demag_sparse_destroy(hc)
@@ -206,3 +212,43 @@ def test_solver_threshold_matches_oracle(oracle, mode, value, precision):
     print(mode, value, precision, "threshold info", p.last_demag_info, "max traj err", err)
     assert err < 1e-4
     assert sum(p.last_demag_info["nnz"]) < 6 * 64 * 64
+
+
+def test_solver_threshold_fp32_tiles_batch(oracle, monkeypatch):
+    """Eight FP32 states with a threshold tensor take the FP32 tiles (FP32
+    MFMAs over the stored tiles, one pass for all states): every state within
+    1e-4 of its oracle run with the same threshold, and within 1e-5 of the
+    same batch run on the CSR form (MT_F32_TILE_STATES=0), which differs only
+    in the FP32 summation order."""
+    from magtense.micromag import MicromagProblem
+
+    res, L = [16, 4, 1], [250e-9, 62.5e-9, 3e-9]
+    n, B = 64, 8
+    rng = np.random.default_rng(33)
+    m0 = rng.normal(size=(B, n, 3))
+    m0 /= np.linalg.norm(m0, axis=2, keepdims=True)
+    ang = np.linspace(0, 2 * np.pi, B, endpoint=False)
+    fields = np.stack([3e4 * np.cos(ang), 3e4 * np.sin(ang), np.zeros(B)], axis=1)
+    rows = np.repeat(fields[:, None, :], 10, axis=1)
+    kw = dict(alpha=4.42e3, gamma=2.21e5, tol=1e-6, demag_approx="threshold_fraction", dem_thres=0.8)
+    p = MicromagProblem(res, L, **kw)
+    _, M_t = p.run_simulation_batch(5e-11, 6, rows, 10, m0)
+    assert p.last_demag_info["form"] == "fp32 tiles", p.last_demag_info
+    monkeypatch.setenv("MT_F32_TILE_STATES", "0")
+    q = MicromagProblem(res, L, **kw)
+    _, M_c = q.run_simulation_batch(5e-11, 6, rows, 10, m0)
+    assert q.last_demag_info["form"] == "csr", q.last_demag_info
+    d = np.abs(M_t - M_c).max()
+    print(f"fp32 tiles vs CSR batch: max |dm| {d:.2e}")
+    assert d < 1e-5, d
+    t = np.linspace(0, 5e-11, 6)
+    for b in (0, 3, 7):
+        hext = np.zeros((10, 4))
+        hext[:, 0] = np.linspace(0, 5e-11, 10)
+        hext[:, 1:] = fields[b]
+        o = oracle.Problem(res, L, hext=hext, t=t, m0=np.concatenate([m0[b][:, 0], m0[b][:, 1], m0[b][:, 2]]),
+                           alpha=4.42e3, gamma=2.21e5, tol=1e-6, demag_approx=4, dem_thres=0.8)
+        _, y, _, _ = o.run()
+        err = np.abs(M_t[b] - np.transpose(y[0], (0, 2, 1))).max()
+        print(f"state {b}: max |m - oracle| {err:.2e}")
+        assert err < 1e-4, (b, err)
