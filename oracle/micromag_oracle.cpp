@@ -28,6 +28,21 @@
 // tensor at receiver i due to source prism j (the transpose of the reference's
 // column-major Kxx(:,j) storage, identical for symmetric grids).
 //
+// Independent checks of every restated function (tests/test_oracle*.py,
+// tests/test_adjoint_oracle.py; none retypes these formulas):
+//   * prism tensor: cube self term -I/3, centre closed form, trace -1 / 0,
+//     symmetries, far-field dipole, surface-charge quadrature (scipy), the
+//     reference's COMSOL line profiles through the tile rotation;
+//   * grid / layout: the reference's centre formula, block symmetry;
+//   * exchange: zero row sums, symmetry, semi-definiteness, linear fields;
+//   * anisotropy, field factors, dm/dt: closed forms, m . dm/dt = 0, the
+//     damping and precession invariants;
+//   * interp1 / locate: numpy interp / searchsorted on non-uniform tables,
+//     endpoint and left-of-table rules;
+//   * RKSUITE 'M': order conditions of the coefficient tables, convergence
+//     orders, the interpolant, the work-limit return, the tolerance check;
+//   * RK4 adjoint: central finite differences.
+//
 // Deviations (documented, all outside normal operation):
 //   * interp1 clamps to the last table value when t lies right of the table
 //     (the reference reads one element past the end there, util.f90:58).
