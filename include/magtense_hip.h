@@ -203,7 +203,7 @@ int mt_demag_sparse_create_bf16x2(const float* K32, int n, int approx, double th
 /* The same block-sparse tiles with FP32 values (the reference's FP32
  * arithmetic): the product runs v_mfma_f32_32x32x2f32 over the stored tiles
  * with FP32 accumulation, for many states at once (the solver takes this form
- * for the threshold modes with precision fp32 and eight or more states).  No
+ * for the threshold modes with precision fp32 and five or more states).  No
  * reference counterpart (the reference multiplies one state at a time by the
  * CSR matrix, MagTenseCudaBlas.cu:106-297, LandauLifshitzEquationSolver.f90:581-624). */
 int mt_demag_sparse_create_f32_tiles(const float* K32, int n, int approx, double threshold, int grid_nx, void** out);
@@ -326,7 +326,7 @@ int mt_solver_set_timing(mt_solver* s, int on);
 int mt_solver_demag_info(mt_solver* s, long long* nnz6, float* threshold);
 /* The form the solver holds its demag tensor in: 0 dense (FP32 or split-BF16,
  * or the translation-invariant table), 1 threshold CSR, 2 threshold tiles in
- * split-BF16, 3 threshold tiles in FP32 (precision fp32 with eight or more
+ * split-BF16, 3 threshold tiles in FP32 (precision fp32 with five or more
  * states).  No reference counterpart (the reference's sparse path is CSR). */
 int mt_solver_demag_form(mt_solver* s, int* form);
 void mt_solver_destroy(mt_solver* s);

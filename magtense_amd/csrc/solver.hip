@@ -1522,11 +1522,13 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       if (rc) return rc;
     }
     int rc = kOk;
-    // FP32 with many states: the kept entries as FP32 tiles on FP32 MFMAs
-    // (one pass over the tiles for all states instead of the CSR gathers);
-    // MT_F32_TILE_STATES sets the smallest batch that takes them (default 8)
+    // FP32 with five or more states: the kept entries as FP32 tiles on FP32
+    // MFMAs (one pass over the tiles for all states; on the 32768-cell film at
+    // 8 % density 1.8 ms for up to 32 states and 3.0 ms for 64, against 1.76
+    // ms for four states on the CSR and 2.7-9.8 ms from five);
+    // MT_F32_TILE_STATES sets the smallest batch that takes them (0: never)
     const char* env_ft = std::getenv("MT_F32_TILE_STATES");
-    const int f32_tile_states = env_ft ? std::atoi(env_ft) : 8;
+    const int f32_tile_states = env_ft ? std::atoi(env_ft) : 5;
     const bool f32_tiles = P.precision != MT_PREC_BF16X2 && f32_tile_states > 0 && P.B >= f32_tile_states;
     if (P.precision == MT_PREC_BF16X2 || f32_tiles) {
       P.sparse.blocks = new BlockSparse();

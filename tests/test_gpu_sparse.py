@@ -4,14 +4,14 @@
   same FP32 tensor (integer, exact); the fraction threshold equals the
   oracle's FP32 bisection exactly (same arithmetic, integer counts).
 * Product: 1e-6 relative (norm-wise) to the FP64 product over the kept
-  entries, for 1, 2, 3, 4, 5 and 64 states (every few-state kernel width
-  and the many-state kernel).
+  entries, for 1, 2, 3, 4, 5, 15, 17 and 64 states (every few-state kernel
+  width, up to 16 states in groups of four, and the many-state kernel).
 * Reference plug-in ABI (icl_*_sparse, 1-based host CSR): same bar.
 * Block-sparse forms: same kept entries, tile count exact; split-BF16 tiles
   (precision bf16x2) within 1e-5 and FP32 tiles (FP32 MFMAs) within 1e-6 of
   the FP64 product over the kept entries and of the FP32 CSR product.
 * Solver with demag_approx 'threshold' / 'threshold_fraction' (FP32 CSR,
-  split-BF16 tiles, and FP32 tiles for batches of eight or more states):
+  split-BF16 tiles, and FP32 tiles for batches of five or more states):
   trajectories within 1e-4 of the oracle run with the same threshold.
 """
 
@@ -72,7 +72,7 @@ def test_sparse_build_and_product(oracle, approx, value):
         expect = [int(np.count_nonzero(np.abs(K[c]) > t)) for c in range(6)]
         assert list(nnz) == expect
         print(f"approx={approx} value={value}: threshold {thr.value:.3e}, nnz {list(nnz)} of {n * n} each")
-        for S in (1, 2, 3, 4, 5, 64):
+        for S in (1, 2, 3, 4, 5, 15, 17, 64):
             rng = np.random.default_rng(S)
             m = rng.normal(size=(S, 3, n)).astype(np.float32)
             ref = -_masked_product(K, t, m.astype(np.float64))
