@@ -202,6 +202,8 @@ def field_error_vs_fp64(W, n_rows=1024, n_states=64, seed=7):
                                  split.data_ptr()))
     tab = torch.empty(L.mt_toeplitz_tensor_bytes(*res), dtype=torch.uint8, device="cuda")
     _lib.check(L.mt_toeplitz_tensor(*res, *Lg, tab.data_ptr()))
+    f32f = torch.empty(L.mt_split_tensor_bytes(n), dtype=torch.uint8, device="cuda")
+    _lib.check(L.mt_f32_mfma_tensor(K32.data_ptr(), n, f32f.data_ptr()))
     names, coh = coherent_states(pts, Lg)
     sets = {"random": W["m0"][:n_states].reshape(n_states, 3, n), "coherent": coh}
     Kr = K64.cpu().numpy().reshape(6, n_rows, n)
@@ -215,9 +217,11 @@ def field_error_vs_fp64(W, n_rows=1024, n_states=64, seed=7):
             for b in range(3):
                 ref[:, a] -= m[:, b] @ Kr[comp[a][b]].T
         h = torch.empty(S * 3 * n, dtype=torch.float32, device="cuda")
-        for kind in ("fp32", "bf16x2", "bf16x2_table", "bf16x2_near", "bf16x2_table_near"):
+        for kind in ("fp32", "fp32_mfma", "bf16x2", "bf16x2_table", "bf16x2_near", "bf16x2_table_near"):
             if kind == "fp32":
                 _lib.check(L.mt_demag_apply_f32(K32.data_ptr(), n, dm.data_ptr(), h.data_ptr(), S, -1.0))
+            elif kind == "fp32_mfma":
+                _lib.check(L.mt_demag_apply_f32_mfma(f32f.data_ptr(), n, dm.data_ptr(), h.data_ptr(), S, -1.0))
             elif kind.startswith("bf16x2_table"):
                 _lib.check(L.mt_demag_apply_bf16x2_toeplitz(tab.data_ptr(), *res, dm.data_ptr(), h.data_ptr(), S,
                                                             -1.0))
@@ -239,10 +243,11 @@ def field_error_vs_fp64(W, n_rows=1024, n_states=64, seed=7):
                 out[key + "_rel_per_state"] = norm_rel
                 out[key + "_row_max_rel_per_state"] = [float(v) for v in row_rel.max(axis=1)]
                 out[key + "_row_max_rel_to_max_field_per_state"] = [float(v) for v in row_to_max.max(axis=1)]
-    del K32, split, tab
+    del K32, split, tab, f32f
     # headline figures: the solver's product (dense split-BF16 + near band) over every state, and FP32
     out["bf16x2_rel"] = max(out["bf16x2_near_random_rel"], max(out["bf16x2_near_coherent_rel_per_state"]))
     out["fp32_rel"] = max(out["fp32_random_rel"], max(out["fp32_coherent_rel_per_state"]))
+    out["fp32_mfma_rel"] = max(out["fp32_mfma_random_rel"], max(out["fp32_mfma_coherent_rel_per_state"]))
     out["note"] = ("rel: norm-wise over the sampled rows; row_max_rel: largest per-receiver relative error "
                    "(ill-conditioned where a receiver's field nearly cancels); row_max_rel_to_max_field: largest "
                    "per-receiver error over the largest field; _near: with the radius-2 near-band correction the "
@@ -757,6 +762,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--mfma32-steps", type=int, default=3,
+                    help="batch steps of the FP32 MFMA product (precision fp32_mfma) timed beside the others (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-field-error", action="store_true")
     ap.add_argument("--batch", type=int, default=64)
@@ -879,6 +886,33 @@ def main():
                                "(demag_llg_f32_kernel: the FP32 rows, 6 N^2 4 B per state, fused with the LL "
                                "right-hand side)")
         log(f"fp32 plain: {args.plain_steps} steps in {wall2:.3f} s = {plain:.4f} steps/s, {plain_passes} passes")
+    # ---- batched FP32 MFMA (precision fp32_mfma): the reference's arithmetic class, one tensor pass per RHS
+    mfma32 = None
+    if args.mfma32_steps > 0:
+        L3, h3, keep3 = make_solver(W, _lib.PREC_FP32_MFMA)
+        time_steps(L3, h3, 1, B)
+        s30 = stats(L3, h3, B)
+        wall3, bs3 = time_steps(L3, h3, args.mfma32_steps, B)
+        s31 = stats(L3, h3, B)
+        L3.mt_solver_set_timing(h3, 1)
+        s3a = stats(L3, h3, B)
+        time_steps(L3, h3, 1, B)
+        s3b = stats(L3, h3, B)
+        L3.mt_solver_set_timing(h3, 0)
+        L3.mt_solver_destroy(h3)
+        del keep3
+        p3 = max(s3b[0].demag_passes - s3a[0].demag_passes, 1)
+        g3 = s3b[0].demag_ms / p3
+        tf3 = alg_flops / (g3 * 1e-3) / 1e12
+        mfma32 = dict(steps_per_s=bs3 / wall3, batch_steps=bs3, wall_s=wall3,
+                      passes_per_batch_step=(s31[0].demag_passes - s30[0].demag_passes) / max(bs3, 1e-9),
+                      gemm_ms=g3, speedup_vs_plain_fp32=(bs3 / wall3 / plain) if plain else None,
+                      roofline={"bound": "mfma", "kernel": "gemm_bf16x2_reg_kernel<4, false, false, true> (FP32 fragments)",
+                                "achieved": tf3, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": tf3 / FP32_PEAK_TFLOPS, "alg_flops_per_launch": alg_flops},
+                      note="precision fp32_mfma: the FP32 tensor as MFMA fragments, FP32 products and sums, one "
+                           "pass over the 25.8 GB tensor for all 64 states")
+        log(f"fp32 mfma: {bs3 / wall3:.3f} steps/s, gemm {g3:.3f} ms = {tf3:.1f} TFLOP/s")
     cfgs = None if args.no_configs else small_configs(no_cpu=args.no_cpu)
     log(f"plain-method configurations: {json.dumps(cfgs)}")
     shard = None if args.no_shard else shard_shape(W)
@@ -942,6 +976,7 @@ def main():
         "rejected_attempts": int(rejected),
         "plain_fp32_steps_per_s": plain,
         "plain_fp32": plain_info,
+        "fp32_mfma": mfma32,
         "speedup_vs_plain_fp32": (steps_per_s / plain) if (plain and steps_per_s) else None,
         "setup_s": t_setup,
         "field_error_vs_fp64": ferr,

@@ -37,6 +37,8 @@ extern "C" {
 #define MT_PREC_FP32 0   /* plain per-state FP32 mat-vec (reference semantics) */
 #define MT_PREC_BF16X2 1 /* split BF16 hi+lo tensor, batched MFMA GEMM, FP32 accumulation */
 #define MT_PREC_FP64 2   /* FP64 tensor, product and Ms scaling (mt_rk4 only: gradient checks) */
+#define MT_PREC_FP32_MFMA 3 /* FP32 tensor as MFMA fragments, batched FP32 MFMA GEMM (v_mfma_f32_32x32x2f32),
+                               FP32 accumulation: the reference's arithmetic class, one tensor pass per 64 states */
 
 /* status codes (every int-returning entry point; MT_ERR_UNSUPPORTED below) */
 #define MT_OK 0
@@ -122,6 +124,10 @@ long long mt_split_tensor_bytes(int n);
 long long mt_packed_m_bytes(int n);
 /* FP32 receiver-major tensor (device) -> split BF16 fragment layout (device). */
 int mt_split_tensor(const float* K32, int n, void* split);
+/* FP32 receiver-major tensor (device) -> the same fragment order with FP32
+ * values for v_mfma_f32_32x32x2f32 (mt_split_tensor_bytes(n) bytes, device).
+ * No reference counterpart (an additive batched form of the FP32 product). */
+int mt_f32_mfma_tensor(const float* K32, int n, void* frags);
 
 /* Translation-invariant split-BF16 table of a uniform nx x ny x nz grid (nx % 32
  * == 0; cell sizes and centres as setupGrid, LandauLifshitzEquationSolver.f90:
@@ -141,6 +147,10 @@ int mt_demag_apply_f32(const float* K32, int n, const float* m, float* h, int ns
 /* Same product from the split tensor: packs m (FP32, device) into BF16 hi/lo
  * fragments internally; nstates <= 64 per call (larger batches are chunked). */
 int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, int nstates, float pref);
+/* Same product from the FP32 fragments of mt_f32_mfma_tensor on FP32 MFMAs
+ * (FP32 products and sums: the arithmetic class of the reference's sgemv,
+ * LandauLifshitzEquationSolver.f90:706-754, one tensor pass per 64 states). */
+int mt_demag_apply_f32_mfma(const void* frags, int n, const float* m, float* h, int nstates, float pref);
 /* Row block of the same product, the per-GPU product of a tensor sharded by
  * receiver rows: h[s*3*nrows + c*nrows + r] = pref * sum_j K_c(row0 + r, j) m[...j]
  * for r < nrows, all n sources.  split_rows points at the fragment rows of
@@ -256,7 +266,7 @@ typedef struct {
   int nt_conv;
   const double* t_conv;
   double conv_tol;
-  int precision;         /* MT_PREC_FP32 or MT_PREC_BF16X2 */
+  int precision;         /* MT_PREC_FP32, MT_PREC_BF16X2 or MT_PREC_FP32_MFMA */
   int return_fields;     /* compute H_exc/H_ext/H_dem/H_ani at every output time */
   long long max_steps;   /* > 0: pause every state after this many accepted steps */
   const float* K_in;     /* optional host FP32 tensor, 6*n*n receiver-major (skips assembly) */

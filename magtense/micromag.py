@@ -8,12 +8,15 @@ goes to the C ABI of ``libmagtense_hip.so`` (ctypes) instead of the f2py
 module ``magtensesource``.
 
 Additive extensions (no reference counterpart):
-  * ``precision='fp32' | 'bf16x2'`` selects the demag product: the plain
-    per-state FP32 mat-vec (reference semantics, the default) or the split-BF16
-    batched MFMA GEMM.  With ``demag_approx`` 'threshold' or
-    'threshold_fraction' the kept entries are an FP32 CSR product ('fp32') or
-    the split-BF16 GEMM over the 32 x 16 tiles that hold a kept entry
-    ('bf16x2').  A boolean ``precision`` (the reference's unused
+  * ``precision='fp32' | 'bf16x2' | 'fp32_mfma'`` selects the demag product:
+    the plain per-state FP32 mat-vec (reference semantics, the default), the
+    split-BF16 batched MFMA GEMM, or the batched FP32 MFMA GEMM (FP32
+    products and sums, the reference's arithmetic class, one tensor pass for
+    up to 64 states).  With ``demag_approx`` 'threshold' or
+    'threshold_fraction' the kept entries are an FP32 CSR product ('fp32',
+    up to four states) or a GEMM over the 32 x 16 tiles that hold a kept
+    entry (FP32 tiles: 'fp32' from five states and 'fp32_mfma'; split-BF16
+    tiles: 'bf16x2').  A boolean ``precision`` (the reference's unused
     ``usePrecision`` flag) keeps the default.
   * ``Ms`` may be an array of per-cell values.
   * ``run_simulation_batch`` integrates several independent states (and
@@ -225,9 +228,9 @@ class MicromagProblem:
         p = self.precision
         if isinstance(p, str):
             try:
-                return {"fp32": _lib.PREC_FP32, "bf16x2": _lib.PREC_BF16X2}[p]
+                return {"fp32": _lib.PREC_FP32, "bf16x2": _lib.PREC_BF16X2, "fp32_mfma": _lib.PREC_FP32_MFMA}[p]
             except KeyError:
-                raise ValueError("precision must be 'fp32' or 'bf16x2'") from None
+                raise ValueError("precision must be 'fp32', 'bf16x2' or 'fp32_mfma'") from None
         return _lib.PREC_FP32
 
     def set_exchange_matrix(self, A):

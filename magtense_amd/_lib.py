@@ -16,6 +16,7 @@ CSRC = os.path.join(_HERE, "csrc")
 PREC_FP32 = 0
 PREC_BF16X2 = 1
 PREC_FP64 = 2
+PREC_FP32_MFMA = 3
 DEMAG_DENSE = 1
 DEMAG_THRESHOLD = 2
 DEMAG_THRESHOLD_FRACTION = 4
@@ -82,7 +83,7 @@ class MtTiles(C.Structure):
 # every symbol declared in include/magtense_hip.h
 EXPORTS = [
     "mt_last_error", "mt_version", "mt_device_count", "mt_measure_bandwidth", "mt_measure_bandwidth_pattern", "mt_prism_tensor", "mt_split_tensor_bytes",
-    "mt_packed_m_bytes", "mt_split_tensor", "mt_demag_apply_f32", "mt_demag_apply_bf16x2", "mt_ll_rhs",
+    "mt_packed_m_bytes", "mt_split_tensor", "mt_f32_mfma_tensor", "mt_demag_apply_f32_mfma", "mt_demag_apply_f32", "mt_demag_apply_bf16x2", "mt_ll_rhs",
     "mt_solver_create", "mt_solver_advance", "mt_solver_run", "mt_solver_advance_total", "mt_solver_steps_total", "mt_solver_results", "mt_solver_state",
     "mt_solver_set_timing", "mt_solver_destroy", "mt_run_micromag", "mt_rk4_create", "mt_rk4_forward",
     "mt_rk4_backward", "mt_rk4_destroy", "mt_tiles_demag_tensor", "mt_tiles_h_field", "mt_demag_sparse_create",
@@ -132,6 +133,8 @@ def load():
     lib.mt_split_tensor.argtypes = [vp, C.c_int, vp]
     lib.mt_demag_apply_f32.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_demag_apply_bf16x2.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_float]
+    lib.mt_f32_mfma_tensor.argtypes = [vp, C.c_int, vp]
+    lib.mt_demag_apply_f32_mfma.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_float]
     lib.mt_ll_rhs.argtypes = [C.c_int] * 3 + [C.c_double] * 3 + [vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                                                   vp, vp, vp, C.c_int, vp, vp]
     lib.mt_solver_create.argtypes = [C.POINTER(MtProblem), C.POINTER(vp)]

@@ -251,6 +251,16 @@ int mt_split_tensor(const float* K32, int n, void* split) {
   return kOk;
 }
 
+int mt_f32_mfma_tensor(const float* K32, int n, void* frags) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n > 0 && K32 && frags, "invalid FP32 fragment arguments");
+  rc = launch_f32_fragments(K32, n, (uint8_t*)frags, nullptr);
+  if (rc) return rc;
+  MT_HIP(hipStreamSynchronize(nullptr));
+  return kOk;
+}
+
 int mt_demag_apply_f32(const float* K32, int n, const float* m, float* h, int nstates, float pref) {
   int rc = device_ok();
   if (rc) return rc;
@@ -322,6 +332,8 @@ int apply_bf16x2_in(const void* K, int n, int nrows, int nx, int ny, int nz, con
     MT_HIP(hipGetLastError());
     if (bsv)
       rc = launch_gemm_bsr((const uint8_t*)K, *bsv, f32, packed, n, part, n3, (int64_t)cols * n3, S, ks, nullptr);
+    else if (f32)
+      rc = launch_gemm_f32((const uint8_t*)K, packed, n, part, n3, (int64_t)cols * n3, S, ks, nullptr);
     else if (nx > 0)
       rc = launch_gemm_bf16x2_toeplitz((const uint8_t*)K, nx, ny, nz, packed, part, n3, (int64_t)cols * n3, S, ks,
                                        nullptr);
@@ -366,6 +378,16 @@ int mt_demag_apply_bf16x2(const void* split, int n, const float* m, float* h, in
   if (rc) return rc;
   MT_CHECK(n > 0 && nstates > 0 && split && m && h, "invalid demag arguments");
   return apply_bf16x2(split, n, n, 0, 0, 0, m, h, nstates, pref);
+}
+
+int mt_demag_apply_f32_mfma(const void* frags, int n, const float* m, float* h, int nstates, float pref) {
+  int rc = device_ok();
+  if (rc) return rc;
+  MT_CHECK(n > 0 && nstates > 0 && frags && m && h, "invalid demag arguments");
+  const Bf16x2Work w = bf16x2_work(n, n, 0, 0, 0, nstates);
+  DevArr<uint8_t> ws;
+  MT_HIP(ws.alloc(w.bytes()));
+  return apply_bf16x2_in(frags, n, n, 0, 0, 0, m, h, nstates, pref, w, ws.p, nullptr, true);
 }
 
 int mt_demag_near_correction(int nx, int ny, int nz, double Lx, double Ly, double Lz, int radius, const float* m,

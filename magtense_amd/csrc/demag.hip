@@ -284,6 +284,11 @@ __global__ void __launch_bounds__(256) gemm_bf16x2_lds_kernel(const uint4* __res
 }
 
 constexpr int kFlushChunks = 128;  // chunks summed in the MFMA accumulators before they are flushed
+// FP32 fragments: 24 roundings per chunk and element (eight FP32 MFMAs per
+// block, three blocks per output component) instead of nine; flushing every 8
+// chunks keeps the inner chain at 192 (the 256x64x2 film's smooth in-plane
+// state: 1.05e-5 of the FP64 field with flushes every 128 chunks)
+constexpr int kFlushChunksF32 = 8;
 constexpr int kTableChainChunks = 256;  // longest partial sum of the entry-reuse table kernel
 
 // Register-ring kernel for the dense tensor.  Each wave owns one 32-row tile
@@ -504,7 +509,7 @@ __global__ void __launch_bounds__(256, 1) gemm_bf16x2_reg_kernel(const uint4* __
         MT_REG_ISSUE(j0 + s + D - 1, (s + D - 1) % D);
         MT_REG_CONSUME(j0 + s, s);
       }
-      if ((j0 + D) % kFlushChunks == 0) {
+      if ((j0 + D) % (F32 ? kFlushChunksF32 : kFlushChunks) == 0) {
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -695,6 +700,11 @@ int launch_gemm_bf16x2_rows(const uint8_t* Krows, const uint8_t* Mpack, int n, i
 int launch_gemm_bf16x2(const uint8_t* Ksplit, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
                        int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
   return launch_gemm_bf16x2_rows(Ksplit, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream);
+}
+
+int launch_gemm_f32(const uint8_t* Kf, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
+                    int64_t split_stride, int ncols, int ksplit, hipStream_t stream) {
+  return launch_reg<4, false, false, true>(Kf, Mpack, n, n, out, col_stride, split_stride, ncols, ksplit, stream);
 }
 
 // Four-tile kernel for the translation-invariant table: a workgroup of four

@@ -17,6 +17,8 @@ int launch_tiles_tensor(const double* tiles, int nt, const double* pts, int npt,
 int launch_tiles_field(const double* tiles, int nt, const double* pts, int npt, const double* Nin, double* H,
                        hipStream_t stream);
 int launch_split(const float* K, int n, uint8_t* split, hipStream_t stream);
+// FP32 planes -> the dense tensor as FP32 MFMA fragments (split_tensor_bytes(n) bytes)
+int launch_f32_fragments(const float* K, int n, uint8_t* frags, hipStream_t stream);
 // Translation-invariant split-BF16 fragment table of a uniform grid (nx % 32 == 0),
 // toeplitz_tensor_bytes(nx, ny, nz) bytes.
 // Unstructured prisms: pts / abc (n x 3) per cell, receiver averaging over
@@ -75,6 +77,10 @@ int launch_gemm_bf16x2_toeplitz(const uint8_t* Ktab, int nx, int ny, int nz, con
 // states (the FP32 layouts of pack_f32_frag) on FP32 MFMAs.
 int launch_gemm_bsr(const uint8_t* frags, const BlockSparseView& bs, bool f32, const uint8_t* Mpack, int n, float* out,
                     int64_t col_stride, int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
+// The dense product over FP32 fragments (launch_f32_fragments) with FP32
+// state fragments, on FP32 MFMAs: partial sums as launch_gemm_bf16x2.
+int launch_gemm_f32(const uint8_t* Kf, const uint8_t* Mpack, int n, float* out, int64_t col_stride,
+                    int64_t split_stride, int ncols, int ksplit, hipStream_t stream);
 
 struct BlockSparse {
   int n = 0, ngroups = 0;

@@ -318,6 +318,34 @@ __global__ void __launch_bounds__(256) split_kernel(const float* __restrict__ K,
   }
 }
 
+// FP32 planes -> FP32 MFMA A-fragments of the dense tensor (v_mfma_f32_32x32x2f32):
+// the fragment sets of the split tensor's order, component c's fragments 2c +
+// p giving lane l the k-steps s = 4p .. 4p + 3, K_c(row 32I + l % 32, k = 16J +
+// 2s + l / 32); rows and sources past n are zero.  One thread per (row,
+// chunk, lane half).
+__global__ void __launch_bounds__(256) f32_frag_kernel(const float* __restrict__ K, int n, int64_t np,
+                                                       uint8_t* __restrict__ frags) {
+  const int64_t nJ = np / kTileK;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = tid / (2 * nJ);
+  if (i >= np) return;
+  const int64_t J = (tid / 2) % nJ;
+  const int half = (int)(tid % 2);
+  const int lane = (int)(i % kTileM) + kTileM * half;
+  const int64_t I = i / kTileM;
+  const size_t plane = (size_t)n * n;
+  for (int c = 0; c < 6; ++c)
+    for (int p = 0; p < 2; ++p) {
+      float v[4];
+      for (int q = 0; q < 4; ++q) {
+        const int64_t j = J * kTileK + 2 * (4 * p + q) + half;
+        v[q] = (i < n && j < n) ? K[c * plane + (size_t)i * n + j] : 0.0f;
+      }
+      const int64_t frag = ((I * nJ + J) * 6 + c) * 2 + p;
+      *reinterpret_cast<float4*>(frags + frag * kFragBytes + (int64_t)lane * 16) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+}
+
 // ---- tetrahedron (tileType 5) ---------------------------------------------------
 // Sum over the four faces of the right-triangle face tensor
 // (getN_tensor_tetrahedron, TileNComponents.f90:944-961; getN_Triangle and the
@@ -742,6 +770,14 @@ int launch_near_table(int nx, int ny, int nz, double Lx, double Ly, double Lz, i
   const int no = (2 * R + 1) * (2 * R + 1) * (2 * Rz + 1);
   hipLaunchKernelGGL(near_table_kernel, dim3((unsigned)((no + 63) / 64)), dim3(64), 0, stream, nx, ny, nz, Lx, Ly, Lz,
                      R, Rz, thr, table);
+  MT_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_f32_fragments(const float* K, int n, uint8_t* frags, hipStream_t stream) {
+  const int64_t np = round_up(n, kTileM);
+  const int64_t threads = np * (np / kTileK) * 2;
+  hipLaunchKernelGGL(f32_frag_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, K, n, np, frags);
   MT_HIP(hipGetLastError());
   return kOk;
 }

@@ -287,6 +287,7 @@ struct Solver {
   hipStream_t stream = nullptr;
   float* K32 = nullptr;
   uint8_t* Ksplit = nullptr;
+  uint8_t* Kf32 = nullptr;  // MT_PREC_FP32_MFMA: the dense tensor as FP32 MFMA fragments
   int *xs = nullptr, *xe = nullptr, *xc = nullptr;  // loaded exchange operator (unstructured grids)
   double* xv = nullptr;
   long long exch_nnz_host = 0;
@@ -334,7 +335,7 @@ struct Solver {
     auto f = [](void* p) {
       if (p) (void)hipFree(p);
     };
-    f(K32); f(Ksplit); f(Ktab); f(near_tab); f(jfact); f(kfact); f(aniso); f(ms32); f(pool); f(yout); f(fout); f(packf); f(packb);
+    f(K32); f(Ksplit); f(Kf32); f(Ktab); f(near_tab); f(jfact); f(kfact); f(aniso); f(ms32); f(pool); f(yout); f(fout); f(packf); f(packb);
     f(hbuf); f(partials); f(results); f(arena_d); f(xs); f(xe); f(xc); f(xv);
     sparse.release();
     if (h_results) (void)hipHostFree(h_results);
@@ -343,7 +344,7 @@ struct Solver {
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     xs = xe = xc = nullptr; xv = nullptr;
-    K32 = nullptr; Ksplit = nullptr; Ktab = nullptr; near_tab = nullptr; jfact = kfact = aniso = nullptr; ms32 = nullptr; pool = yout = fout = nullptr;
+    K32 = nullptr; Ksplit = nullptr; Kf32 = nullptr; Ktab = nullptr; near_tab = nullptr; jfact = kfact = aniso = nullptr; ms32 = nullptr; pool = yout = fout = nullptr;
     packf = nullptr; packb = nullptr; hbuf = nullptr; partials = results = nullptr; arena_d = nullptr;
     h_results = nullptr; arena_h = nullptr; stream = nullptr;
     for (hipEvent_t& e : ev) e = nullptr;
@@ -1140,7 +1141,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       L.fields = s.op.fields;
       L.hpart = hbuf + (size_t)k * N3;
       L.split_stride = (int64_t)hcols * N3;
-      L.nsplit = precision == MT_PREC_BF16X2 ? ksplit + (near_r > 0 ? 1 : 0) : (sparse.blocks ? ksplit : 1);
+      L.nsplit = precision == MT_PREC_BF16X2 ? ksplit + (near_r > 0 ? 1 : 0) : (sparse.blocks || Kf32 ? ksplit : 1);
       hext_at(s, s.op.t, L.hext);
       L.alpha = alpha_at(s.op.t);
       L.gamma = gamma;
@@ -1148,7 +1149,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     }
     PackSpec pk;
     if (precision == MT_PREC_BF16X2) pk = PackSpec{kPackBf16, nullptr, 0, packb};
-    else if (sparse.blocks) pk = PackSpec{kPackF32Frag, nullptr, 0, packb};
+    else if (sparse.blocks || Kf32) pk = PackSpec{kPackF32Frag, nullptr, 0, packb};
     else pk = PackSpec{kPackF32, packf, (int64_t)N3, nullptr};
     if (timing) MT_HIP(hipEventRecord(ev[2], stream));
     // Small groups pass their descriptors by value; otherwise the group's
@@ -1156,7 +1157,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     // reads the LL descriptors from device memory)
     const bool inl = S <= kInlineStates && !(precision == MT_PREC_BF16X2 && near_r > 0);
     // the FP32 dense path runs its demag rows and the LL right-hand side in one kernel
-    const bool fused = precision != MT_PREC_BF16X2 && demag_approx == MT_DEMAG_DENSE;
+    const bool fused = precision == MT_PREC_FP32 && demag_approx == MT_DEMAG_DENSE;
     const VecDesc* dpre = nullptr;
     const LlgDesc* dl = nullptr;
     const VecDesc* dpost = nullptr;
@@ -1189,6 +1190,8 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       if (rc == kOk && near_r > 0)
         rc = launch_near_correction(near_tab, near_r, nx, ny, nz, dl, nullptr, 0, S,
                                     hbuf + (size_t)ksplit * hcols * N3, (int64_t)N3, 1.0f, false, stream);
+    } else if (Kf32) {
+      rc = launch_gemm_f32(Kf32, packb, n, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit, stream);
     } else if (sparse.blocks) {
       rc = launch_gemm_bsr(sparse.blocks->frags, sparse.blocks->view(), true, packb, n, hbuf, (int64_t)N3,
                            (int64_t)hcols * N3, S, ksplit, stream);
@@ -1346,7 +1349,8 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   MT_CHECK(q->nt >= 2 && q->t, "need at least two output times");
   MT_CHECK(q->nt_hext >= 1 && q->hext, "need an applied-field table");
   MT_CHECK(q->solver == 1 || q->solver == 2, "solver must be 1 (explicit) or 2 (dynamic)");
-  MT_CHECK(q->precision == MT_PREC_FP32 || q->precision == MT_PREC_BF16X2, "unknown precision");
+  MT_CHECK(q->precision == MT_PREC_FP32 || q->precision == MT_PREC_BF16X2 || q->precision == MT_PREC_FP32_MFMA,
+           "unknown precision");
   const int approx = q->demag_approx == 0 ? MT_DEMAG_DENSE : q->demag_approx;
   MT_CHECK(approx == MT_DEMAG_DENSE || approx == MT_DEMAG_THRESHOLD || approx == MT_DEMAG_THRESHOLD_FRACTION,
            "demag_approx must be 1 (dense), 2 (threshold) or 4 (threshold fraction)");
@@ -1427,7 +1431,8 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   const size_t nn = (size_t)n * n;
   // the threshold modes build their sparse form from the FP32 tensor (CSR in
   // FP32, block-sparse split-BF16 tiles in bf16x2)
-  const bool need32 = P.precision == MT_PREC_FP32 || q->keep_fp32_tensor || approx != MT_DEMAG_DENSE;
+  const bool need32 = P.precision == MT_PREC_FP32 || P.precision == MT_PREC_FP32_MFMA || q->keep_fp32_tensor ||
+                      approx != MT_DEMAG_DENSE;
   const bool can_toep = !unstructured && P.precision == MT_PREC_BF16X2 && !q->K_in && P.nx >= 32 && P.nx % 32 == 0 &&
                         approx == MT_DEMAG_DENSE;
   MT_CHECK(q->toeplitz != 1 || can_toep,
@@ -1514,6 +1519,17 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
     MT_CHECK(P.K32 != nullptr, "K_out needs the FP32 tensor (precision fp32 or keep_fp32_tensor)");
     MT_HIP(hipMemcpyAsync(q->K_out, P.K32, 6 * nn * sizeof(float), hipMemcpyDeviceToHost, P.stream));
   }
+  if (P.precision == MT_PREC_FP32_MFMA && approx == MT_DEMAG_DENSE) {
+    // the FP32 planes as FP32 MFMA fragments; the planes go unless kept
+    MT_HIP(hipMalloc(&P.Kf32, (size_t)split_tensor_bytes(n)));
+    int rc = launch_f32_fragments(P.K32, n, P.Kf32, P.stream);
+    if (rc) return rc;
+    MT_HIP(hipStreamSynchronize(P.stream));
+    if (!q->keep_fp32_tensor) {
+      MT_HIP(hipFree(P.K32));
+      P.K32 = nullptr;
+    }
+  }
   if (approx != MT_DEMAG_DENSE) {
     // ApplyThresholdDense (DemagAuxFunctions.f90:20-122) after the tensor (LandauLifshitzEquationSolver.f90:1157)
     float thr = (float)q->demag_threshold;
@@ -1529,7 +1545,8 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
     // MT_F32_TILE_STATES sets the smallest batch that takes them (0: never)
     const char* env_ft = std::getenv("MT_F32_TILE_STATES");
     const int f32_tile_states = env_ft ? std::atoi(env_ft) : 5;
-    const bool f32_tiles = P.precision != MT_PREC_BF16X2 && f32_tile_states > 0 && P.B >= f32_tile_states;
+    const bool f32_tiles = P.precision == MT_PREC_FP32_MFMA ||
+                           (P.precision == MT_PREC_FP32 && f32_tile_states > 0 && P.B >= f32_tile_states);
     if (P.precision == MT_PREC_BF16X2 || f32_tiles) {
       P.sparse.blocks = new BlockSparse();
       rc = P.sparse.blocks->build(P.K32, n, thr, unstructured ? 0 : P.nx, P.stream, f32_tiles);
@@ -1655,9 +1672,10 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
       }
     }
     MT_HIP(hipMalloc(&P.hbuf, (size_t)(P.ksplit + (P.near_r > 0 ? 1 : 0)) * P.hcols * n3 * sizeof(float)));
-  } else if (P.sparse.blocks) {
-    // FP32 tiles: FP32 state fragments, the schedule's partial-sum slots
-    P.ksplit = P.sparse.blocks->ksplit;
+  } else if (P.sparse.blocks || P.Kf32) {
+    // FP32 tiles or the FP32 MFMA dense tensor: FP32 state fragments, the
+    // schedule's (or the dense kernel's) partial-sum slots
+    P.ksplit = P.sparse.blocks ? P.sparse.blocks->ksplit : (q->ksplit > 0 ? q->ksplit : gemm_ksplit_dense(n, n));
     MT_HIP(hipMalloc(&P.packb, (size_t)packed_m_bytes(n)));
     MT_HIP(hipMemsetAsync(P.packb, 0, (size_t)packed_m_bytes(n), P.stream));
     MT_HIP(hipMalloc(&P.hbuf, (size_t)P.ksplit * P.hcols * n3 * sizeof(float)));

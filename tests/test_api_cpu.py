@@ -60,6 +60,7 @@ def test_precision_codes():
     assert MicromagProblem([2, 2, 1])._precision_code() == _lib.PREC_FP32
     assert MicromagProblem([2, 2, 1], precision=True)._precision_code() == _lib.PREC_FP32
     assert MicromagProblem([2, 2, 1], precision="bf16x2")._precision_code() == _lib.PREC_BF16X2
+    assert MicromagProblem([2, 2, 1], precision="fp32_mfma")._precision_code() == _lib.PREC_FP32_MFMA
     with pytest.raises(ValueError):
         MicromagProblem([2, 2, 1], precision="fp8")._precision_code()
 
