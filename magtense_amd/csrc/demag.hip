@@ -23,6 +23,7 @@
 #include <hip/hip_bf16.h>
 
 #include <cstdlib>
+#include <type_traits>
 
 #include "matvec_rows.h"
 #include "mt_common.h"
@@ -1050,14 +1051,14 @@ __device__ inline float near_dot(uint32_t t, uint32_t v, float acc) {
                                          false);
 }
 
-template <bool F64>
-__global__ void __launch_bounds__(256) near_corr_kernel(const uint32_t* __restrict__ table, int R, int Rz, int nx,
+template <bool F64, int R>
+__global__ void __launch_bounds__(256) near_corr_kernel(const uint32_t* __restrict__ table, int Rz, int nx,
                                                         int ny, int nz, const LlgDesc* __restrict__ descs,
                                                         const float* __restrict__ m32, int64_t m_stride,
                                                         float* __restrict__ out, int64_t col_stride, float pref,
                                                         int accumulate) {
-  extern __shared__ uint2 near_lds[];
-  const int hx = kNearTX + 2 * R, hy = kNearTY + 2 * R, hc = hx * hy, w = 2 * R + 1;
+  constexpr int hx = kNearTX + 2 * R, hy = kNearTY + 2 * R, hc = hx * hy, w = 2 * R + 1;
+  __shared__ uint2 near_lds[(hc * 20 + 7) / 8];
   uint2* p0 = near_lds;                                // (W_x, W_y)
   uint2* p1 = near_lds + hc;                           // (W_z, M_xy)
   uint32_t* p2 = reinterpret_cast<uint32_t*>(near_lds + 2 * hc);  // M_z
@@ -1100,10 +1101,12 @@ __global__ void __launch_bounds__(256) near_corr_kernel(const uint32_t* __restri
     }
     __syncthreads();
     const uint32_t* tz = table + (int64_t)(oz + Rz) * w * w * kNearWords;
+#pragma unroll
     for (int oy = -R; oy <= R; ++oy) {
       // source (ix - ox, iy - oy): halo cell (lx - ox + R, ly - oy + R)
       const int hrow = (ly - oy + R) * hx + lx + R;
       const uint32_t* ty = tz + (oy + R) * w * kNearWords;
+#pragma unroll
       for (int ox = -R; ox <= R; ++ox) {
         const uint2 v0 = p0[hrow - ox], v1 = p1[hrow - ox];
         const uint32_t v2 = p2[hrow - ox];
@@ -1145,15 +1148,22 @@ int launch_near_correction(const uint32_t* table, int R, int nx, int ny, int nz,
   }
   const int Rz = near_band_rz(R, nz);
   const int tiles = ((nx + kNearTX - 1) / kNearTX) * ((ny + kNearTY - 1) / kNearTY) * nz;
-  const size_t lds = (size_t)(kNearTX + 2 * R) * (kNearTY + 2 * R) * 20;  // 8 + 8 + 4 bytes per cell
   const dim3 grid((unsigned)tiles, (unsigned)nstates);
   const int acc = accumulate ? 1 : 0;
-  if (descs)
-    hipLaunchKernelGGL(near_corr_kernel<true>, grid, dim3(256), lds, stream, table, R, Rz, nx, ny, nz, descs, m32,
-                       m_stride, out, col_stride, pref, acc);
-  else
-    hipLaunchKernelGGL(near_corr_kernel<false>, grid, dim3(256), lds, stream, table, R, Rz, nx, ny, nz, descs, m32,
-                       m_stride, out, col_stride, pref, acc);
+  // the radius as a template argument: the offset loops unroll, the halo
+  // index arithmetic and the table offsets are constants
+  auto go = [&](auto f64, auto r) {
+    hipLaunchKernelGGL((near_corr_kernel<decltype(f64)::value, decltype(r)::value>), grid, dim3(256), 0, stream,
+                       table, Rz, nx, ny, nz, descs, m32, m_stride, out, col_stride, pref, acc);
+  };
+  using T = std::true_type;
+  using F = std::false_type;
+  switch (R) {
+    case 1: descs ? go(T{}, std::integral_constant<int, 1>{}) : go(F{}, std::integral_constant<int, 1>{}); break;
+    case 2: descs ? go(T{}, std::integral_constant<int, 2>{}) : go(F{}, std::integral_constant<int, 2>{}); break;
+    case 3: descs ? go(T{}, std::integral_constant<int, 3>{}) : go(F{}, std::integral_constant<int, 3>{}); break;
+    default: descs ? go(T{}, std::integral_constant<int, 4>{}) : go(F{}, std::integral_constant<int, 4>{}); break;
+  }
   MT_HIP(hipGetLastError());
   return kOk;
 }

@@ -321,18 +321,20 @@ __global__ void __launch_bounds__(256) split_kernel(const float* __restrict__ K,
 // FP32 planes -> FP32 MFMA A-fragments of the dense tensor (v_mfma_f32_32x32x2f32):
 // the fragment sets of the split tensor's order, component c's fragments 2c +
 // p giving lane l the k-steps s = 4p .. 4p + 3, K_c(row 32I + l % 32, k = 16J +
-// 2s + l / 32); rows and sources past n are zero.  One thread per (row,
-// chunk, lane half).
+// 2s + l / 32); rows and sources past n are zero.  One wave per (row tile I,
+// chunk J), lane l of the wave writing lane l of each of the twelve fragments,
+// so every fragment is one contiguous 1 KiB store; consecutive waves take
+// consecutive chunks of the same 32 rows.
 __global__ void __launch_bounds__(256) f32_frag_kernel(const float* __restrict__ K, int n, int64_t np,
                                                        uint8_t* __restrict__ frags) {
   const int64_t nJ = np / kTileK;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t i = tid / (2 * nJ);
-  if (i >= np) return;
-  const int64_t J = (tid / 2) % nJ;
-  const int half = (int)(tid % 2);
-  const int lane = (int)(i % kTileM) + kTileM * half;
-  const int64_t I = i / kTileM;
+  const int64_t w = tid / 64;
+  const int lane = (int)(tid % 64);
+  const int64_t I = w / nJ, J = w % nJ;
+  if (I >= np / kTileM) return;
+  const int64_t i = I * kTileM + lane % 32;
+  const int half = lane / 32;
   const size_t plane = (size_t)n * n;
   for (int c = 0; c < 6; ++c)
     for (int p = 0; p < 2; ++p) {
@@ -776,7 +778,7 @@ int launch_near_table(int nx, int ny, int nz, double Lx, double Ly, double Lz, i
 
 int launch_f32_fragments(const float* K, int n, uint8_t* frags, hipStream_t stream) {
   const int64_t np = round_up(n, kTileM);
-  const int64_t threads = np * (np / kTileK) * 2;
+  const int64_t threads = (np / kTileM) * (np / kTileK) * 64;  // one wave per (row tile, chunk)
   hipLaunchKernelGGL(f32_frag_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, K, n, np, frags);
   MT_HIP(hipGetLastError());
   return kOk;
