@@ -691,10 +691,10 @@ def small_configs(no_cpu=False):
     _, st_t, _ = _gpu_run(res, L, m_rel, h_dyn, 1e-9, 200, 2000, timing=True, gamma=2.21e5, **kw)
     c1 = {"workload": "32x8x1 film (15.6x15.6x3 nm), N = 256: relaxation 5 ns (gamma 0) + 1 ns dynamics, FP32 demag",
           "relax": _line(w_rel, st_rel, None, 256, 0), "dynamics": _line(w_dyn, st_dyn, st_t, 256, 0),
-          "note": "the 1.6 MB tensor stays in L2: the step is bound by kernel latency (two launches per right-hand "
-                  "side: stage combination, then FP32 demag rows fused with the LL right-hand side) and the one host "
-                  "round trip per step of the RKSUITE controller, not by HBM; kernel_ms_per_step.demag includes "
-                  "the LL right-hand side"}
+          "note": "the 1.6 MB tensor stays in L2: the step is bound by kernel latency and the host's launch "
+                  "submissions (one launch per right-hand side: stage combination, FP32 demag rows and LL "
+                  "right-hand side in one kernel) and the one host round trip per step of the RKSUITE controller, "
+                  "not by HBM; kernel_ms_per_step.demag includes the stage combination and the LL right-hand side"}
     if not no_cpu:
         ht = np.linspace(0, 5e-9, 200)
         hext = np.zeros((200, 4))

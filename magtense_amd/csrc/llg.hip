@@ -175,6 +175,70 @@ __global__ void __launch_bounds__(256) demag_llg_f32_kernel(const float* __restr
   });
 }
 
+// Stage combination of a small problem, carried by value into the fused launch.
+struct StageDesc {
+  double* dst;
+  const double* base;
+  double bcoef, bconst, post_scale;
+  int nterm;
+  const double* src[kMaxTerms];
+  double coef[kMaxTerms];
+};
+struct StageDescsInline {
+  StageDesc d[kInlineStates];
+};
+
+// Stage argument + FP32 demag rows + LL right-hand side (launch_stage_demag_llg_f32).
+// Block = RB receiver rows of state blockIdx.y.  The whole stage vector goes to
+// LDS (3n FP64, then its FP32 rounding); each element sums bconst, bcoef*base
+// and the terms in the vector kernel's order, all of its operands loaded
+// before the sum (an index past nterm reloads operand 0, which is not used).
+template <bool VEC4, int RB>
+__global__ void __launch_bounds__(256) stage_demag_llg_f32_kernel(const float* __restrict__ K, const LlgGrid g,
+                                                                  StageDescsInline sd, LlgDescsInline inl) {
+  extern __shared__ double stage_lds[];
+  const int n = g.n, n3 = 3 * n;
+  const int s = blockIdx.y;
+  const int i0 = blockIdx.x * RB;
+  double* m64 = stage_lds;
+  float* m32 = reinterpret_cast<float*>(stage_lds + n3);
+  const StageDesc& S = sd.d[s];
+  const int nt = S.nterm;
+  for (int e = threadIdx.x; e < n3; e += blockDim.x) {
+    double x[kMaxTerms];
+    if (nt > 0) {
+#pragma unroll
+      for (int k = 0; k < kMaxTerms; ++k) x[k] = S.src[k < nt ? k : 0][e];
+    }
+    const double b = S.base ? S.base[e] : 0.0;
+    double v = S.bconst;
+    if (S.base) v += S.bcoef * b;
+#pragma unroll
+    for (int k = 0; k < kMaxTerms; ++k)
+      if (k < nt) v += S.coef[k] * x[k];
+    v = S.post_scale * v;
+    m64[e] = v;
+    m32[e] = (float)v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3 * RB) {  // this block's cells of the stage argument
+    const int c = threadIdx.x / RB, i = i0 + threadIdx.x % RB;
+    if (i < n) S.dst[(int64_t)c * n + i] = m64[c * n + i];
+  }
+  __shared__ float red[4][RB * 3];
+  __shared__ float sums[RB * 3];
+  matvec_rows_f32<RB, VEC4>(K, n, m32, i0, red, sums);
+  const int r = threadIdx.x;
+  if (r >= RB || i0 + r >= n) return;
+  LlgDesc D = inl.d[s];
+  D.m = m64;  // the stage argument from LDS (other blocks write its global copy)
+  llg_cell(g, D, i0 + r, [&](int c) {
+    float v = 0.0f;
+    v += sums[r * 3 + c];
+    return v;
+  });
+}
+
 __device__ inline double nan_to_inf(double v) { return isnan(v) ? INFINITY : v; }
 
 // Generic vector op.  A block covers 2048 consecutive elements of one state;
@@ -544,6 +608,47 @@ int launch_demag_llg_f32(const float* K, const LlgGrid& g, const LlgDesc* d_desc
   // four receiver rows per block (the 128 x 32 film's 402.7 MB tensor: 2 / 4 / 8
   // rows -> 72.2 / 66.8 / 114.4 us per launch)
   return launch_demag_llg_rb<4>(K, g, d_descs, inl, m32, m_stride, nstates, stream);
+}
+
+static_assert(sizeof(StageDescsInline) + sizeof(LlgDescsInline) + sizeof(LlgGrid) + 256 <= 4096,
+              "stage + demag + LL kernel arguments exceed 4 KiB");
+
+int launch_stage_demag_llg_f32(const float* K, const LlgGrid& g, const VecDesc* h_pre, const LlgDesc* h_descs,
+                               int nstates, hipStream_t stream) {
+  if (nstates < 1 || nstates > kInlineStates || g.n < 1 || g.n > kStageFusedMaxCells) {
+    set_error("stage + demag + LL kernel: 1..kInlineStates states, 1..kStageFusedMaxCells cells");
+    return kErrArg;
+  }
+  StageDescsInline sd{};
+  LlgDescsInline inl{};
+  for (int s = 0; s < nstates; ++s) {
+    const VecDesc& v = h_pre[s];
+    if (!v.dst || v.rkind != kRedNone || v.nterm < 0 || v.nterm > kMaxTerms) {
+      set_error("stage + demag + LL kernel: a stage combination with a destination and no reduction");
+      return kErrArg;
+    }
+    StageDesc& d = sd.d[s];
+    d.dst = v.dst;
+    d.base = v.base;
+    d.bcoef = v.bcoef;
+    d.bconst = v.bconst;
+    d.post_scale = v.post_scale;
+    d.nterm = v.nterm;
+    for (int k = 0; k < v.nterm; ++k) {
+      d.src[k] = v.src[k];
+      d.coef[k] = v.coef[k];
+    }
+    inl.d[s] = h_descs[s];
+  }
+  constexpr int RB = 4;
+  const dim3 grid((unsigned)((g.n + RB - 1) / RB), (unsigned)nstates);
+  const size_t lds = (size_t)3 * g.n * (sizeof(double) + sizeof(float));
+  if (g.n % 4 == 0)
+    hipLaunchKernelGGL((stage_demag_llg_f32_kernel<true, RB>), grid, dim3(256), lds, stream, K, g, sd, inl);
+  else
+    hipLaunchKernelGGL((stage_demag_llg_f32_kernel<false, RB>), grid, dim3(256), lds, stream, K, g, sd, inl);
+  MT_HIP(hipGetLastError());
+  return kOk;
 }
 
 int vec_blocks_per_state(int n3) { return (n3 + 256 * 8 - 1) / (256 * 8); }

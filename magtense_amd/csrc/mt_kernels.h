@@ -338,4 +338,16 @@ struct VecDescsInline {
 int launch_vec_inline(const VecDesc* h_descs, int nstates, int n, double* partials, double* results,
                       const PackSpec& pack, bool reduce, hipStream_t stream);
 
+// Small problems on the FP32 dense path: the stage argument formed in the
+// demag + LL launch.  Every workgroup evaluates the whole combination of
+// h_pre[s] (the vector kernel's arithmetic, in its order) into LDS, writes its
+// own cells of it to h_pre[s].dst, and runs the FP32 demag rows and the LL
+// right-hand side from LDS: one launch per right-hand side instead of two
+// (bit-identical to launch_vec_inline with an FP32 pack followed by
+// launch_demag_llg_f32).  Up to kInlineStates states, n <= kStageFusedMaxCells,
+// no reduction in h_pre.
+constexpr int kStageFusedMaxCells = 1024;
+int launch_stage_demag_llg_f32(const float* K, const LlgGrid& g, const VecDesc* h_pre, const LlgDesc* h_descs,
+                               int nstates, hipStream_t stream);
+
 }  // namespace mt

@@ -293,6 +293,7 @@ struct Solver {
   long long exch_nnz_host = 0;
   uint8_t* Ktab = nullptr;    // translation-invariant fragment table (bf16x2 on uniform grids)
   bool defer_early = false;   // read the early error estimate with stage 8's (cheap right-hand sides)
+  bool stage_fusion = true;   // small FP32 dense problems: stage argument formed in the demag + LL launch
   uint32_t* near_tab = nullptr;  // near-band correction table (bf16x2 on uniform grids, dense tensor)
   int near_r = 0;             // its radius (0: no correction)
   int demag_approx = 1;       // 1 dense, 2 threshold, 4 threshold fraction
@@ -1158,6 +1159,8 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     const bool inl = S <= kInlineStates && !(precision == MT_PREC_BF16X2 && near_r > 0);
     // the FP32 dense path runs its demag rows and the LL right-hand side in one kernel
     const bool fused = precision == MT_PREC_FP32 && demag_approx == MT_DEMAG_DENSE;
+    // and on small problems forms the stage argument in that kernel as well
+    const bool stage_fused = fused && inl && !pre_red && stage_fusion && n <= kStageFusedMaxCells;
     const VecDesc* dpre = nullptr;
     const LlgDesc* dl = nullptr;
     const VecDesc* dpost = nullptr;
@@ -1172,9 +1175,11 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       if (rc) return rc;
     }
     double* pre_parts = partials + (size_t)B * vec_blocks_per_state((int)N3) * 4;
-    rc = inl ? launch_vec_inline(pre.data(), S, n, pre_parts, res_pre, pk, pre_red, stream)
-             : launch_vec(dpre, S, n, pre_parts, res_pre, pk, pre_red, stream);
-    if (rc) return rc;
+    if (!stage_fused) {
+      rc = inl ? launch_vec_inline(pre.data(), S, n, pre_parts, res_pre, pk, pre_red, stream)
+               : launch_vec(dpre, S, n, pre_parts, res_pre, pk, pre_red, stream);
+      if (rc) return rc;
+    }
     if (timing) MT_HIP(hipEventRecord(ev[3], stream));
     if (precision == MT_PREC_BF16X2) {
       if (Ktab)
@@ -1197,6 +1202,8 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
                            (int64_t)hcols * N3, S, ksplit, stream);
     } else if (demag_approx != MT_DEMAG_DENSE) {
       rc = sparse.apply(packf, (int64_t)N3, S, 1.0f, hbuf, stream);
+    } else if (stage_fused) {
+      rc = launch_stage_demag_llg_f32(K32, grid, pre.data(), ld.data(), S, stream);
     } else {
       rc = launch_demag_llg_f32(K32, grid, dl, ld.data(), packf, (int64_t)N3, S, stream);
     }
@@ -1232,9 +1239,11 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       llg_ms += fused ? 0.0f : ms[2];
       llg_launches += fused ? 0 : 1;
       llg_bytes += fused ? 0.0 : llg_launch_bytes(ld);
-      vec_ms += ms[0] + (post_red ? ms[3] : 0.0f);
-      vec_launches += post_red ? 2 : 1;
-      for (const VecDesc& v : pre) vec_bytes += vec_desc_bytes(v, pk.mode);
+      // stage_fused: the stage combination ran inside the demag kernel (its time is in gemm_ms)
+      vec_ms += (stage_fused ? 0.0f : ms[0]) + (post_red ? ms[3] : 0.0f);
+      vec_launches += (stage_fused ? 0 : 1) + (post_red ? 1 : 0);
+      if (!stage_fused)
+        for (const VecDesc& v : pre) vec_bytes += vec_desc_bytes(v, pk.mode);
       if (post_red)
         for (const VecDesc& v : post) vec_bytes += vec_desc_bytes(v, kPackNone);
     }
@@ -1626,6 +1635,9 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
   {
     const char* env = std::getenv("MT_DEFER_EARLY");
     P.defer_early = env ? std::atoi(env) != 0 : (double)n * n * 24.0 < 2e9;
+    // MT_STAGE_FUSED=0: the stage argument in its own launch on small problems too
+    const char* env_sf = std::getenv("MT_STAGE_FUSED");
+    P.stage_fusion = env_sf ? std::atoi(env_sf) != 0 : true;
   }
   if (P.precision == MT_PREC_BF16X2) {
     // a block-sparse tensor's schedule fixes its partial-sum slots
