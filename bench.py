@@ -486,7 +486,7 @@ def next_rows(W, reps=5):
         "list_entries": int(ent.value), "stored_fraction_of_dense": 4.0 * ent.value / tiles_all,
         "fragment_bytes": float(nbytes.value), "ksplit": int(ksb.value), "ms_per_product": ms_apply_b,
         "rel_diff_vs_fp32_csr": rel_b,
-        "roofline": {"bound": "hbm", "kernel": "gemm_bf16x2_reg_kernel<4, false, true>",
+        "roofline": {"bound": "hbm", "kernel": "gemm_bf16x2_reg_kernel<4, false, true, false>",
                      "achieved": alg_b / (ms_apply_b * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg_b / (ms_apply_b * 1e-3) / 1e9 / HBM_PEAK_GBS, "alg_bytes_per_launch": alg_b},
         "note": "wall time per synchronous mt_demag_apply_sparse call (state packing, GEMM over the stored tiles, "
@@ -826,7 +826,7 @@ def main():
         near_ms = stb[0].near_ms / max(gemm_launches, 1)  # near-band correction after each GEMM
         gemm_ms = stb[0].demag_ms / max(gemm_launches, 1) - near_ms
         breakdown = step_breakdown(stb[0], gemm_launches, timed_steps)
-        breakdown["near_band"] = {"kernel": "near_corr_kernel<true> (radius 2)", "ms_per_launch": near_ms,
+        breakdown["near_band"] = {"kernel": "near_corr_kernel<true, 2> (radius 2)", "ms_per_launch": near_ms,
                                   "ms_per_batch_step": near_ms * gemm_launches / timed_steps}
         L.mt_solver_destroy(h)
         del keep
@@ -940,7 +940,7 @@ def main():
         achieved = alg_bytes / (gemm_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("dense"),
-                    "kernel": "gemm_bf16x2_reg_kernel<4, false>", "kernel_ms": gemm_ms,
+                    "kernel": "gemm_bf16x2_reg_kernel<4, false, false, false>", "kernel_ms": gemm_ms,
                     "alg_bytes_per_launch": alg_bytes, "measured_read_peak": rd.value,
                     "frac_of_measured_read_peak": achieved / rd.value,
                     "measured_read_peak_random_bits": rdr.value,
