@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <type_traits>
 
+#include "llg_cell.h"
 #include "matvec_rows.h"
 #include "mt_common.h"
 #include "mt_kernels.h"
@@ -1042,7 +1043,11 @@ namespace {
 // scalar load of the table (the same for the whole workgroup) and 15 BF16
 // pair dot products with FP32 sums.  F64: the state comes from the LL
 // descriptors (FP64 stage arguments, rounded to FP32 as the product's packing
-// rounds them), else from FP32 states m32 + s * m_stride.
+// rounds them), else from FP32 states m32 + s * m_stride.  LLG (with F64):
+// instead of storing the correction, each thread evaluates the LL right-hand
+// side of its cell with the demag sum the LL kernel would form, the GEMM's
+// partial-sum slots 0 .. nsplit-2 of the descriptor and then the correction
+// (the value the correction slot would hold, pref = 1), in that order.
 constexpr int kNearTX = 64, kNearTY = 4;
 typedef __bf16 near_bf16x2 __attribute__((ext_vector_type(2)));
 
@@ -1051,12 +1056,12 @@ __device__ inline float near_dot(uint32_t t, uint32_t v, float acc) {
                                          false);
 }
 
-template <bool F64, int R>
+template <bool F64, int R, bool LLG>
 __global__ void __launch_bounds__(256) near_corr_kernel(const uint32_t* __restrict__ table, int Rz, int nx,
                                                         int ny, int nz, const LlgDesc* __restrict__ descs,
                                                         const float* __restrict__ m32, int64_t m_stride,
                                                         float* __restrict__ out, int64_t col_stride, float pref,
-                                                        int accumulate) {
+                                                        int accumulate, const LlgGrid g) {
   constexpr int hx = kNearTX + 2 * R, hy = kNearTY + 2 * R, hc = hx * hy, w = 2 * R + 1;
   __shared__ uint2 near_lds[(hc * 20 + 7) / 8];
   uint2* p0 = near_lds;                                // (W_x, W_y)
@@ -1125,6 +1130,16 @@ __global__ void __launch_bounds__(256) near_corr_kernel(const uint32_t* __restri
   }
   if (ix >= nx || iy >= ny) return;
   const int64_t i = ix + (int64_t)iy * nx + (int64_t)iz * nx * ny;
+  if (LLG) {
+    const LlgDesc& D = descs[s];
+    llg_cell(g, D, (int)i, [&](int c) {
+      float v = 0.0f;
+      for (int k = 0; k + 1 < D.nsplit; ++k) v += D.hpart[(int64_t)k * D.split_stride + (int64_t)c * n + i];
+      v += acc[c];
+      return v;
+    });
+    return;
+  }
   float* o = out + (int64_t)s * col_stride + i;
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -1141,9 +1156,13 @@ int64_t near_table_words(int R, int nz) {
 
 int launch_near_correction(const uint32_t* table, int R, int nx, int ny, int nz, const LlgDesc* descs,
                            const float* m32, int64_t m_stride, int nstates, float* out, int64_t col_stride, float pref,
-                           bool accumulate, hipStream_t stream) {
+                           bool accumulate, hipStream_t stream, const LlgGrid* llg) {
   if (R < 1 || R > 4 || nx < 1 || ny < 1 || nz < 1 || nstates < 1 || nstates > 65535) {
     set_error("near-band correction: radius 1..4, a non-empty grid and 1..65535 states");
+    return kErrArg;
+  }
+  if (llg && (!descs || pref != 1.0f || accumulate || llg->n != nx * ny * nz)) {
+    set_error("near-band correction with the LL right-hand side: FP64 descriptors, pref 1, no accumulation");
     return kErrArg;
   }
   const int Rz = near_band_rz(R, nz);
@@ -1152,17 +1171,23 @@ int launch_near_correction(const uint32_t* table, int R, int nx, int ny, int nz,
   const int acc = accumulate ? 1 : 0;
   // the radius as a template argument: the offset loops unroll, the halo
   // index arithmetic and the table offsets are constants
-  auto go = [&](auto f64, auto r) {
-    hipLaunchKernelGGL((near_corr_kernel<decltype(f64)::value, decltype(r)::value>), grid, dim3(256), 0, stream,
-                       table, Rz, nx, ny, nz, descs, m32, m_stride, out, col_stride, pref, acc);
+  const LlgGrid g = llg ? *llg : LlgGrid{};
+  auto go = [&](auto mode, auto r) {
+    // mode: 0 FP32 states, 1 FP64 stage arguments, 2 those with the LL right-hand side
+    constexpr int M = decltype(mode)::value;
+    hipLaunchKernelGGL((near_corr_kernel<M != 0, decltype(r)::value, M == 2>), grid, dim3(256), 0, stream, table, Rz,
+                       nx, ny, nz, descs, m32, m_stride, out, col_stride, pref, acc, g);
   };
-  using T = std::true_type;
-  using F = std::false_type;
+  auto by_mode = [&](auto r) {
+    if (llg) go(std::integral_constant<int, 2>{}, r);
+    else if (descs) go(std::integral_constant<int, 1>{}, r);
+    else go(std::integral_constant<int, 0>{}, r);
+  };
   switch (R) {
-    case 1: descs ? go(T{}, std::integral_constant<int, 1>{}) : go(F{}, std::integral_constant<int, 1>{}); break;
-    case 2: descs ? go(T{}, std::integral_constant<int, 2>{}) : go(F{}, std::integral_constant<int, 2>{}); break;
-    case 3: descs ? go(T{}, std::integral_constant<int, 3>{}) : go(F{}, std::integral_constant<int, 3>{}); break;
-    default: descs ? go(T{}, std::integral_constant<int, 4>{}) : go(F{}, std::integral_constant<int, 4>{}); break;
+    case 1: by_mode(std::integral_constant<int, 1>{}); break;
+    case 2: by_mode(std::integral_constant<int, 2>{}); break;
+    case 3: by_mode(std::integral_constant<int, 3>{}); break;
+    default: by_mode(std::integral_constant<int, 4>{}); break;
   }
   MT_HIP(hipGetLastError());
   return kOk;

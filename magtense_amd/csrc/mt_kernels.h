@@ -261,9 +261,13 @@ int launch_near_table(int nx, int ny, int nz, double Lx, double Ly, double Lz, i
                       uint32_t* table, hipStream_t stream);
 // out[s*col_stride + a*n + i] (+)= pref * correction of state s; the state is
 // read from descs[s].m (FP64) when descs is set, else from m32 + s*m_stride.
+// llg (with descs, pref 1, no accumulation): the correction is not stored;
+// each cell's LL right-hand side is evaluated with it instead (the LL kernel's
+// arithmetic, the GEMM slots of descs[s] and then the correction, in the order
+// the LL kernel sums the slots), replacing launch_llg after the product.
 int launch_near_correction(const uint32_t* table, int R, int nx, int ny, int nz, const LlgDesc* descs,
                            const float* m32, int64_t m_stride, int nstates, float* out, int64_t col_stride, float pref,
-                           bool accumulate, hipStream_t stream);
+                           bool accumulate, hipStream_t stream, const LlgGrid* llg = nullptr);
 
 // ---- (d) vector operations -------------------------------------------------------
 constexpr int kMaxTerms = 12;

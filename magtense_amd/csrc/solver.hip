@@ -294,6 +294,7 @@ struct Solver {
   uint8_t* Ktab = nullptr;    // translation-invariant fragment table (bf16x2 on uniform grids)
   bool defer_early = false;   // read the early error estimate with stage 8's (cheap right-hand sides)
   bool stage_fusion = true;   // small FP32 dense problems: stage argument formed in the demag + LL launch
+  bool near_llg = true;       // split product: the LL right-hand side evaluated by the near-band kernel
   uint32_t* near_tab = nullptr;  // near-band correction table (bf16x2 on uniform grids, dense tensor)
   int near_r = 0;             // its radius (0: no correction)
   int demag_approx = 1;       // 1 dense, 2 threshold, 4 threshold fraction
@@ -1161,6 +1162,8 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     const bool fused = precision == MT_PREC_FP32 && demag_approx == MT_DEMAG_DENSE;
     // and on small problems forms the stage argument in that kernel as well
     const bool stage_fused = fused && inl && !pre_red && stage_fusion && n <= kStageFusedMaxCells;
+    // the split product's near-band kernel evaluates the LL right-hand side
+    const bool llg_in_near = precision == MT_PREC_BF16X2 && near_r > 0 && near_llg;
     const VecDesc* dpre = nullptr;
     const LlgDesc* dl = nullptr;
     const VecDesc* dpost = nullptr;
@@ -1194,7 +1197,8 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
       if (rc == kOk && near_r > 0 && timing) MT_HIP(hipEventRecord(ev[7], stream));
       if (rc == kOk && near_r > 0)
         rc = launch_near_correction(near_tab, near_r, nx, ny, nz, dl, nullptr, 0, S,
-                                    hbuf + (size_t)ksplit * hcols * N3, (int64_t)N3, 1.0f, false, stream);
+                                    hbuf + (size_t)ksplit * hcols * N3, (int64_t)N3, 1.0f, false, stream,
+                                    llg_in_near ? &grid : nullptr);
     } else if (Kf32) {
       rc = launch_gemm_f32(Kf32, packb, n, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit, stream);
     } else if (sparse.blocks) {
@@ -1210,7 +1214,7 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
     if (rc) return rc;
     if (timing) MT_HIP(hipEventRecord(ev[4], stream));
     demag_passes += 1;
-    if (!fused) {
+    if (!fused && !llg_in_near) {
       rc = inl ? launch_llg_inline(grid, ld.data(), S, stream) : launch_llg(grid, dl, S, stream);
       if (rc) return rc;
     }
@@ -1235,10 +1239,12 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
         MT_HIP(hipEventElapsedTime(&mn, ev[7], ev[4]));
         near_ms += mn;
       }
-      // fused: the LL right-hand side ran inside the demag kernel (its time is in gemm_ms)
-      llg_ms += fused ? 0.0f : ms[2];
-      llg_launches += fused ? 0 : 1;
-      llg_bytes += fused ? 0.0 : llg_launch_bytes(ld);
+      // fused / llg_in_near: the LL right-hand side ran inside the demag or the
+      // near-band kernel (its time is in gemm_ms, and in near_ms for the latter)
+      const bool llg_own = !fused && !llg_in_near;
+      llg_ms += llg_own ? ms[2] : 0.0f;
+      llg_launches += llg_own ? 1 : 0;
+      llg_bytes += llg_own ? llg_launch_bytes(ld) : 0.0;
       // stage_fused: the stage combination ran inside the demag kernel (its time is in gemm_ms)
       vec_ms += (stage_fused ? 0.0f : ms[0]) + (post_red ? ms[3] : 0.0f);
       vec_launches += (stage_fused ? 0 : 1) + (post_red ? 1 : 0);
@@ -1638,6 +1644,9 @@ int mt_solver_create(const mt_micromag_problem* q, mt_solver** out) {
     // MT_STAGE_FUSED=0: the stage argument in its own launch on small problems too
     const char* env_sf = std::getenv("MT_STAGE_FUSED");
     P.stage_fusion = env_sf ? std::atoi(env_sf) != 0 : true;
+    // MT_NEAR_LLG=0: the LL kernel after the near-band correction, as its own launch
+    const char* env_nl = std::getenv("MT_NEAR_LLG");
+    P.near_llg = env_nl ? std::atoi(env_nl) != 0 : true;
   }
   if (P.precision == MT_PREC_BF16X2) {
     // a block-sparse tensor's schedule fixes its partial-sum slots
