@@ -312,7 +312,9 @@ typedef struct {
   double demag_ms;
   double vec_ms, llg_ms, vec_bytes, llg_bytes;
   long long vec_launches, llg_launches;
-  double near_ms;  /* of those demag milliseconds, the near-band correction's (uniform grids, bf16x2) */
+  /* of those demag milliseconds, the near-band correction's (uniform grids, bf16x2),
+   * which includes the LL right-hand side that kernel evaluates (llg_ms then 0) */
+  double near_ms;
 } mt_stats;
 
 typedef struct mt_solver mt_solver;
