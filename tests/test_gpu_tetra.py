@@ -107,6 +107,7 @@ def test_tetra_grid_matches_oracle(oracle, tmp_path):
     path = str(tmp_path / "K_tetra.bin")
     t, M32, p32, pts_out = run("fp32", path)
     _, Mbf, _, _ = run("bf16x2")
+    _, Mmf, _, _ = run("fp32_mfma")  # the dense tensor as FP32 MFMA fragments
     assert np.array_equal(pts_out, cent)
     K = micromag.load_demag_tensor(path, n)
     ref = np.zeros((6, n, n))
@@ -133,7 +134,9 @@ def test_tetra_grid_matches_oracle(oracle, tmp_path):
     orc = np.transpose(y[0], (0, 2, 1))
     d_or = np.abs(M32 - orc).max()
     d_bf = np.abs(Mbf - M32).max()
+    d_mf = np.abs(Mmf - M32).max()
     print(f"tetra grid: tensor max err/bar {(err / bar).max():.2f}; GPU vs oracle {d_or:.2e}; bf16x2 vs fp32 "
-          f"{d_bf:.2e}; gpu {p32.last_stats[0]}; oracle {st}")
+          f"{d_bf:.2e}, fp32_mfma vs fp32 {d_mf:.2e}; gpu {p32.last_stats[0]}; oracle {st}")
     assert p32.last_stats[0]["flag3"] == 0 and st["flag3"] == 0
     assert d_or < 1e-4 and d_bf < 1e-4
+    assert d_mf < 1e-5  # FP32 products and sums in another order than the per-state rows

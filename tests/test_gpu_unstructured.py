@@ -149,6 +149,7 @@ def test_graded_film_matches_oracle(oracle, tmp_path, nave):
     path = str(tmp_path / "K_graded.bin")
     t, M32, p32 = run("fp32", path)
     _, Mbf, _ = run("bf16x2")
+    _, Mmf, _ = run("fp32_mfma")  # the dense tensor as FP32 MFMA fragments (grid-agnostic)
     # the tensor the solver assembled (written to the tensor file) vs the oracle's FP64 average
     K = micromag.load_demag_tensor(path, n).reshape(6, n, n)
     ref = oracle.tensor_unstructured(pts, abc, nave)
@@ -171,8 +172,10 @@ def test_graded_film_matches_oracle(oracle, tmp_path, nave):
     orc = np.transpose(y[0], (0, 2, 1))
     d_or = np.abs(M32 - orc).max()
     d_bf = np.abs(Mbf - M32).max()
+    d_mf = np.abs(Mmf - M32).max()
     print(f"graded film N_ave {nave}: tensor {np.mean(ulps <= 2):.4f} within 2 ulp, max err/bar {(err / bar).max():.2f}; GPU vs oracle {d_or:.2e}; bf16x2 vs fp32 "
-          f"{d_bf:.2e}; gpu {p32.last_stats[0]}; oracle {st}")
+          f"{d_bf:.2e}, fp32_mfma vs fp32 {d_mf:.2e}; gpu {p32.last_stats[0]}; oracle {st}")
     assert p32.last_stats[0]["flag3"] == 0 and st["flag3"] == 0
     assert d_or < 1e-4 and d_bf < 1e-4
+    assert d_mf < 1e-5  # FP32 products and sums in another order than the per-state rows
     assert np.abs(np.linalg.norm(M32, axis=2) - 1).max() < 1e-4
