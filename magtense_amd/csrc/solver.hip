@@ -1193,7 +1193,8 @@ int Solver::execute_tick(std::vector<Sim*>& evals, std::vector<Sim*>& vecs) {
                                     (int64_t)hcols * N3, S, ksplit, stream);
       else
         rc = launch_gemm_bf16x2(Ksplit, packb, n, hbuf, (int64_t)N3, (int64_t)hcols * N3, S, ksplit, stream);
-      // the near band's FP32 correction goes into one more partial-sum slot
+      // the near band's FP32 correction: straight into each cell's LL right-hand
+      // side (llg_in_near), or into one more partial-sum slot for the LL kernel
       if (rc == kOk && near_r > 0 && timing) MT_HIP(hipEventRecord(ev[7], stream));
       if (rc == kOk && near_r > 0)
         rc = launch_near_correction(near_tab, near_r, nx, ny, nz, dl, nullptr, 0, S,
